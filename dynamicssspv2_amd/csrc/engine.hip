@@ -1,0 +1,967 @@
+// engine.hip -- device objects and the update pipeline behind the C ABI.
+//
+// Update pipeline (one HIP stream per graph, no host round-trip per round):
+//   1. insert   k_ins_count / k_ins_alloc / k_ins_scatter / k_ins_reset:
+//               the batch's insertions become delta-row entries in HBM
+//               (the reference appends them to its host adjacency before upload,
+//               all_structure_undir.cuh:106-117)
+//   2. classify k_classify: tree-edge test + tombstones, roots of invalidated
+//               subtrees queued (deleteEdge, gpuFunctions_undir.cuh:179-215)
+//   3. invalidate rounds of k_frontier<M_INVAL> over the queue, descending the
+//               pre-update tree (updateNeighbors_del + filterAffectedNodes loop,
+//               SSSPmainForUndirected.cu:219-239, gpuFunctions_undir.cuh:125-144,312-349)
+//   4. seed     k_frontier<M_SEED> over all invalidated vertices + k_ins_offer
+//               (insertEdge, gpuFunctions_undir.cuh:217-251)
+//   5. relax    rounds of k_frontier<M_RELAX> until the frontier is empty
+//               (updateNeighbors + filterAllAffected loop, :162-174,370-433,
+//               SSSPmainForUndirected.cu:263-281)
+//   6. repair   only for graphs that ever held a zero weight: tight-graph BFS
+//               levels and parent re-pick so the tree stays acyclic
+// Rounds are launched in groups; the host reads one 8-byte counter per group to
+// decide whether to continue (the reference syncs and reads UVM every round).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cinttypes>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/dsssp.h"
+#include "common.h"
+#include "host.h"
+#include "kernels.h"
+
+using namespace dsssp;
+
+namespace {
+
+dsssp_tuning g_tuning = {FRONTIER_BLOCK, 2048, 1024, 1024, 4, 0};
+std::mutex g_tuning_mu;
+thread_local int t_device = 0;
+
+dsssp_tuning tuning()
+{
+    std::lock_guard<std::mutex> lk(g_tuning_mu);
+    return g_tuning;
+}
+
+#define HIP_TRY(expr)                                                                                    \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess) return fail(DSSSP_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+int dev_alloc(void **p, size_t bytes, const char *what)
+{
+    *p = nullptr;
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(DSSSP_ERR_NOMEM, "hipMalloc of %zu bytes for %s failed: %s", bytes, what, hipGetErrorString(e));
+    }
+    return DSSSP_OK;
+}
+
+template <class T>
+void dev_free(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+int check_device(int dev)
+{
+    static std::mutex mu;
+    static std::vector<int> ok_devs;
+    std::lock_guard<std::mutex> lk(mu);
+    if (std::find(ok_devs.begin(), ok_devs.end(), dev) != ok_devs.end()) return DSSSP_OK;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        return fail(DSSSP_ERR_NODEV, "no HIP device visible");
+    }
+    if (dev < 0 || dev >= count) return fail(DSSSP_ERR_NODEV, "device %d not present (%d visible)", dev, count);
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(DSSSP_ERR_NODEV, "device %d is %s; this library is built for gfx950 only", dev, prop.gcnArchName);
+    ok_devs.push_back(dev);
+    return DSSSP_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Objects.
+// ---------------------------------------------------------------------------
+
+struct dsssp_graph {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t n = 0;
+    uint64_t E = 0;  // base entries
+    uint64_t *rp = nullptr, *adj = nullptr, *dbl = nullptr, *pool = nullptr;
+    uint64_t pool_cap = 0;
+    uint64_t pool_top = 0;       // host copy, valid between updates
+    uint64_t delta_entries = 0;  // sum of delta row lengths
+    uint32_t *tcnt = nullptr;
+    GraphCtl *gc = nullptr;
+    GraphCtl *gc_host = nullptr;  // pinned
+    bool zero_w = false;          // some weight was ever 0: parents need the tight-BFS repair
+    hipEvent_t ev[8] = {};
+    std::vector<hipEvent_t> kev;  // per-launch timing events (kernel_timing)
+
+    GView view() const { return GView{rp, adj, dbl, pool}; }
+};
+
+struct dsssp_tree {
+    dsssp_graph *g = nullptr;
+    int32_t src = -1;
+    uint64_t *st = nullptr;
+    uint32_t *stamp = nullptr;
+    uint32_t *q[2] = {nullptr, nullptr};
+    uint64_t *hq[2] = {nullptr, nullptr};
+    uint32_t hcap = 0;
+    uint32_t *invl = nullptr;
+    uint32_t *lvl = nullptr, *best = nullptr;
+    Counters *c = nullptr;
+    Counters *c_host = nullptr;  // pinned
+    uint32_t epoch = 1;
+};
+
+struct dsssp_batch {
+    dsssp_graph *g = nullptr;
+    int64_t k = 0, n_ins = 0, n_del = 0;
+    bool zero_w = false;
+    int4 *rec = nullptr;
+    uint64_t *slot = nullptr, *pos = nullptr;
+    uint32_t *touched = nullptr;
+};
+
+namespace {
+
+struct DevGuard {
+    explicit DevGuard(int d) { (void)hipSetDevice(d); }
+};
+
+// Ring of frontier buffers: counters cycle mod 3, queues mod 2.
+struct Ring {
+    int cs = 0, qs = 0;
+};
+
+FArgs base_args(dsssp_tree *t)
+{
+    const dsssp_tuning tu = tuning();
+    FArgs a{};
+    a.g = t->g->view();
+    a.st = t->st;
+    a.stamp = t->stamp;
+    a.lvl = t->lvl;
+    a.best = t->best;
+    a.out_cap = (uint32_t)t->g->n;
+    a.out_hcap = t->hcap;
+    a.inv_total = &t->c->inv_total;
+    a.invl = t->invl;
+    a.c = t->c;
+    a.heavy = (uint32_t)tu.heavy_degree;
+    a.chunk = (uint32_t)tu.chunk;
+    return a;
+}
+
+uint32_t next_epoch(dsssp_tree *t)
+{
+    if (t->epoch >= 0xF0000000u) {  // stamps restart well before the counter wraps
+        launch_fill_u32(t->stamp, 0u, (uint64_t)t->g->n, t->g->stream);
+        t->epoch = 1;
+    }
+    return ++t->epoch;
+}
+
+void zero_ring(dsssp_tree *t, Ring &r)
+{
+    launch_fill_u32(&t->c->ring[0][0], 0u, 6, t->g->stream);
+    r.cs = 0;
+    r.qs = 0;
+}
+
+// Producer stage outputs go to ring slot cs+1 / queue qs^1.
+void producer_out(dsssp_tree *t, const Ring &r, uint32_t *&q, uint32_t *&cnt)
+{
+    q = t->q[r.qs ^ 1];
+    cnt = &t->c->ring[(r.cs + 1) % 3][0];
+}
+void advance(Ring &r)
+{
+    r.cs = (r.cs + 1) % 3;
+    r.qs ^= 1;
+}
+
+// Launches frontier rounds until the current ring slot is empty.
+int run_rounds(dsssp_tree *t, int mode, Ring &r, int64_t &rounds, double *kernel_ms)
+{
+    dsssp_graph *g = t->g;
+    const dsssp_tuning tu = tuning();
+    int group = std::max(1, tu.round_batch);
+    const int max_group = 64;
+    FArgs base = base_args(t);
+    const int grid = std::max(1, tu.grid);
+    const bool timed = tu.kernel_timing && kernel_ms;
+    const int64_t start_rounds = rounds;
+    for (;;) {
+        if (timed && g->kev.size() < (size_t)(2 * group)) {
+            size_t old = g->kev.size();
+            g->kev.resize((size_t)(2 * group));
+            for (size_t i = old; i < g->kev.size(); ++i) HIP_TRY(hipEventCreate(&g->kev[i]));
+        }
+        for (int i = 0; i < group; ++i) {
+            FArgs a = base;
+            a.in_q = t->q[r.qs];
+            a.in_cnt = &t->c->ring[r.cs][0];
+            a.in_hq = t->hq[r.qs];
+            a.in_hcnt = &t->c->ring[r.cs][1];
+            a.out_q = t->q[r.qs ^ 1];
+            a.out_cnt = &t->c->ring[(r.cs + 1) % 3][0];
+            a.out_hq = t->hq[r.qs ^ 1];
+            a.out_hcnt = &t->c->ring[(r.cs + 1) % 3][1];
+            a.zero_cnt = &t->c->ring[(r.cs + 2) % 3][0];
+            a.epoch = next_epoch(t);
+            if (timed) HIP_TRY(hipEventRecord(g->kev[2 * i], g->stream));
+            launch_frontier(mode, a, grid, g->stream);
+            if (timed) HIP_TRY(hipEventRecord(g->kev[2 * i + 1], g->stream));
+            advance(r);
+            ++rounds;
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(t->c_host->ring[r.cs], t->c->ring[r.cs], 2 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        if (timed) {
+            for (int i = 0; i < group; ++i) {
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, g->kev[2 * i], g->kev[2 * i + 1]));
+                *kernel_ms += ms;
+            }
+        }
+        if (t->c_host->ring[r.cs][0] == 0 && t->c_host->ring[r.cs][1] == 0) break;
+        // Every round settles at least one more hop of every shortest path, so
+        // n + 64 rounds can only be exceeded by an internal error.
+        if (rounds - start_rounds > g->n + 64)
+            return fail(DSSSP_ERR_STATE, "frontier did not empty after %lld rounds", (long long)(rounds - start_rounds));
+        group = std::min(group * 2, max_group);
+    }
+    return DSSSP_OK;
+}
+
+int ensure_pick_buffers(dsssp_tree *t)
+{
+    if (!t->lvl) {
+        if (int s = dev_alloc((void **)&t->lvl, sizeof(uint32_t) * (size_t)t->g->n, "levels")) return s;
+    }
+    if (!t->best) {
+        if (int s = dev_alloc((void **)&t->best, sizeof(uint32_t) * (size_t)t->g->n, "parent picks")) return s;
+    }
+    return DSSSP_OK;
+}
+
+// Recomputes every parent: smallest-id tight neighbour (use_lvl = 0), or the
+// smallest-id tight neighbour one BFS level closer to a root in the tight graph
+// (use_lvl = 1), which stays acyclic with zero weights.
+int repick_parents(dsssp_tree *t, bool use_lvl, int64_t &rounds)
+{
+    dsssp_graph *g = t->g;
+    if (int s = ensure_pick_buffers(t)) return s;
+    const uint32_t n = (uint32_t)g->n;
+    const dsssp_tuning tu = tuning();
+    if (use_lvl) {
+        Ring r;
+        zero_ring(t, r);
+        uint32_t *oq, *oc;
+        producer_out(t, r, oq, oc);
+        launch_lvl_init(n, t->st, t->lvl, t->best, t->stamp, next_epoch(t), oq, oc, n, t->c, g->stream);
+        advance(r);
+        if (int s = run_rounds(t, M_TBFS, r, rounds, nullptr)) return s;
+    } else {
+        launch_fill_u32(t->best, 0xFFFFFFFFu, n, g->stream);
+    }
+    FArgs a = base_args(t);
+    a.lvl = t->lvl;
+    a.best = t->best;
+    a.in_q = nullptr;
+    a.in_cnt = nullptr;
+    a.in_n = n;
+    a.use_lvl = use_lvl ? 1u : 0u;
+    launch_frontier(M_PICK, a, std::max(1, tu.grid), g->stream);
+    launch_apply_best(n, t->st, t->best, t->c, g->stream);
+    HIP_TRY(hipGetLastError());
+    return DSSSP_OK;
+}
+
+int ensure_hq(dsssp_tree *t, uint64_t total_entries)
+{
+    const dsssp_tuning tu = tuning();
+    const uint64_t need = 2 * total_entries / (uint64_t)std::max(64, tu.chunk) + 1024;
+    if (need > 0xFFFFFFFFull) return fail(DSSSP_ERR_ARG, "heavy queue would exceed 2^32 entries; raise chunk");
+    if (t->hcap >= need) return DSSSP_OK;
+    dev_free(t->hq[0]);
+    dev_free(t->hq[1]);
+    t->hcap = 0;
+    const uint64_t cap = need + need / 4;
+    if (int s = dev_alloc((void **)&t->hq[0], sizeof(uint64_t) * cap, "heavy queue")) return s;
+    if (int s = dev_alloc((void **)&t->hq[1], sizeof(uint64_t) * cap, "heavy queue")) return s;
+    t->hcap = (uint32_t)std::min<uint64_t>(cap, 0xFFFFFFFFull);
+    return DSSSP_OK;
+}
+
+// Makes room in the delta pool for `add` new entries: compacts the pool (live
+// blocks only) and grows it when needed.
+int ensure_pool(dsssp_graph *g, uint64_t add)
+{
+    if (g->pool_top + g->delta_entries + add <= g->pool_cap) return DSSSP_OK;
+    uint64_t cap = std::max<uint64_t>(g->pool_cap, 2 * (g->delta_entries + add) + (1u << 20));
+    if (cap > 0xFFFFFFFFull) cap = 0xFFFFFFFFull;
+    if (g->delta_entries + add > cap) return fail(DSSSP_ERR_NOMEM, "delta pool would exceed 2^32 entries");
+    // Compact on the host: rare, and keeps the kernels free of pool bookkeeping.
+    std::vector<uint64_t> dbl((size_t)g->n), pool((size_t)g->pool_top), out;
+    HIP_TRY(hipMemcpyAsync(dbl.data(), g->dbl, sizeof(uint64_t) * (size_t)g->n, hipMemcpyDeviceToHost, g->stream));
+    if (g->pool_top)
+        HIP_TRY(hipMemcpyAsync(pool.data(), g->pool, sizeof(uint64_t) * (size_t)g->pool_top, hipMemcpyDeviceToHost,
+                               g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    out.reserve((size_t)g->delta_entries);
+    for (int64_t v = 0; v < g->n; ++v) {
+        const uint64_t d = dbl[v];
+        const uint64_t beg = d >> 32, len = (uint32_t)d;
+        if (!len) continue;
+        const uint64_t nb = out.size();
+        out.insert(out.end(), pool.begin() + (int64_t)beg, pool.begin() + (int64_t)(beg + len));
+        dbl[v] = (nb << 32) | len;
+    }
+    uint64_t *np = nullptr;
+    if (int s = dev_alloc((void **)&np, sizeof(uint64_t) * cap, "delta pool")) return s;
+    if (!out.empty())
+        HIP_TRY(hipMemcpyAsync(np, out.data(), sizeof(uint64_t) * out.size(), hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipMemcpyAsync(g->dbl, dbl.data(), sizeof(uint64_t) * (size_t)g->n, hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    dev_free(g->pool);
+    g->pool = np;
+    g->pool_cap = cap;
+    g->pool_top = out.size();
+    g->gc_host->pool_top = g->pool_top;
+    HIP_TRY(hipMemcpy(&g->gc->pool_top, &g->pool_top, sizeof(uint64_t), hipMemcpyHostToDevice));
+    return DSSSP_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return ms;
+}
+
+int finish_errors(dsssp_tree *t, uint32_t gerr)
+{
+    const uint32_t err = t->c_host->err | gerr;
+    if (err & ERR_DIST_OVERFLOW) return fail(DSSSP_ERR_OVERFLOW, "a distance reached 2^32-1; the tree is not exact");
+    if (err & ERR_POOL) return fail(DSSSP_ERR_STATE, "delta pool overflow (internal)");
+    if (err & ERR_QUEUE) return fail(DSSSP_ERR_STATE, "frontier queue overflow (internal)");
+    if (err & ERR_NO_TIGHT) return fail(DSSSP_ERR_TREE, "a reachable vertex has no tight in-edge: distances not exact");
+    return DSSSP_OK;
+}
+
+int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stats, double h2d_ms)
+{
+    DevGuard dg(g->device);
+    hipStream_t s = g->stream;
+    const uint32_t n = (uint32_t)g->n;
+    if (b->n_ins > 0) {
+        if (int e = ensure_pool(g, 2 * (uint64_t)b->n_ins)) return e;
+    }
+    if (int e = ensure_hq(t, g->E + g->delta_entries + 2 * (uint64_t)b->n_ins)) return e;
+    const dsssp_tuning tu = tuning();
+    int64_t inv_rounds = 0, relax_rounds = 0, repair_rounds = 0;
+    double relax_kernel_ms = 0;
+
+    HIP_TRY(hipEventRecord(g->ev[0], s));
+    launch_reset_counters(t->c, s);
+    // 1. insertions into the resident CSR
+    if (b->n_ins > 0) {
+        const uint32_t max_touched = (uint32_t)std::min<uint64_t>((uint64_t)n, 2 * (uint64_t)b->n_ins);
+        launch_reset_graphctl(g->gc, s);
+        launch_ins_count(b->rec, b->k, g->tcnt, b->slot, b->touched, g->gc, s);
+        launch_ins_alloc(b->touched, max_touched, g->dbl, g->tcnt, g->pool, g->pool_cap, g->gc, s);
+        launch_ins_scatter(b->rec, b->k, b->slot, g->dbl, g->tcnt, g->pool, b->pos, s);
+        launch_ins_reset(b->touched, max_touched, g->tcnt, g->gc, s);
+        if (b->zero_w) g->zero_w = true;
+    }
+    HIP_TRY(hipEventRecord(g->ev[1], s));
+    // 2-3. deletions: classify, then invalidate the subtrees
+    Ring r;
+    zero_ring(t, r);
+    if (b->n_del > 0) {
+        uint32_t *oq, *oc;
+        producer_out(t, r, oq, oc);
+        launch_classify(b->rec, b->k, g->view(), t->st, oq, oc, t->invl, &t->c->inv_total, n, t->c, s);
+        advance(r);
+    }
+    HIP_TRY(hipEventRecord(g->ev[2], s));
+    if (b->n_del > 0) {
+        if (int e = run_rounds(t, M_INVAL, r, inv_rounds, nullptr)) return e;
+    }
+    HIP_TRY(hipEventRecord(g->ev[3], s));
+    // 4. seed invalidated vertices from valid neighbours, offer inserted edges
+    zero_ring(t, r);
+    {
+        uint32_t *oq, *oc;
+        producer_out(t, r, oq, oc);
+        const uint32_t ep = next_epoch(t);
+        if (b->n_del > 0) {
+            FArgs a = base_args(t);
+            a.in_q = t->invl;
+            a.in_cnt = &t->c->inv_total;
+            a.out_q = oq;
+            a.out_cnt = oc;
+            a.epoch = ep;
+            launch_frontier(M_SEED, a, std::max(1, tu.grid), s);
+        }
+        if (b->n_ins > 0) launch_ins_offer(b->rec, b->k, b->pos, g->pool, t->st, t->stamp, ep, oq, oc, n, t->c, s);
+        advance(r);
+    }
+    HIP_TRY(hipEventRecord(g->ev[4], s));
+    // 5. relaxation
+    if (int e = run_rounds(t, M_RELAX, r, relax_rounds, &relax_kernel_ms)) return e;
+    HIP_TRY(hipEventRecord(g->ev[5], s));
+    // 6. zero-weight parent repair
+    if (g->zero_w) {
+        if (int e = repick_parents(t, true, repair_rounds)) return e;
+    }
+    HIP_TRY(hipEventRecord(g->ev[6], s));
+    HIP_TRY(hipMemcpyAsync(t->c_host, t->c, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(g->gc_host, g->gc, sizeof(GraphCtl), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipGetLastError());
+    if (b->n_ins > 0) {
+        g->pool_top = g->gc_host->pool_top;
+        g->delta_entries += 2 * (uint64_t)b->n_ins;
+    }
+    const int st = finish_errors(t, b->n_ins > 0 ? g->gc_host->err : 0u);
+    if (stats) {
+        const Counters &c = *t->c_host;
+        dsssp_stats S{};
+        S.n_insert = b->n_ins;
+        S.n_delete = b->n_del;
+        S.n_invalidated = c.inv_total;
+        S.inv_rounds = inv_rounds;
+        S.relax_rounds = relax_rounds;
+        S.relax_launches = relax_rounds;
+        S.frontier_vertices = (int64_t)c.relax_frontier;
+        S.relax_edges = (int64_t)c.relax_edges;
+        S.relax_improved = (int64_t)c.relax_improved;
+        S.relax_enqueued = (int64_t)c.relax_enq;
+        S.other_edges = (int64_t)c.other_edges;
+        S.probes = (int64_t)c.probes;
+        S.t_h2d_ms = h2d_ms;
+        S.t_insert_ms = elapsed(g->ev[0], g->ev[1]);
+        S.t_classify_ms = elapsed(g->ev[1], g->ev[2]);
+        S.t_invalidate_ms = elapsed(g->ev[2], g->ev[3]);
+        S.t_seed_ms = elapsed(g->ev[3], g->ev[4]);
+        S.t_relax_ms = elapsed(g->ev[4], g->ev[5]) + elapsed(g->ev[5], g->ev[6]);
+        S.t_update_ms = elapsed(g->ev[0], g->ev[6]);
+        S.t_relax_kernel_ms = relax_kernel_ms;
+        // Algorithmic bytes: per frontier vertex 4 B id + 16 B row bounds +
+        // 8 B delta descriptor + 8 B own state; per scanned entry 8 B entry +
+        // 8 B neighbour state; per improvement 8 B; per enqueue 4 B stamp + 4 B id.
+        const double relax = 36.0 * c.relax_frontier + 16.0 * c.relax_edges + 8.0 * c.relax_improved + 8.0 * c.relax_enq;
+        const double other = 16.0 * c.other_edges + 2 * 36.0 * c.inv_total + 16.0 * c.inv_total +
+                             8.0 * c.probes + 32.0 * b->n_del + 72.0 * b->n_ins;
+        S.alg_bytes_relax = relax;
+        S.alg_bytes = relax + other;
+        *stats = S;
+    }
+    return st;
+}
+
+int validate_changes(int64_t n, const dsssp_change *chg, int64_t k, int64_t &ni, int64_t &nd, bool &zero_w)
+{
+    ni = nd = 0;
+    zero_w = false;
+    for (int64_t i = 0; i < k; ++i) {
+        const dsssp_change &c = chg[i];
+        if (c.op != 0 && c.op != 1) continue;  // ignored, as readin_changes does (all_structure_undir.cuh:106,119)
+        if (c.a < 0 || c.a >= n || c.b < 0 || c.b >= n)
+            return fail(DSSSP_ERR_RANGE, "change %lld (%d %d %d %d): vertex outside [0,%lld)", (long long)i, c.a, c.b,
+                        c.w, c.op, (long long)n);
+        if (c.w < 0)
+            return fail(DSSSP_ERR_WEIGHT, "change %lld (%d %d %d %d): negative weight", (long long)i, c.a, c.b, c.w,
+                        c.op);
+        if (c.op == 1) {
+            ++ni;
+            if (c.w == 0) zero_w = true;
+        } else {
+            ++nd;
+        }
+    }
+    return DSSSP_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI.
+// ---------------------------------------------------------------------------
+
+extern "C" const char *dsssp_last_error(void) { return last_error(); }
+extern "C" const char *dsssp_version(void) { return "dsssp-mi355x 0.1 (gfx950)"; }
+
+extern "C" int dsssp_set_device(int device)
+{
+    clear_error();
+    if (int s = check_device(device)) return s;
+    t_device = device;
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_get_tuning(dsssp_tuning *t)
+{
+    clear_error();
+    if (!t) return fail(DSSSP_ERR_ARG, "null argument");
+    *t = tuning();
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_set_tuning(const dsssp_tuning *t)
+{
+    clear_error();
+    if (!t) return fail(DSSSP_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(g_tuning_mu);
+    dsssp_tuning nt = g_tuning;
+    if (t->block && t->block != FRONTIER_BLOCK) return fail(DSSSP_ERR_ARG, "block must be %d", FRONTIER_BLOCK);
+    if (t->grid) nt.grid = t->grid;
+    if (t->heavy_degree) nt.heavy_degree = t->heavy_degree;
+    if (t->chunk) nt.chunk = t->chunk;
+    if (t->round_batch) nt.round_batch = t->round_batch;
+    nt.kernel_timing = t->kernel_timing;
+    if (nt.grid < 1 || nt.grid > 65536 || nt.heavy_degree < 64 || nt.chunk < 64 || nt.chunk % 64 ||
+        nt.round_batch < 1 || nt.round_batch > 64)
+        return fail(DSSSP_ERR_ARG, "tuning out of range");
+    g_tuning = nt;
+    return DSSSP_OK;
+}
+
+static void graph_release(dsssp_graph *g)
+{
+    if (!g) return;
+    DevGuard dg(g->device);
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    dev_free(g->rp);
+    dev_free(g->adj);
+    dev_free(g->dbl);
+    dev_free(g->pool);
+    dev_free(g->tcnt);
+    dev_free(g->gc);
+    if (g->gc_host) (void)hipHostFree(g->gc_host);
+    for (auto &e : g->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : g->kev) (void)hipEventDestroy(e);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+}
+
+static int graph_alloc(dsssp_graph *g, uint64_t pool_cap)
+{
+    DevGuard dg(g->device);
+    HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    for (auto &e : g->ev) HIP_TRY(hipEventCreate(&e));
+    const size_t n = (size_t)g->n;
+    if (int s = dev_alloc((void **)&g->rp, sizeof(uint64_t) * (n + 1), "row offsets")) return s;
+    if (int s = dev_alloc((void **)&g->adj, sizeof(uint64_t) * g->E, "adjacency")) return s;
+    if (int s = dev_alloc((void **)&g->dbl, sizeof(uint64_t) * n, "delta rows")) return s;
+    if (int s = dev_alloc((void **)&g->tcnt, sizeof(uint32_t) * n, "insert counters")) return s;
+    if (int s = dev_alloc((void **)&g->gc, sizeof(GraphCtl), "graph control")) return s;
+    g->pool_cap = pool_cap;
+    if (int s = dev_alloc((void **)&g->pool, sizeof(uint64_t) * pool_cap, "delta pool")) return s;
+    HIP_TRY(hipHostMalloc((void **)&g->gc_host, sizeof(GraphCtl), hipHostMallocDefault));
+    memset(g->gc_host, 0, sizeof(GraphCtl));
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_graph_create(int64_t n, int64_t m, const int32_t *a, const int32_t *b, const int32_t *w,
+                                  uint32_t flags, dsssp_graph **out)
+{
+    clear_error();
+    if (!out) return fail(DSSSP_ERR_ARG, "null output");
+    *out = nullptr;
+    if (flags != 0) return fail(DSSSP_ERR_ARG, "unknown flags");
+    if (n < 1) return fail(DSSSP_ERR_ARG, "n must be positive");
+    if (int s = check_device(t_device)) return s;
+    std::vector<uint64_t> rp, adj;
+    if (int s = build_csr(n, m, a, b, w, rp, adj)) return s;
+    dsssp_graph *g = new dsssp_graph();
+    g->device = t_device;
+    g->n = n;
+    g->E = adj.size();
+    for (int64_t i = 0; i < m && !g->zero_w; ++i) g->zero_w = w[i] == 0;
+    const uint64_t pool_cap = std::max<uint64_t>(1u << 20, std::min<uint64_t>(g->E / 4, 0xFFFFFFFFull / 2));
+    if (int s = graph_alloc(g, pool_cap)) {
+        graph_release(g);
+        return s;
+    }
+    DevGuard dg(g->device);
+    int s = DSSSP_OK;
+    if (hipMemcpy(g->rp, rp.data(), sizeof(uint64_t) * rp.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        (g->E && hipMemcpy(g->adj, adj.data(), sizeof(uint64_t) * g->E, hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemset(g->dbl, 0, sizeof(uint64_t) * (size_t)n) != hipSuccess ||
+        hipMemset(g->tcnt, 0, sizeof(uint32_t) * (size_t)n) != hipSuccess ||
+        hipMemset(g->gc, 0, sizeof(GraphCtl)) != hipSuccess)
+        s = fail(DSSSP_ERR_HIP, "graph upload failed: %s", hipGetErrorString(hipGetLastError()));
+    if (s) {
+        graph_release(g);
+        return s;
+    }
+    *out = g;
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_graph_destroy(dsssp_graph *g)
+{
+    clear_error();
+    graph_release(g);
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_graph_info(const dsssp_graph *g, int64_t *n, int64_t *m_live, uint64_t *device_bytes)
+{
+    clear_error();
+    if (!g) return fail(DSSSP_ERR_ARG, "null graph");
+    if (n) *n = g->n;
+    if (m_live) {
+        int64_t m = 0;
+        int s = dsssp_graph_export(const_cast<dsssp_graph *>(g), &m, nullptr, nullptr, nullptr);
+        if (s) return s;
+        *m_live = m;
+    }
+    if (device_bytes)
+        *device_bytes = 8 * ((uint64_t)g->n + 1) + 8 * g->E + 8 * (uint64_t)g->n + 8 * g->pool_cap + 4 * (uint64_t)g->n;
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_graph_export(dsssp_graph *g, int64_t *m, int32_t *a, int32_t *b, int32_t *w)
+{
+    clear_error();
+    if (!g || !m) return fail(DSSSP_ERR_ARG, "null argument");
+    DevGuard dg(g->device);
+    const size_t n = (size_t)g->n;
+    std::vector<uint64_t> rp(n + 1), adj(g->E), dbl(n), pool(g->pool_top);
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    HIP_TRY(hipMemcpy(rp.data(), g->rp, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost));
+    if (g->E) HIP_TRY(hipMemcpy(adj.data(), g->adj, sizeof(uint64_t) * g->E, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(dbl.data(), g->dbl, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+    if (g->pool_top) HIP_TRY(hipMemcpy(pool.data(), g->pool, sizeof(uint64_t) * g->pool_top, hipMemcpyDeviceToHost));
+    // Each undirected edge is stored once per endpoint (a self-loop twice in its
+    // row): keep the copy in the row of the smaller endpoint.
+    int64_t cnt = 0;
+    auto take = [&](uint32_t v, uint64_t e, uint64_t &selfs) {
+        if (e & E_DEAD) return;
+        const uint32_t c = e_col(e);
+        if (c < v) return;
+        if (c == v && (selfs++ & 1)) return;
+        if (a) {
+            a[cnt] = (int32_t)v;
+            b[cnt] = (int32_t)c;
+            w[cnt] = (int32_t)e_w(e);
+        }
+        ++cnt;
+    };
+    for (size_t v = 0; v < n; ++v) {
+        uint64_t selfs = 0;
+        for (uint64_t j = rp[v]; j < rp[v + 1]; ++j) take((uint32_t)v, adj[j], selfs);
+        const uint64_t beg = dbl[v] >> 32, len = (uint32_t)dbl[v];
+        for (uint64_t j = 0; j < len; ++j) take((uint32_t)v, pool[beg + j], selfs);
+    }
+    *m = cnt;
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_graph_clone(dsssp_graph *src, dsssp_graph **out)
+{
+    clear_error();
+    if (!src || !out) return fail(DSSSP_ERR_ARG, "null argument");
+    *out = nullptr;
+    dsssp_graph *g = new dsssp_graph();
+    g->device = src->device;
+    g->n = src->n;
+    g->E = src->E;
+    if (int s = graph_alloc(g, src->pool_cap)) {
+        graph_release(g);
+        return s;
+    }
+    if (int s = dsssp_graph_copy(g, src)) {
+        graph_release(g);
+        return s;
+    }
+    *out = g;
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_graph_copy(dsssp_graph *dst, dsssp_graph *src)
+{
+    clear_error();
+    if (!dst || !src) return fail(DSSSP_ERR_ARG, "null argument");
+    if (dst->n != src->n || dst->E != src->E) return fail(DSSSP_ERR_ARG, "graphs differ in shape");
+    DevGuard dg(dst->device);
+    HIP_TRY(hipStreamSynchronize(src->stream));
+    if (dst->pool_cap < src->pool_top) {
+        dev_free(dst->pool);
+        if (int s = dev_alloc((void **)&dst->pool, sizeof(uint64_t) * src->pool_cap, "delta pool")) return s;
+        dst->pool_cap = src->pool_cap;
+    }
+    const size_t n = (size_t)src->n;
+    hipStream_t s = dst->stream;
+    HIP_TRY(hipMemcpyAsync(dst->rp, src->rp, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToDevice, s));
+    if (src->E) HIP_TRY(hipMemcpyAsync(dst->adj, src->adj, sizeof(uint64_t) * src->E, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dst->dbl, src->dbl, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, s));
+    if (src->pool_top)
+        HIP_TRY(hipMemcpyAsync(dst->pool, src->pool, sizeof(uint64_t) * src->pool_top, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dst->gc, src->gc, sizeof(GraphCtl), hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemsetAsync(dst->tcnt, 0, sizeof(uint32_t) * n, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    dst->pool_top = src->pool_top;
+    dst->delta_entries = src->delta_entries;
+    dst->zero_w = src->zero_w;
+    return DSSSP_OK;
+}
+
+static void tree_release(dsssp_tree *t)
+{
+    if (!t) return;
+    DevGuard dg(t->g->device);
+    (void)hipStreamSynchronize(t->g->stream);
+    dev_free(t->st);
+    dev_free(t->stamp);
+    dev_free(t->q[0]);
+    dev_free(t->q[1]);
+    dev_free(t->hq[0]);
+    dev_free(t->hq[1]);
+    dev_free(t->invl);
+    dev_free(t->lvl);
+    dev_free(t->best);
+    dev_free(t->c);
+    if (t->c_host) (void)hipHostFree(t->c_host);
+    delete t;
+}
+
+static int tree_alloc(dsssp_tree *t)
+{
+    dsssp_graph *g = t->g;
+    DevGuard dg(g->device);
+    const size_t n = (size_t)g->n;
+    if (int s = dev_alloc((void **)&t->st, sizeof(uint64_t) * n, "tree state")) return s;
+    if (int s = dev_alloc((void **)&t->stamp, sizeof(uint32_t) * n, "stamps")) return s;
+    if (int s = dev_alloc((void **)&t->q[0], sizeof(uint32_t) * n, "frontier")) return s;
+    if (int s = dev_alloc((void **)&t->q[1], sizeof(uint32_t) * n, "frontier")) return s;
+    if (int s = dev_alloc((void **)&t->invl, sizeof(uint32_t) * n, "invalidated list")) return s;
+    if (int s = dev_alloc((void **)&t->c, sizeof(Counters), "counters")) return s;
+    HIP_TRY(hipHostMalloc((void **)&t->c_host, sizeof(Counters), hipHostMallocDefault));
+    memset(t->c_host, 0, sizeof(Counters));
+    HIP_TRY(hipMemsetAsync(t->stamp, 0, sizeof(uint32_t) * n, g->stream));
+    HIP_TRY(hipMemsetAsync(t->c, 0, sizeof(Counters), g->stream));
+    if (int s = ensure_hq(t, g->E + g->delta_entries)) return s;
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_tree_create(dsssp_graph *g, int32_t src, const int32_t *parent, const int64_t *dist,
+                                 uint32_t flags, dsssp_tree **out)
+{
+    clear_error();
+    if (!g || !out) return fail(DSSSP_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (flags & ~DSSSP_TREE_CANONICAL) return fail(DSSSP_ERR_ARG, "unknown flags");
+    const int64_t n = g->n;
+    std::vector<uint64_t> packed;
+    if (parent) {
+        if (!dist) return fail(DSSSP_ERR_ARG, "parent given without dist");
+        packed.resize((size_t)n);
+        for (int64_t v = 0; v < n; ++v) {
+            const int32_t p = parent[v];
+            if (p == -1 || dist[v] == DSSSP_INF) {
+                packed[v] = ST_INF;
+                continue;
+            }
+            if (p < 0 || p >= n) return fail(DSSSP_ERR_RANGE, "vertex %lld: parent %d outside [0,n)", (long long)v, p);
+            if (dist[v] < 0 || dist[v] >= (int64_t)DIST_INF)
+                return fail(DSSSP_ERR_OVERFLOW, "vertex %lld: distance %lld outside [0, 2^32-1)", (long long)v,
+                            (long long)dist[v]);
+            if (p == v && dist[v] != 0)
+                return fail(DSSSP_ERR_TREE, "root %lld has distance %lld, not 0", (long long)v, (long long)dist[v]);
+            packed[v] = pack_state((uint32_t)dist[v], (uint32_t)p);
+        }
+    } else if (src < 0 || src >= n) {
+        return fail(DSSSP_ERR_RANGE, "source %d outside [0,%lld)", src, (long long)n);
+    }
+    dsssp_tree *t = new dsssp_tree();
+    t->g = g;
+    t->src = src;
+    if (int s = tree_alloc(t)) {
+        tree_release(t);
+        return s;
+    }
+    DevGuard dg(g->device);
+    int64_t rounds = 0;
+    int s = DSSSP_OK;
+    launch_reset_counters(t->c, g->stream);
+    if (parent) {
+        if (hipMemcpyAsync(t->st, packed.data(), sizeof(uint64_t) * (size_t)n, hipMemcpyHostToDevice, g->stream) !=
+            hipSuccess)
+            s = fail(DSSSP_ERR_HIP, "tree upload failed");
+        if (!s && (flags & DSSSP_TREE_CANONICAL)) s = repick_parents(t, g->zero_w, rounds);
+    } else {
+        // From scratch: every vertex unreachable but the source, then relax.
+        launch_fill_u64(t->st, ST_INF, (uint64_t)n, g->stream);
+        Ring r;
+        zero_ring(t, r);
+        uint32_t *oq, *oc;
+        producer_out(t, r, oq, oc);
+        launch_set_source(t->st, (uint32_t)src, t->stamp, next_epoch(t), oq, oc, g->stream);
+        advance(r);
+        s = run_rounds(t, M_RELAX, r, rounds, nullptr);
+        if (!s && g->zero_w) s = repick_parents(t, true, rounds);
+    }
+    if (!s) {
+        if (hipMemcpyAsync(t->c_host, t->c, sizeof(Counters), hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+            hipStreamSynchronize(g->stream) != hipSuccess)
+            s = fail(DSSSP_ERR_HIP, "tree setup failed: %s", hipGetErrorString(hipGetLastError()));
+        else
+            s = finish_errors(t, 0);
+    }
+    if (s) {
+        tree_release(t);
+        return s;
+    }
+    *out = t;
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_tree_destroy(dsssp_tree *t)
+{
+    clear_error();
+    tree_release(t);
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_tree_read(dsssp_tree *t, int32_t *parent, int64_t *dist)
+{
+    clear_error();
+    if (!t || !parent || !dist) return fail(DSSSP_ERR_ARG, "null argument");
+    DevGuard dg(t->g->device);
+    const size_t n = (size_t)t->g->n;
+    std::vector<uint64_t> st(n);
+    HIP_TRY(hipStreamSynchronize(t->g->stream));
+    HIP_TRY(hipMemcpy(st.data(), t->st, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+    for (size_t v = 0; v < n; ++v) {
+        if (st[v] == ST_INF) {
+            parent[v] = -1;
+            dist[v] = DSSSP_INF;
+        } else {
+            parent[v] = (int32_t)st_parent(st[v]);
+            dist[v] = (int64_t)st_dist(st[v]);
+        }
+    }
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_tree_copy(dsssp_tree *dst, dsssp_tree *src)
+{
+    clear_error();
+    if (!dst || !src) return fail(DSSSP_ERR_ARG, "null argument");
+    if (dst->g->n != src->g->n) return fail(DSSSP_ERR_ARG, "trees differ in size");
+    DevGuard dg(dst->g->device);
+    HIP_TRY(hipStreamSynchronize(src->g->stream));
+    const size_t n = (size_t)src->g->n;
+    HIP_TRY(hipMemcpyAsync(dst->st, src->st, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, dst->g->stream));
+    HIP_TRY(hipMemcpyAsync(dst->stamp, src->stamp, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, dst->g->stream));
+    HIP_TRY(hipStreamSynchronize(dst->g->stream));
+    dst->epoch = std::max(dst->epoch, src->epoch);
+    dst->src = src->src;
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_batch_upload(dsssp_graph *g, const dsssp_change *chg, int64_t k, dsssp_batch **out)
+{
+    clear_error();
+    if (!g || !out || k < 0 || (k > 0 && !chg)) return fail(DSSSP_ERR_ARG, "bad argument");
+    *out = nullptr;
+    int64_t ni, nd;
+    bool zw;
+    if (int s = validate_changes(g->n, chg, k, ni, nd, zw)) return s;
+    DevGuard dg(g->device);
+    dsssp_batch *b = new dsssp_batch();
+    b->g = g;
+    b->k = k;
+    b->n_ins = ni;
+    b->n_del = nd;
+    b->zero_w = zw;
+    int s = dev_alloc((void **)&b->rec, sizeof(int4) * (size_t)std::max<int64_t>(k, 1), "batch");
+    if (!s) s = dev_alloc((void **)&b->slot, sizeof(uint64_t) * (size_t)std::max<int64_t>(k, 1), "batch slots");
+    if (!s) s = dev_alloc((void **)&b->pos, sizeof(uint64_t) * (size_t)std::max<int64_t>(k, 1), "batch positions");
+    if (!s)
+        s = dev_alloc((void **)&b->touched, sizeof(uint32_t) * (size_t)std::max<int64_t>(2 * ni, 1), "touched list");
+    if (!s && k > 0 &&
+        hipMemcpyAsync(b->rec, chg, sizeof(int4) * (size_t)k, hipMemcpyHostToDevice, g->stream) != hipSuccess)
+        s = fail(DSSSP_ERR_HIP, "batch upload failed");
+    if (!s && hipStreamSynchronize(g->stream) != hipSuccess) s = fail(DSSSP_ERR_HIP, "batch upload failed");
+    if (s) {
+        dsssp_batch_destroy(b);
+        return s;
+    }
+    *out = b;
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_batch_destroy(dsssp_batch *b)
+{
+    if (!b) return DSSSP_OK;
+    DevGuard dg(b->g->device);
+    (void)hipStreamSynchronize(b->g->stream);
+    dev_free(b->rec);
+    dev_free(b->slot);
+    dev_free(b->pos);
+    dev_free(b->touched);
+    delete b;
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_update_batch(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stats)
+{
+    clear_error();
+    if (!g || !t || !b) return fail(DSSSP_ERR_ARG, "null argument");
+    if (t->g != g || b->g != g) return fail(DSSSP_ERR_ARG, "tree or batch belongs to another graph");
+    if (b->k == 0) {
+        if (stats) *stats = dsssp_stats{};
+        return DSSSP_OK;
+    }
+    return update_impl(g, t, b, stats, 0.0);
+}
+
+extern "C" int dsssp_update(dsssp_graph *g, dsssp_tree *t, const dsssp_change *chg, int64_t k, dsssp_stats *stats)
+{
+    clear_error();
+    if (!g || !t) return fail(DSSSP_ERR_ARG, "null argument");
+    if (t->g != g) return fail(DSSSP_ERR_ARG, "tree belongs to another graph");
+    const auto t0 = std::chrono::steady_clock::now();
+    dsssp_batch *b = nullptr;
+    if (int s = dsssp_batch_upload(g, chg, k, &b)) return s;
+    const double h2d = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    int s = b->k ? update_impl(g, t, b, stats, h2d) : DSSSP_OK;
+    if (!b->k && stats) *stats = dsssp_stats{};
+    std::string keep = s ? last_error() : std::string();
+    dsssp_batch_destroy(b);
+    if (s) fail(s, "%s", keep.c_str());
+    return s;
+}

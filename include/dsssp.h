@@ -1,0 +1,209 @@
+/*
+ * dsssp.h -- C ABI of the MI355X (gfx950) dynamic SSSP update library
+ * (dynamicssspv2_amd/libdsssp.so).
+ *
+ * The reference (ArindamK92/DynamicSSSPv2) has no library interface: its
+ * boundary is the process CLI
+ *     op_main <graph> <n> <m> <sssp> <changes>      (SSSPmainForUndirected.cu:17-32)
+ * plus three text formats (README.md:22-37).  Internally it hands raw AoS
+ * device pointers to its kernels (gpuFunctions_undir.cuh:125-433).  This header
+ * is the boundary those pieces map onto; each entry point names the reference
+ * code it replaces.  Every entry point returns an int status (DSSSP_OK = 0);
+ * dsssp_last_error() gives the message of the calling thread's last failure.
+ * No C++ types, exceptions or torch types cross this ABI.
+ *
+ * Semantics kept from the reference:
+ *   - vertices are 0-based int32 ids below n; weights are int32 >= 0;
+ *   - the graph is undirected; an edge "a b w" is one line of the graph file;
+ *   - a change record (a, b, w, op) has op 0 = delete, 1 = insert
+ *     (inst_status, README.md:33-37); other op values are ignored
+ *     (all_structure_undir.cuh:106,119);
+ *   - insertions are applied to the graph before deletions, and a deletion of
+ *     (a,b,w) removes every a-b edge of weight w, parallel copies and copies
+ *     inserted by the same batch included (all_structure_undir.cuh:106-117,
+ *     gpuFunctions_undir.cuh:200-213);
+ *   - a tree root is a vertex whose parent is itself; an unreachable vertex has
+ *     parent -1 (all_structure_undir.cuh:153-156, seqSSSPwithDist_undir.cpp:81).
+ * Differences (documented fixes of reference defects): distances are exact
+ * int64 on output (held as uint32 on the device; a distance that would reach
+ * 2^32-1 fails with DSSSP_ERR_OVERFLOW), vertices that end unreachable get
+ * parent -1 instead of a stale parent, and updates are race-free (packed
+ * 64-bit atomicMin of (distance << 32 | parent)).
+ *
+ * Threading: a graph handle and everything attached to it (trees, batches)
+ * must be used from one host thread at a time.  Each graph owns one HIP stream
+ * of default priority; calls are synchronous unless stated.
+ */
+#ifndef DSSSP_H
+#define DSSSP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define DSSSP_OK 0
+#define DSSSP_ERR_ARG 1       /* null pointer, negative size, bad flag      */
+#define DSSSP_ERR_RANGE 2     /* vertex id outside [0, n)                    */
+#define DSSSP_ERR_WEIGHT 3    /* negative weight                             */
+#define DSSSP_ERR_OVERFLOW 4  /* a distance reached 2^32-1                   */
+#define DSSSP_ERR_NOMEM 5     /* host or device allocation failed            */
+#define DSSSP_ERR_HIP 6       /* HIP runtime error                           */
+#define DSSSP_ERR_NODEV 7     /* no usable gfx950 device                     */
+#define DSSSP_ERR_IO 8        /* file could not be opened / written          */
+#define DSSSP_ERR_FORMAT 9    /* malformed line in a text file               */
+#define DSSSP_ERR_TREE 10     /* input tree inconsistent with the graph      */
+#define DSSSP_ERR_BUDGET 11   /* served form: memory budget exceeded         */
+#define DSSSP_ERR_STATE 12    /* call not valid in the object's state        */
+
+/* Distance reported for an unreachable vertex (its parent is -1). */
+#define DSSSP_INF INT64_MAX
+
+/* One change record; same fields and meaning as a change-file line
+ * "a b w inst" (README.md:33-37, all_structure_undir.cuh:52-57). */
+typedef struct {
+    int32_t a, b, w, op;
+} dsssp_change;
+
+typedef struct dsssp_graph dsssp_graph;  /* device-resident mutable CSR          */
+typedef struct dsssp_tree dsssp_tree;    /* device-resident SSSP tree of a graph  */
+typedef struct dsssp_batch dsssp_batch;  /* change batch resident in HBM          */
+
+/* Per-update statistics.  Times are device time measured with HIP events on
+ * the graph's stream.  t_update_ms spans "batch resident in HBM" to
+ * "converged": insertion into the resident CSR, deletion classification and
+ * tombstoning, subtree invalidation, seeding and relaxation. */
+typedef struct {
+    int64_t n_insert, n_delete;     /* records with op 1 / op 0                */
+    int64_t n_invalidated;          /* vertices reset by subtree invalidation  */
+    int64_t inv_rounds;             /* invalidation rounds launched            */
+    int64_t relax_rounds;           /* relaxation rounds launched              */
+    int64_t relax_launches;         /* relaxation kernel launches (incl. empty)*/
+    int64_t frontier_vertices;      /* frontier vertices processed (relax)     */
+    int64_t relax_edges;            /* adjacency entries scanned (relax)       */
+    int64_t relax_improved;         /* successful packed atomicMin (relax)     */
+    int64_t relax_enqueued;         /* vertices enqueued (relax)               */
+    int64_t other_edges;            /* entries scanned by invalidation + seed  */
+    int64_t probes;                 /* entries read by deletion searches       */
+    double t_h2d_ms;                /* batch host->device copy (dsssp_update)  */
+    double t_update_ms;             /* total device time of the update         */
+    double t_insert_ms, t_classify_ms, t_invalidate_ms, t_seed_ms, t_relax_ms;
+    double t_relax_kernel_ms;       /* sum of relaxation kernel durations      */
+    double alg_bytes;               /* algorithmic HBM bytes, whole update     */
+    double alg_bytes_relax;         /* algorithmic HBM bytes, relaxation only  */
+} dsssp_stats;
+
+/* Tuning knobs (zero fields keep the current value). */
+typedef struct {
+    int32_t block;            /* threads per workgroup of frontier kernels (64..1024) */
+    int32_t grid;             /* workgroups per frontier launch                        */
+    int32_t heavy_degree;     /* rows longer than this are split into chunks           */
+    int32_t chunk;            /* entries per heavy chunk (multiple of 64)              */
+    int32_t round_batch;      /* relaxation rounds launched between convergence checks */
+    int32_t kernel_timing;    /* 1: time every relaxation launch with events           */
+} dsssp_tuning;
+
+/* ---- library ---------------------------------------------------------- */
+const char *dsssp_last_error(void);
+const char *dsssp_version(void);
+/* Select the HIP device for objects created afterwards by this thread. */
+int dsssp_set_device(int device);
+int dsssp_get_tuning(dsssp_tuning *t);
+int dsssp_set_tuning(const dsssp_tuning *t);
+/* Frees memory the library returned (file readers, generators). */
+void dsssp_free(void *p);
+
+/* ---- text formats (host only) --------------------------------------------
+ * Replace read_graphEdges / readin_changes / read_SSSP
+ * (all_structure_undir.cuh:178-198, 89-126, 133-171) and add the writer the
+ * reference lacks (it only prints 40 distances, SSSPmainForUndirected.cu:298-312).
+ * Readers accept any run of blanks between fields and a missing final newline,
+ * and reject malformed lines, ids outside [0,n) (n > 0) and negative weights. */
+int dsssp_read_graph(const char *path, int64_t n, int64_t *m,
+                     int32_t **a, int32_t **b, int32_t **w);
+int dsssp_read_changes(const char *path, int64_t n, int64_t *k, dsssp_change **chg);
+/* parent/dist have n entries.  Missing vertices and parent -1 are unreachable
+ * (all_structure_undir.cuh:144-156); the distance token of an unreachable
+ * vertex may be anything (the reference's Dijkstra prints it as a lossy
+ * double, seqSSSPwithDist_undir.cpp:172). */
+int dsssp_read_sssp(const char *path, int64_t n, int32_t *parent, int64_t *dist);
+/* "v parent dist" per line, exact integers; unreachable -> "v -1 -1". */
+int dsssp_write_sssp(const char *path, int64_t n, const int32_t *parent, const int64_t *dist);
+
+/* ---- synthetic inputs (host, deterministic for a seed, multi-threaded) ---- */
+/* Random recursive spanning tree (vertex i>0 joined to a uniform j<i) followed
+ * by uniform distinct pairs until m undirected edges; weights U[wlo,whi].
+ * The first n-1 edges are the spanning tree.  a/b/w have m entries. */
+int dsssp_gen_tree_plus_random(int64_t n, int64_t m, int32_t wlo, int32_t whi, uint64_t seed,
+                               int32_t *a, int32_t *b, int32_t *w);
+/* R-MAT (a,b,c, d = 1-a-b-c), 2^scale vertices, edge_factor*2^scale draws,
+ * deduplicated, symmetrised, self-loops dropped; weights U[wlo,whi] per pair.
+ * Returns library-allocated arrays (dsssp_free) with *m undirected edges. */
+int dsssp_gen_rmat(int32_t scale, int32_t edge_factor, double pa, double pb, double pc,
+                   int32_t wlo, int32_t whi, uint64_t seed, int64_t *m,
+                   int32_t **a, int32_t **b, int32_t **w);
+/* side x side 4-neighbour grid, a fraction drop of the edges removed at random,
+ * restricted to the component of vertex 0 (v = row*side + col); weights U[wlo,whi]. */
+int dsssp_gen_grid(int32_t side, double drop, int32_t wlo, int32_t whi, uint64_t seed, int64_t *m,
+                   int32_t **a, int32_t **b, int32_t **w);
+/* k change records: round(k*ins_frac) insertions of distinct absent pairs
+ * (no self-loops) with weights U[wlo,whi], the rest deletions sampled without
+ * replacement from edges [first_deletable, m); records in random order. */
+int dsssp_gen_changes(int64_t n, int64_t m, const int32_t *a, const int32_t *b, const int32_t *w,
+                      int64_t k, double ins_frac, int32_t wlo, int32_t whi,
+                      int64_t first_deletable, uint64_t seed, dsssp_change *out);
+
+/* ---- device graph ----------------------------------------------------------
+ * Replaces the host CSR flatten + UVM upload (SSSPmainForUndirected.cu:66-91):
+ * the CSR is built on the host (rows sorted by (neighbour, weight)) and copied
+ * once into HBM, where later batches edit it in place. */
+#define DSSSP_GRAPH_DEFAULT 0u
+int dsssp_graph_create(int64_t n, int64_t m, const int32_t *a, const int32_t *b, const int32_t *w,
+                       uint32_t flags, dsssp_graph **out);
+int dsssp_graph_destroy(dsssp_graph *g);
+/* n, live undirected edges, HBM bytes held by the graph. */
+int dsssp_graph_info(const dsssp_graph *g, int64_t *n, int64_t *m_live, uint64_t *device_bytes);
+/* Live undirected edges (each once, a <= b).  With a == NULL only *m is set. */
+int dsssp_graph_export(dsssp_graph *g, int64_t *m, int32_t *a, int32_t *b, int32_t *w);
+/* Device-to-device copy of the whole graph state; dst must have been created
+ * from a graph of the same n and base size (dsssp_graph_clone makes one). */
+int dsssp_graph_clone(dsssp_graph *src, dsssp_graph **out);
+int dsssp_graph_copy(dsssp_graph *dst, dsssp_graph *src);
+
+/* ---- device tree -----------------------------------------------------------
+ * Replaces read_SSSP into UVM (SSSPmainForUndirected.cu:146-165).
+ * parent == NULL: compute the tree from src on the GPU (the op_ssspSeq role,
+ * seqSSSPwithDist_undir.cpp:66-128), giving every vertex the smallest-id
+ * neighbour on a shortest path as parent.
+ * Otherwise parent/dist (n entries) is the given tree; DSSSP_TREE_CANONICAL
+ * re-derives parents by that smallest-id rule (distances must be exact). */
+#define DSSSP_TREE_DEFAULT 0u
+#define DSSSP_TREE_CANONICAL 1u
+int dsssp_tree_create(dsssp_graph *g, int32_t src, const int32_t *parent, const int64_t *dist,
+                      uint32_t flags, dsssp_tree **out);
+int dsssp_tree_destroy(dsssp_tree *t);
+int dsssp_tree_read(dsssp_tree *t, int32_t *parent, int64_t *dist);
+int dsssp_tree_copy(dsssp_tree *dst, dsssp_tree *src);
+
+/* ---- the update --------------------------------------------------------------
+ * Replaces STEP1 + STEP2 of op_main (SSSPmainForUndirected.cu:196-292):
+ * deleteEdge/insertEdge (gpuFunctions_undir.cuh:179-251), the invalidation loop
+ * (updateNeighbors_del + filterAffectedNodes, :125-144, :312-349) and the
+ * relaxation loop (updateNeighbors + filterAllAffected, :162-174, :370-433).
+ * Applies the batch to the graph and brings the tree to the exact SSSP tree of
+ * the updated graph.  Changes are validated on the host (ids in [0,n),
+ * weights >= 0) before anything is launched; stats may be NULL. */
+int dsssp_update(dsssp_graph *g, dsssp_tree *t, const dsssp_change *chg, int64_t k,
+                 dsssp_stats *stats);
+/* Upload a batch once (validated) so that updates start with it in HBM. */
+int dsssp_batch_upload(dsssp_graph *g, const dsssp_change *chg, int64_t k, dsssp_batch **out);
+int dsssp_batch_destroy(dsssp_batch *b);
+int dsssp_update_batch(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DSSSP_H */

@@ -1,0 +1,322 @@
+"""GPU parity: the HIP update path (through the C ABI) against the CPU oracle.
+
+Bar: distances bit-exact to Dijkstra on the updated graph; parents bit-exact to
+the canonical tree (smallest-id tight in-neighbour; for graphs that ever held a
+zero weight, the smallest-id tight in-neighbour one BFS level up); the updated
+CSR equal, as a multiset, to the reference's graph-after-batch semantics.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import dynamicssspv2_amd as ds
+import oracle
+from tests._util import (GOLDEN, GOLDEN_N, INF, golden_case, random_batch, random_connected,
+                         random_multigraph, read_ref_sssp)
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", file=sys.stderr, flush=True)
+
+
+def edge_multiset(a, b, w):
+    a, b, w = np.asarray(a, np.int64), np.asarray(b, np.int64), np.asarray(w, np.int64)
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    keys = (lo << 40) | (hi << 8) | 0
+    order = np.lexsort((w, hi, lo))
+    return np.stack([lo[order], hi[order], w[order]], 1)
+
+
+class Mirror:
+    """CPU copy of the graph driven through the oracle's change semantics."""
+
+    def __init__(self, n, a, b, w):
+        self.n = n
+        self.a, self.b, self.w = (np.asarray(x, np.int32) for x in (a, b, w))
+        self.zero = bool(np.any(self.w == 0))
+
+    def apply(self, chg):
+        chg = np.asarray(chg, np.int32).reshape(-1, 4)
+        if np.any((chg[:, 3] == 1) & (chg[:, 2] == 0)):
+            self.zero = True
+        self.a, self.b, self.w = oracle.apply_changes(self.n, self.a, self.b, self.w, chg)
+
+    def csr(self):
+        return oracle.CSR.from_edges(self.n, self.a, self.b, self.w)
+
+
+def check_tree(mirror, par, dist, srcs, what=""):
+    csr = mirror.csr()
+    _, dref = oracle.dijkstra(csr, srcs)
+    bad = np.flatnonzero(dist != dref)
+    assert len(bad) == 0, f"{what}: {len(bad)} distances differ, first {bad[:5]}: {dist[bad[:5]]} vs {dref[bad[:5]]}"
+    pref = oracle.canonical_parents(csr, dref, srcs, rule=1 if mirror.zero else 0)
+    badp = np.flatnonzero(par != pref)
+    assert len(badp) == 0, f"{what}: {len(badp)} parents differ, first {badp[:5]}: {par[badp[:5]]} vs {pref[badp[:5]]}"
+    assert oracle.certificate(csr, dist, par, srcs)[0] == 0
+
+
+def check_graph(mirror, g):
+    ga, gb, gw = g.export()
+    assert np.array_equal(edge_multiset(ga, gb, gw), edge_multiset(mirror.a, mirror.b, mirror.w))
+
+
+# ----------------------------------------------------------------------------- known answers
+
+def test_toy_known_answer():
+    say("toy known answer")
+    a, b, w, chg, n = golden_case("toy")
+    g = ds.Graph(n, a, b, w)
+    p0, d0 = ds.read_sssp(os.path.join(GOLDEN, "toy_SSSP.txt"), n)
+    t = ds.Tree(g, parent=p0, dist=d0)
+    st = ds.update(g, t, chg)
+    par, dist = t.read()
+    assert dist.tolist() == [0, 4, 6, 6, 4, 9]
+    # the batch inserts a zero-weight edge: BFS-level canonical parents, 1<-4 via w0
+    assert par.tolist() == [0, 4, 0, 1, 0, 2]
+    assert st["n_insert"] == 2 and st["n_delete"] == 1 and st["n_invalidated"] == 5
+    m = Mirror(n, a, b, w)
+    m.apply(chg)
+    check_graph(m, g)
+    check_tree(m, par, dist, [0], "toy")
+
+
+@pytest.mark.parametrize("name", ["r300", "r2000", "sparse1500"])
+def test_golden_reference_vectors(name):
+    """Input tree = the reference Dijkstra's SSSP file; expected distances = the
+    reference Dijkstra on the updated graph (tests/golden)."""
+    say(f"golden {name}")
+    n = GOLDEN_N[name]
+    a, b, w, chg, _ = golden_case(name)
+    g = ds.Graph(n, a, b, w)
+    p0, d0 = ds.read_sssp(os.path.join(GOLDEN, f"{name}_SSSP.txt"), n)
+    t = ds.Tree(g, parent=p0, dist=d0)
+    ds.update(g, t, chg)
+    par, dist = t.read()
+    _, ed = read_ref_sssp(os.path.join(GOLDEN, f"{name}_updated.txt"), n)
+    assert np.array_equal(dist, ed)
+    m = Mirror(n, a, b, w)
+    m.apply(chg)
+    check_graph(m, g)
+    csr = m.csr()
+    assert oracle.certificate(csr, dist, par)[0] == 0
+    if m.zero:  # every parent is re-derived: canonical even from a non-canonical input tree
+        assert np.array_equal(par, oracle.canonical_parents(csr, dist, rule=1))
+    # the reference's own update rules, run sequentially, reach the same distances
+    _, rdist = oracle.ref_update(n, a, b, w, p0, d0, chg)
+    assert np.array_equal(rdist, dist)
+
+
+# ----------------------------------------------------------------------------- from scratch
+
+@pytest.mark.parametrize("seed", range(6))
+def test_tree_from_scratch(seed):
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(2, 3000))
+    if seed % 3 == 0:
+        a, b, w = random_multigraph(rng, n, int(rng.integers(0, 6 * n)), 0, 5)  # zero weights, parallel, loops
+    else:
+        a, b, w = random_connected(rng, n, int(n * rng.uniform(1, 8)), 1, 1000)
+    say(f"from scratch seed={seed} n={n} m={len(a)}")
+    g = ds.Graph(n, a, b, w)
+    src = int(rng.integers(0, n))
+    t = ds.Tree(g, src)
+    par, dist = t.read()
+    check_tree(Mirror(n, a, b, w), par, dist, [src], f"scratch {seed}")
+
+
+def test_canonicalize_given_tree():
+    rng = np.random.default_rng(7)
+    n = 2000
+    a, b, w = random_connected(rng, n, 8000, 1, 5)  # many ties
+    csr = oracle.CSR.from_edges(n, a, b, w)
+    p, d, _ = oracle.djk(csr, 0, oracle.INF)
+    say("canonicalize")
+    g = ds.Graph(n, a, b, w)
+    t = ds.Tree(g, parent=p, dist=d, canonical=True)
+    par, dist = t.read()
+    assert np.array_equal(dist, d)
+    assert np.array_equal(par, oracle.canonical_parents(csr, d, rule=0))
+    t2 = ds.Tree(g, parent=p, dist=d)
+    assert np.array_equal(t2.read()[0], p)  # kept as given without the flag
+
+
+# ----------------------------------------------------------------------------- updates
+
+def run_sequence(n, a, b, w, batches, src=0, what=""):
+    g = ds.Graph(n, a, b, w)
+    t = ds.Tree(g, src)
+    m = Mirror(n, a, b, w)
+    check_tree(m, *t.read(), [src], f"{what} initial")
+    for i, chg in enumerate(batches):
+        say(f"{what} batch {i} k={len(chg)}")
+        st = ds.update(g, t, chg)
+        m.apply(chg)
+        par, dist = t.read()
+        check_tree(m, par, dist, [src], f"{what} batch {i}")
+        assert st["n_insert"] == int(np.sum(np.asarray(chg).reshape(-1, 4)[:, 3] == 1))
+    check_graph(m, g)
+    return g, t
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_update_sequences(seed):
+    rng = np.random.default_rng(200 + seed)
+    n = int(rng.integers(5, 4000))
+    zero = seed % 4 == 3
+    a, b, w = random_connected(rng, n, int(n * rng.uniform(1, 6)), 0 if zero else 1, 50)
+    batches = []
+    ca, cb, cw = a, b, w
+    for _ in range(4):
+        k = int(rng.integers(1, max(2, n // 2)))
+        chg = random_batch(rng, n, ca, cb, cw, k, rng.uniform(0, 1), 0 if zero else 1, 50)
+        batches.append(chg)
+        ca, cb, cw = oracle.apply_changes(n, ca, cb, cw, chg)
+    run_sequence(n, a, b, w, batches, src=int(rng.integers(0, n)), what=f"seq{seed}")
+
+
+def test_edge_cases():
+    n = 12
+    # path 0-1-...-7, a triangle 8-9-10, vertex 11 isolated; parallel edge 2-3
+    a = [0, 1, 2, 3, 4, 5, 6, 8, 9, 10, 2]
+    b = [1, 2, 3, 4, 5, 6, 7, 9, 10, 8, 3]
+    w = [1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 5]
+    batches = [
+        np.zeros((0, 4), np.int32),                          # empty batch
+        [[5, 9, 3, 7]],                                      # unknown op: ignored
+        [[2, 3, 1, 0], [3, 2, 1, 0]],                        # duplicate deletion of a tree edge (parallel 2-3 w5 stays)
+        [[2, 3, 4, 0]],                                      # deletion whose weight matches nothing
+        [[2, 3, 5, 0]],                                      # last 2-3 copy: 3..7 disconnected
+        [[7, 8, 1, 1], [4, 4, 0, 1]],                        # reconnect through the triangle side; self-loop
+        [[4, 4, 0, 0], [0, 9, 3, 1], [0, 9, 3, 0]],          # delete self-loop; insert+delete same edge in one batch
+        [[11, 0, 1, 1], [11, 0, 1, 1], [11, 0, 1, 0]],       # parallel inserts then deletion of both
+        [[0, 1, 1, 0]],                                      # cut at the source: almost everything unreachable
+        [[0, 11, 2, 1], [11, 1, 2, 1]],                      # reconnect via the isolated vertex
+    ]
+    g, t = run_sequence(n, a, b, w, [np.asarray(x, np.int32).reshape(-1, 4) for x in batches], 0, "edge")
+    par, dist = t.read()
+    assert dist[0] == 0 and par[0] == 0
+
+
+def test_errors_leave_state_untouched():
+    n = 5
+    g = ds.Graph(n, [0, 1, 2], [1, 2, 3], [1, 1, 1])
+    t = ds.Tree(g, 0)
+    before = t.read()
+    for bad, code in (([[0, 7, 1, 1]], ds.ERR_RANGE), ([[0, 1, -3, 0]], ds.ERR_WEIGHT), ([[-1, 1, 1, 0]], ds.ERR_RANGE)):
+        with pytest.raises(ds.DssspError) as e:
+            ds.update(g, t, bad)
+        assert e.value.code == code
+    after = t.read()
+    assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+    with pytest.raises(ds.DssspError) as e:
+        ds.Tree(g, parent=[0, 0, 1, 2, 9], dist=[0, 1, 2, 3, 4])
+    assert e.value.code == ds.ERR_RANGE
+    with pytest.raises(ds.DssspError) as e:
+        ds.Tree(g, parent=[0, 1, 1, 2, -1], dist=[0, 5, 2, 3, 0])
+    assert e.value.code == ds.ERR_TREE  # a root (parent itself) with nonzero distance
+
+
+def test_overflow_is_reported():
+    n = 3
+    g = ds.Graph(n, [0, 1], [1, 2], [2_000_000_000, 2_000_000_000])
+    with pytest.raises(ds.DssspError) as e:
+        ds.Tree(g, 0)
+    assert e.value.code == ds.ERR_OVERFLOW
+
+
+def test_heavy_rows_and_chunks():
+    """Tiny heavy threshold so that most rows go through the chunk path."""
+    old = ds.get_tuning()
+    ds.set_tuning(heavy_degree=64, chunk=64, grid=64)
+    try:
+        a, b, w = ds.gen_rmat(13, 16, wlo=1, whi=1000, seed=77)
+        n = 1 << 13
+        rng = np.random.default_rng(3)
+        batches = []
+        ca, cb, cw = a, b, w
+        for ins in (0.0, 0.5, 1.0):
+            chg = ds.gen_changes(n, ca, cb, cw, 3000, ins, 1, 1000, seed=int(rng.integers(1 << 30)))
+            batches.append(chg)
+            ca, cb, cw = oracle.apply_changes(n, ca, cb, cw, chg)
+        deg = np.bincount(np.concatenate([a, b]), minlength=n)
+        run_sequence(n, a, b, w, batches, src=int(np.argmax(deg)), what="rmat13-heavy")
+    finally:
+        ds.set_tuning(**old)
+
+
+def test_config1_full_size():
+    """n=100,000, m=1,000,000, w U[1,100], src 0, batch 10,000 (50/50), deletions
+    from the non-spanning-tree edges; the oracle is Dijkstra on the updated graph."""
+    n, m = 100_000, 1_000_000
+    a, b, w = ds.gen_tree_plus_random(n, m, 1, 100, seed=1001)
+    chg = ds.gen_changes(n, a, b, w, 10_000, 0.5, 1, 100, seed=1002, first_deletable=n - 1)
+    say("config1 full size")
+    g, t = run_sequence(n, a, b, w, [chg], 0, "config1")
+    # and the reference's own update rules agree on this input
+    csr = oracle.CSR.from_edges(n, a, b, w)
+    p0, d0, _ = oracle.djk(csr, 0)
+    _, rdist = oracle.ref_update(n, a, b, w, p0, d0, chg)
+    assert np.array_equal(rdist, t.read()[1])
+
+
+def test_grid_deep_updates():
+    side = 300
+    a, b, w = ds.gen_grid(side, 0.01, 1, 10000, seed=31)
+    n = side * side
+    rng = np.random.default_rng(4)
+    chg = random_batch(rng, n, a, b, w, 2000, 0.3, 1, 10000)
+    run_sequence(n, a, b, w, [chg], 0, "grid300")
+
+
+def test_deterministic_bytes():
+    a, b, w = ds.gen_rmat(12, 16, seed=8)
+    n = 1 << 12
+    chg = ds.gen_changes(n, a, b, w, 2000, 0.5, 1, 1000, seed=9)
+    outs = []
+    for _ in range(3):
+        g = ds.Graph(n, a, b, w)
+        t = ds.Tree(g, 0)
+        ds.update(g, t, chg)
+        outs.append(t.read())
+    for p, d in outs[1:]:
+        assert np.array_equal(p, outs[0][0]) and np.array_equal(d, outs[0][1])
+
+
+def test_resident_batch_and_copies():
+    a, b, w = ds.gen_tree_plus_random(5000, 30000, 1, 100, seed=3)
+    n = 5000
+    chg = ds.gen_changes(n, a, b, w, 3000, 0.5, 1, 100, seed=4, first_deletable=n - 1)
+    g = ds.Graph(n, a, b, w)
+    t = ds.Tree(g, 0)
+    g0, t0 = g.clone(), ds.Tree(g, 0)
+    t0.copy_from(t)
+    batch = ds.Batch(g, chg)
+    ds.update(g, t, batch)
+    r1 = t.read()
+    # restore and repeat: identical result
+    g.copy_from(g0)
+    t.copy_from(t0)
+    ds.update(g, t, batch)
+    r2 = t.read()
+    assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[1], r2[1])
+    m = Mirror(n, a, b, w)
+    m.apply(chg)
+    check_tree(m, *r2, [0], "resident")
+
+
+def test_cli_drop_in(tmp_path):
+    out = tmp_path / "out.txt"
+    cmd = [ds.CLI_PATH, os.path.join(GOLDEN, "toy_graph.txt"), "6", "6", os.path.join(GOLDEN, "toy_SSSP.txt"),
+           os.path.join(GOLDEN, "toy_cE.txt"), "--out", str(out)]
+    say("cli toy")
+    res = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    assert "Total Time taken for SSSP update" in res.stdout
+    p, d = ds.read_sssp(str(out), 6)
+    assert d.tolist() == [0, 4, 6, 6, 4, 9]
