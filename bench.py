@@ -1,26 +1,33 @@
 """Benchmark of the dynamic SSSP batch update on one MI355X.
 
-Workload (default): R-MAT scale 24, edge factor 16, (a,b,c,d) = (0.57, 0.19,
-0.19, 0.05), deduplicated, symmetrised, self-loops dropped, int32 weights
-U[1,1000]; source = the highest-degree vertex; change batches of k = 1,000,000
-records, 50% insertions of absent pairs, 50% deletions of current edges.  A step
-is one batch applied to the resident graph and tree (classify, tombstone,
-in-HBM insert, invalidate, seed, relax), with the batch already in HBM.  Steps
-apply distinct batches one after another to the evolving graph.
+Default workload (configuration 2, the headline): R-MAT scale 24, edge factor
+16, (a,b,c,d) = (0.57, 0.19, 0.19, 0.05), deduplicated, symmetrised, self-loops
+dropped, int32 weights U[1,1000]; source = the highest-degree vertex; change
+batches of k = 1,000,000 records, 50% insertions of absent pairs, 50% deletions
+of current edges.  A step is one batch applied to the resident graph and tree
+(in-HBM insert, classify + tombstone, invalidate, seed, relax) with the batch
+already in HBM; steps apply distinct batches one after another to the evolving
+graph.  The timed region is the K steps between two device synchronisations.
 
-Prints one JSON line (the driver's contract) with the changed-edges/s
-throughput, p50/p99 per-batch latency, the relaxation kernel's roofline
-figures, a bit-exactness check against a CPU Dijkstra of the final graph, and
-the CPU baseline (the reference's sequential Dijkstra, restated in
-oracle/oracle.c, recomputing the tree from scratch as the reference's CPU path
-does).
+Other workloads (--config): 1 small random graph; 3 high-diameter grid;
+4a K=64 trees on one R-MAT graph; 4b 32 graphs in one launch; 5 served stream.
+Each prints one JSON line of the same shape.
 
-    python bench.py [--steps K] [--warmup W] [--scale 24] [--k 1000000] [--ins 0.5]
+The JSON line carries the changed-edges/s throughput, p50/p99 per-batch latency,
+the relaxation kernel's roofline figures (algorithmic bytes / summed launch
+durations, measured with HIP events on the graph's stream), a bit-exactness
+check against a CPU Dijkstra of the final graph, and the CPU baseline: one
+from-scratch sequential Dijkstra (the reference's CPU path, restated in
+oracle/oracle.c) on the final graph, on one core.
+
+    python bench.py [--steps K] [--warmup W] [--config 2] [--scale 24] [--k 1000000] [--ins 0.5]
 """
 import argparse
 import json
+import math
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -39,19 +46,31 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--config", default="2", choices=["1", "2", "3", "4a", "4b", "5"])
     p.add_argument("--scale", type=int, default=24)
     p.add_argument("--edge-factor", type=int, default=16)
     p.add_argument("--k", type=int, default=1_000_000)
     p.add_argument("--ins", type=float, default=0.5)
+    p.add_argument("--side", type=int, default=4096)
+    p.add_argument("--trees", type=int, default=64)
+    p.add_argument("--graphs", type=int, default=32)
+    p.add_argument("--instances", type=int, default=16)
+    p.add_argument("--requests", type=int, default=400)
+    p.add_argument("--rate", type=float, default=0.0, help="served: arrivals per second (0: all at once)")
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU Dijkstra baseline / exactness check")
     p.add_argument("--no-timing", action="store_true", help="no per-launch events (roofline unmeasured)")
     return p.parse_args()
 
 
-def make_batches(ds, np, n, a, b, w, k, ins, count, seed):
+def pct(x, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(x), q)) if len(x) else None
+
+
+def disjoint_batches(ds, np, n, a, b, w, k, ins, count, seed, wlo=1, whi=1000, first_deletable=0):
     """count disjoint batches of exactly k records each (round(k*ins) insertions)."""
-    allc = ds.gen_changes(n, a, b, w, k * count, ins, 1, 1000, seed=seed)
+    allc = ds.gen_changes(n, a, b, w, k * count, ins, wlo, whi, seed=seed, first_deletable=first_deletable)
     ins_rec = allc[allc[:, 3] == 1]
     del_rec = allc[allc[:, 3] == 0]
     ki = int(round(k * ins))
@@ -65,6 +84,324 @@ def make_batches(ds, np, n, a, b, w, k, ins, count, seed):
     return out
 
 
+def cpu_check(ds, np, g, n, srcs, par, dist, k, label, run_cpu=True):
+    """Exactness on the final graph + the CPU baseline (one Dijkstra per source)."""
+    import oracle
+    ea, eb, ew = g.export()
+    csr = oracle.CSR.from_edges(n, ea, eb, ew)
+    del ea, eb, ew
+    tc = time.perf_counter()
+    if len(srcs) == 1:
+        _, cd, _ = oracle.djk(csr, int(srcs[0]), oracle.INF)
+    else:
+        _, cd = oracle.dijkstra(csr, [int(s) for s in srcs])
+    t_cpu = time.perf_counter() - tc
+    mism = int(np.sum(cd != dist))
+    code, _ = oracle.certificate(csr, dist, par, [int(s) for s in srcs])
+    canon = bool(np.array_equal(par, oracle.canonical_parents(csr, cd, [int(s) for s in srcs], rule=0)))
+    check = {"dijkstra_mismatches": mism, "certificate": int(code), "parents_canonical": canon}
+    cpu = {"value": k / t_cpu, "unit": "changed edges/s", "cores": 1, "kind": "port",
+           "sample": f"one from-scratch sequential Dijkstra (oracle/oracle.c, restating "
+                     f"seqSSSPwithDist_undir.cpp:66-128) on the final {label} graph ({len(csr.col) // 2} edges) "
+                     f"= the reference's CPU cost of one k={k} batch; {t_cpu:.2f} s"}
+    log(f"cpu dijkstra {t_cpu:.2f}s, mismatches {mism}, certificate {code}, canonical {canon}")
+    return check, cpu
+
+
+def roofline(stats):
+    relax_bytes = sum(s["alg_bytes_relax"] for s in stats)
+    relax_ms = sum(s["t_relax_kernel_ms"] for s in stats)
+    launches = sum(s["relax_launches"] for s in stats)
+    achieved = relax_bytes / (relax_ms / 1e3) / 1e9 if relax_ms > 0 else None
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBPS) if achieved else None, "traffic": None,
+            "kernel": "k_frontier<M_RELAX>", "launches": launches,
+            "avg_launch_us": relax_ms / max(1, launches) * 1e3,
+            "alg_bytes_per_launch": relax_bytes / max(1, launches)}
+
+
+def line(args, value, wall, steps, cfg, stats, check, cpu, unit="changed edges/s", extra=None):
+    import numpy as np
+    lat = [s["t_update_ms"] for s in stats]
+    alg = sum(s["alg_bytes"] for s in stats)
+    out = {
+        "metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": wall / max(1, steps) * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic", "config": cfg,
+        "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+        "update_alg_GBps": alg / (sum(lat) / 1e3) / 1e9 if sum(lat) > 0 else None,
+        "dist_bitexact": bool(check.get("gpu_recompute_equal", True)
+                              and check.get("dijkstra_mismatches", 0) == 0 and check.get("certificate", 0) == 0),
+        "check": check, "roofline": roofline(stats), "cpu_baseline": cpu,
+    }
+    if extra:
+        out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def timed_steps(ds, g, t, res, warmup, timing=True):
+    for i in range(warmup):
+        ds.update(g, t, res[i])
+    ds.set_tuning(kernel_timing=1 if timing else 0)
+    stats = []
+    t0 = time.perf_counter()
+    for r in res[warmup:]:
+        stats.append(ds.update(g, t, r))
+    wall = time.perf_counter() - t0
+    ds.set_tuning(kernel_timing=0)
+    return stats, wall
+
+
+def run_plain(args, ds, np, n, a, b, w, src, batches, label, cfg):
+    g = ds.Graph(n, a, b, w)
+    t = ds.Tree(g, src)
+    res = [ds.Batch(g, c) for c in batches]
+    stats, wall = timed_steps(ds, g, t, res, args.warmup, not args.no_timing)
+    for r in res:
+        r.close()
+    k = len(batches[-1])
+    value = k * args.steps / wall
+    log(f"{label}: {args.steps} steps in {wall:.3f}s -> {value:.3e} changed edges/s, "
+        f"p50 {pct([s['t_update_ms'] for s in stats], 50):.3f} ms")
+    par, dist = t.read()
+    fpar, fdist = ds.Tree(g, src).read()
+    check = {"gpu_recompute_equal": bool(np.array_equal(dist, fdist) and np.array_equal(par, fpar))}
+    cpu = None
+    if not args.no_cpu:
+        c2, cpu = cpu_check(ds, np, g, n, [src], par, dist, k, label)
+        check.update(c2)
+    line(args, value, wall, args.steps, cfg, stats, check, cpu)
+
+
+def config1(args, ds, np):
+    n, m, k = 100_000, 1_000_000, 10_000
+    a, b, w = ds.gen_tree_plus_random(n, m, 1, 100, seed=args.seed)
+    batches = disjoint_batches(ds, np, n, a, b, w, k, 0.5, args.warmup + args.steps, args.seed + 1, 1, 100,
+                               first_deletable=n - 1)
+    run_plain(args, ds, np, n, a, b, w, 0, batches, "config1",
+              {"workload": "random tree+pairs n=1e5 m=1e6 k=1e4 ins=0.5", "n": n, "m": m, "k": k, "src": 0})
+
+
+def config2(args, ds, np):
+    t0 = time.time()
+    a, b, w = ds.gen_rmat(args.scale, args.edge_factor, (0.57, 0.19, 0.19), 1, 1000, seed=args.seed)
+    n = 1 << args.scale
+    deg = np.bincount(np.concatenate([a, b]), minlength=n)
+    src = int(np.argmax(deg))  # ties -> smallest id
+    log(f"rmat scale={args.scale} n={n} m={len(a)} max_deg={deg.max()} src={src} ({time.time() - t0:.1f}s)")
+    t0 = time.time()
+    batches = disjoint_batches(ds, np, n, a, b, w, args.k, args.ins, args.warmup + args.steps, args.seed + 1)
+    log(f"{args.warmup + args.steps} batches of k={args.k} generated ({time.time() - t0:.1f}s)")
+    cfg = {"workload": f"rmat-s{args.scale}-ef{args.edge_factor} batch k={args.k} ins={args.ins:g}",
+           "scale": args.scale, "n": n, "m": len(a), "k": args.k, "ins_frac": args.ins, "src": src,
+           "weights": "U[1,1000]"}
+    run_plain(args, ds, np, n, a, b, w, src, batches, f"rmat-s{args.scale}", cfg)
+
+
+def config3(args, ds, np):
+    side = args.side
+    n = side * side
+    a, b, w = ds.gen_grid(side, 0.01, 1, 10000, seed=args.seed)
+    g0 = ds.Graph(n, a, b, w)
+    t0 = ds.Tree(g0, 0)
+    par0, _ = t0.read()
+    k = 100_000
+    batches = [ds.gen_changes_grid(side, a, b, w, par0, k, 0.7, 0.5, 3, 1, 10000, seed=args.seed + i)
+               for i in range(args.warmup + args.steps)]
+    # every step starts from the same graph and tree (restored outside the timing)
+    g, t = g0.clone(), ds.Tree(g0, 0)
+    stats = []
+    for i, c in enumerate(batches):
+        g.copy_from(g0)
+        t.copy_from(t0)
+        bt = ds.Batch(g, c)
+        ds.set_tuning(kernel_timing=0 if args.no_timing or i < args.warmup else 1)
+        st = ds.update(g, t, bt)
+        bt.close()
+        if i >= args.warmup:
+            stats.append(st)
+    ds.set_tuning(kernel_timing=0)
+    dev = sum(s["t_update_ms"] for s in stats) / 1e3
+    value = k * args.steps / dev
+    log(f"grid {side}: {value:.3e} changed edges/s (device time), p50 {pct([s['t_update_ms'] for s in stats], 50):.2f} ms"
+        f", rounds inv {stats[-1]['inv_rounds']} relax {stats[-1]['relax_rounds']}")
+    par, dist = t.read()
+    check = {}
+    cpu = None
+    if not args.no_cpu:
+        check, cpu = cpu_check(ds, np, g, n, [0], par, dist, k, f"grid-{side}")
+    line(args, value, dev, args.steps, {"workload": f"grid {side}x{side} drop 1% k=1e5 del 70% (half tree)",
+                                          "n": n, "m": len(a), "k": k, "src": 0,
+                                          "note": "value = k / device update time; state restored between steps"},
+         stats, check, cpu)
+
+
+def config4a(args, ds, np):
+    scale, K, k = min(args.scale, 22), args.trees, 1_000_000 if args.k == 1_000_000 else args.k
+    a, b, w = ds.gen_rmat(scale, 16, (0.57, 0.19, 0.19), 1, 1000, seed=args.seed)
+    n = 1 << scale
+    rng = np.random.default_rng(args.seed)
+    srcs = rng.integers(0, n, K).astype(np.int32)
+    batches = disjoint_batches(ds, np, n, a, b, w, k, 0.5, args.warmup + args.steps, args.seed + 1)
+    g = ds.Graph(n, a, b, w)
+    f = ds.Tree(g, sources=srcs)
+    res = [ds.Batch(g, c) for c in batches]
+    stats, wall = timed_steps(ds, g, f, res, args.warmup, not args.no_timing)
+    for r in res:
+        r.close()
+    # plain comparison: the same batches applied to one tree at a time (first 4 sources, extrapolated)
+    plain_ms = []
+    for j in range(min(4, K)):
+        gp = ds.Graph(n, a, b, w)
+        tp = ds.Tree(gp, int(srcs[j]))
+        for i, c in enumerate(batches):
+            st = ds.update(gp, tp, c)
+            if i >= args.warmup:
+                plain_ms.append(st["t_update_ms"])
+        if j == 0:
+            pp, pd = tp.read()
+            fp, fd = f.read(0)
+            same = bool(np.array_equal(pp, fp) and np.array_equal(pd, fd))
+        gp.close()
+    plain_per_tree = sum(plain_ms) / len(plain_ms)
+    tree_updates = k * K * args.steps
+    value = tree_updates / wall
+    log(f"forest K={K} s{scale}: {value:.3e} tree-edge updates/s; plain {plain_per_tree:.2f} ms/tree/batch")
+    line(args, value, wall, args.steps,
+         {"workload": f"rmat-s{scale} K={K} trees, shared batch k={k} ins=0.5", "n": n, "m": len(a), "k": k,
+          "trees": K}, stats, {"forest_equals_plain_tree0": same}, None, unit="changed edges x trees /s",
+         extra={"speedup_vs_plain": (plain_per_tree * K * args.steps / 1e3) / wall})
+
+
+def config4b(args, ds, np):
+    rng = np.random.default_rng(args.seed)
+    parts, off = [], 0
+    for i in range(args.graphs):
+        n_i = int(math.exp(rng.uniform(math.log(2 ** 14), math.log(2 ** 21))))
+        deg = math.exp(rng.uniform(math.log(4), math.log(32)))
+        m_i = max(n_i - 1, int(n_i * deg / 2))
+        a, b, w = ds.gen_tree_plus_random(n_i, m_i, 1, 1000, seed=args.seed + 10 * i)
+        c = ds.gen_changes(n_i, a, b, w, max(1, m_i // 100), 0.5, 1, 1000, seed=args.seed + 10 * i + 1)
+        parts.append((n_i, a, b, w, c, int(rng.integers(0, n_i)), off))
+        off += n_i
+    N = off
+    A = np.concatenate([p[1] + p[6] for p in parts])
+    B = np.concatenate([p[2] + p[6] for p in parts])
+    W = np.concatenate([p[3] for p in parts])
+    C = np.concatenate([p[4] + np.array([p[6], p[6], 0, 0], np.int32) for p in parts])
+    roots = np.array([p[5] + p[6] for p in parts], np.int32)
+    g0 = ds.Graph(N, A, B, W)
+    t0 = ds.Tree(g0, roots=roots)
+    g, t = g0.clone(), ds.Tree(g0, roots=roots)
+    stats = []
+    for i in range(args.warmup + args.steps):
+        g.copy_from(g0)
+        t.copy_from(t0)
+        bt = ds.Batch(g, C)
+        ds.set_tuning(kernel_timing=0 if args.no_timing or i < args.warmup else 1)
+        st = ds.update(g, t, bt)
+        bt.close()
+        if i >= args.warmup:
+            stats.append(st)
+    ds.set_tuning(kernel_timing=0)
+    dev = sum(s["t_update_ms"] for s in stats) / 1e3
+    # plain: each graph separately, one launch sequence per graph
+    plain = 0.0
+    par, dist = t.read()
+    same = True
+    for n_i, a, b, w, c, s, o in parts:
+        gp = ds.Graph(n_i, a, b, w)
+        tp = ds.Tree(gp, s)
+        st = ds.update(gp, tp, c)
+        plain += st["t_update_ms"]
+        pp, pd = tp.read()
+        sp = par[o:o + n_i].astype(np.int64)
+        sp[sp >= 0] -= o
+        same &= bool(np.array_equal(sp, pp) and np.array_equal(dist[o:o + n_i], pd))
+        gp.close()
+    value = len(C) * args.steps / dev
+    log(f"{args.graphs} graphs in one launch: {value:.3e} changed edges/s; plain total {plain:.2f} ms vs "
+        f"{dev / args.steps * 1e3:.2f} ms")
+    line(args, value, dev, args.steps, {"workload": f"{args.graphs} random graphs n~logU[2^14,2^21] 1% batches",
+                                          "N": N, "M": len(A), "k_total": len(C)}, stats,
+         {"each_equals_plain": same}, None,
+         extra={"speedup_vs_plain": plain / (dev / args.steps * 1e3)})
+
+
+def config5(args, ds, np):
+    rng = np.random.default_rng(args.seed)
+    scales = [20 + (i * 5) // args.instances for i in range(args.instances)]  # 20..24 spread evenly
+    if args.scale < 24:
+        scales = [min(s, args.scale) for s in scales]
+    srv = ds.Server(mem_budget_bytes=96 << 30)
+    insts = []
+    for i, sc in enumerate(scales):
+        a, b, w = ds.gen_rmat(sc, 16, (0.57, 0.19, 0.19), 1, 1000, seed=args.seed + i)
+        n = 1 << sc
+        deg = np.bincount(np.concatenate([a, b]), minlength=n)
+        src = int(np.argmax(deg))
+        insts.append((n, a, b, w, src))
+        srv.add_instance(n, a, b, w, src)
+        log(f"instance {i}: s{sc} m={len(a)}")
+    t0 = time.time()
+    srv.start()
+    log(f"server started ({time.time() - t0:.1f}s)")
+    reqs = []
+    for r in range(args.requests):
+        i = int(rng.integers(0, len(insts)))
+        n, a, b, w, _ = insts[i]
+        k = int(math.exp(rng.uniform(0, math.log(100_000))))
+        c = ds.gen_changes(n, a, b, w, k, float(rng.uniform(0, 1)), 1, 1000, seed=args.seed + 1000 + r)
+        reqs.append((i, c))
+    log("requests generated")
+    # served: Poisson arrivals (or all at once when rate == 0)
+    gaps = rng.exponential(1.0 / args.rate, len(reqs)) if args.rate > 0 else np.zeros(len(reqs))
+    tickets = []
+    t0 = time.perf_counter()
+    tnext = t0
+    for (i, c), gap in zip(reqs, gaps):
+        tnext += gap
+        d = tnext - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        tickets.append(srv.submit(i, c))
+    rs = [srv.wait(tk) for tk in tickets]
+    wall = time.perf_counter() - t0
+    total = sum(len(c) for _, c in reqs)
+    served = total / wall
+    lat = [r["latency_ms"] for r in rs]
+    # plain: the same requests one launch each, on per-instance graphs
+    plain_ms = 0.0
+    same = True
+    per = {}
+    for i in range(len(insts)):
+        n, a, b, w, src = insts[i]
+        gp = ds.Graph(n, a, b, w)
+        tp = ds.Tree(gp, src)
+        for j, c in reqs:
+            if j == i:
+                plain_ms += ds.update(gp, tp, c)["t_update_ms"]
+        pp, pd = tp.read()
+        sp, sd = srv.read(i)
+        same &= bool(np.array_equal(pp, sp) and np.array_equal(pd, sd))
+        gp.close()
+    plain = total / (plain_ms / 1e3)
+    info = srv.info()
+    srv.close()
+    log(f"served {served:.3e} vs plain {plain:.3e} changed edges/s; p50 {pct(lat, 50):.2f} p99 {pct(lat, 99):.2f} ms")
+    out = {"metric": METRIC, "value": served, "unit": "changed edges/s", "n_gpus": 1, "steps": len(reqs),
+           "warmup": 0, "ms_per_step": wall / len(reqs) * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+           "config": {"workload": f"served {len(insts)} rmat instances s{min(scales)}-s{max(scales)}, "
+                                  f"{len(reqs)} requests k~logU[1,1e5]", "rate_per_s": args.rate,
+                      "scales": scales},
+           "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "plain_changed_edges_per_s": plain,
+           "speedup_vs_plain": served / plain, "dist_bitexact": same, "check": {"each_equals_plain": same},
+           "server": info, "mean_merged": float(np.mean([r["merged"] for r in rs])),
+           "roofline": None, "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -74,105 +411,8 @@ def main():
     import numpy as np
 
     import dynamicssspv2_amd as ds
-
-    t0 = time.time()
-    a, b, w = ds.gen_rmat(args.scale, args.edge_factor, (0.57, 0.19, 0.19), 1, 1000, seed=args.seed)
-    n = 1 << args.scale
-    m = len(a)
-    deg = np.bincount(np.concatenate([a, b]), minlength=n)
-    src = int(np.argmax(deg))  # ties -> smallest id
-    log(f"rmat scale={args.scale} n={n} m={m} max_deg={deg.max()} src={src} ({time.time() - t0:.1f}s)")
-
-    count = args.warmup + args.steps
-    t0 = time.time()
-    batches = make_batches(ds, np, n, a, b, w, args.k, args.ins, count, args.seed + 1)
-    log(f"{count} batches of k={args.k} generated ({time.time() - t0:.1f}s)")
-
     ds.set_device(0)
-    t0 = time.time()
-    g = ds.Graph(n, a, b, w)
-    t = ds.Tree(g, src)
-    log(f"graph + initial tree on GPU ({time.time() - t0:.1f}s)")
-    res = [ds.Batch(g, c) for c in batches]
-    del batches
-
-    for i in range(args.warmup):
-        ds.update(g, t, res[i])
-    ds.set_tuning(kernel_timing=0 if args.no_timing else 1)
-    stats = []
-    t0 = time.perf_counter()
-    for i in range(args.warmup, count):
-        stats.append(ds.update(g, t, res[i]))
-    wall = time.perf_counter() - t0
-    ds.set_tuning(kernel_timing=0)
-    for r in res:
-        r.close()
-
-    lat = np.array([s["t_update_ms"] for s in stats])
-    changed = args.k * args.steps
-    value = changed / wall
-    alg = sum(s["alg_bytes"] for s in stats)
-    relax_bytes = sum(s["alg_bytes_relax"] for s in stats)
-    relax_ms = sum(s["t_relax_kernel_ms"] for s in stats)
-    relax_launches = sum(s["relax_launches"] for s in stats)
-    log(f"timed {args.steps} steps in {wall:.3f}s: {value:.3e} changed edges/s, p50 {np.percentile(lat, 50):.3f} ms")
-
-    # exactness: GPU tree vs a fresh GPU recompute and vs CPU Dijkstra on the final graph
-    par, dist = t.read()
-    t_fresh = ds.Tree(g, src)
-    fpar, fdist = t_fresh.read()
-    fresh_equal = bool(np.array_equal(dist, fdist) and np.array_equal(par, fpar))
-    check = {"gpu_recompute_equal": fresh_equal}
-    cpu = None
-    if not args.no_cpu:
-        import oracle
-        ea, eb, ew = g.export()
-        csr = oracle.CSR.from_edges(n, ea, eb, ew)
-        del ea, eb, ew
-        tc = time.perf_counter()
-        _, cd, pops = oracle.djk(csr, src, oracle.INF)
-        t_cpu = time.perf_counter() - tc
-        mism = int(np.sum(cd != dist))
-        code, where = oracle.certificate(csr, dist, par, [src])
-        canon = bool(np.array_equal(par, oracle.canonical_parents(csr, cd, [src], rule=0)))
-        check.update({"dijkstra_mismatches": mism, "certificate": int(code), "parents_canonical": canon})
-        cpu = {"value": args.k / t_cpu, "unit": "changed edges/s", "cores": 1, "kind": "port",
-               "sample": f"one from-scratch sequential Dijkstra (oracle/oracle.c orc_djk, restating "
-                         f"seqSSSPwithDist_undir.cpp:66-128) on the final scale-{args.scale} graph "
-                         f"({m} edges) = the CPU cost of one k={args.k} batch; {t_cpu:.2f} s"}
-        log(f"cpu djk {t_cpu:.2f}s, mismatches {mism}, certificate {code}, canonical {canon}")
-    exact = fresh_equal and (cpu is None or (check["dijkstra_mismatches"] == 0 and check["certificate"] == 0))
-
-    achieved = relax_bytes / (relax_ms / 1e3) / 1e9 if relax_ms > 0 else None
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "changed edges/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32",
-        "data": "synthetic",
-        "config": {"workload": f"rmat-s{args.scale}-ef{args.edge_factor} batch k={args.k} ins={args.ins:g}",
-                   "scale": args.scale, "n": n, "m": m, "k": args.k, "ins_frac": args.ins, "src": src,
-                   "weights": "U[1,1000]"},
-        "p50_ms": float(np.percentile(lat, 50)),
-        "p99_ms": float(np.percentile(lat, 99)),
-        "update_alg_GBps": alg / (lat.sum() / 1e3) / 1e9,
-        "dist_bitexact": exact,
-        "check": check,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBPS) if achieved else None, "traffic": None,
-                     "kernel": "k_frontier<M_RELAX>", "launches": relax_launches,
-                     "avg_launch_us": relax_ms / max(1, relax_launches) * 1e3,
-                     "alg_bytes_per_launch": relax_bytes / max(1, relax_launches)},
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
+    {"1": config1, "2": config2, "3": config3, "4a": config4a, "4b": config4b, "5": config5}[args.config](args, ds, np)
 
 
 if __name__ == "__main__":

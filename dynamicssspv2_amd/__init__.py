@@ -54,6 +54,28 @@ class Stats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("mem_budget_bytes", ctypes.c_uint64), ("max_merge", ctypes.c_int32), ("pad0", ctypes.c_int32),
+                ("max_records", ctypes.c_int64)]
+
+
+class RequestStats(ctypes.Structure):
+    _fields_ = [("instance", ctypes.c_int32), ("merged", ctypes.c_int32), ("records", ctypes.c_int64),
+                ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double), ("update_ms", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class ServerTotals(ctypes.Structure):
+    _fields_ = [("instances", ctypes.c_int32), ("pad0", ctypes.c_int32), ("budget_bytes", ctypes.c_uint64),
+                ("used_bytes", ctypes.c_uint64), ("launches", ctypes.c_int64), ("records", ctypes.c_int64),
+                ("requests", ctypes.c_int64), ("pending", ctypes.c_int64), ("busy_ms", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad0"}
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [(f, ctypes.c_int32) for f in ("block", "grid", "heavy_degree", "chunk", "round_batch",
                                               "kernel_timing")]
@@ -98,6 +120,8 @@ def lib():
                             pp, pp, pp], ctypes.c_int),
         "dsssp_gen_grid": ([i32, ctypes.c_double, i32, i32, u64, P64, pp, pp, pp], ctypes.c_int),
         "dsssp_gen_changes": ([i64, i64, p, p, p, i64, ctypes.c_double, i32, i32, i64, u64, p], ctypes.c_int),
+        "dsssp_gen_changes_grid": ([i32, i64, p, p, p, p, i64, ctypes.c_double, ctypes.c_double, i32, i32, i32,
+                                    u64, p], ctypes.c_int),
         "dsssp_graph_create": ([i64, i64, p, p, p, u32, pp], ctypes.c_int),
         "dsssp_graph_destroy": ([p], ctypes.c_int),
         "dsssp_graph_info": ([p, P64, P64, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
@@ -108,10 +132,22 @@ def lib():
         "dsssp_tree_destroy": ([p], ctypes.c_int),
         "dsssp_tree_read": ([p, p, p], ctypes.c_int),
         "dsssp_tree_copy": ([p, p], ctypes.c_int),
+        "dsssp_tree_create_roots": ([p, i32, p, u32, pp], ctypes.c_int),
+        "dsssp_forest_create": ([p, i32, p, u32, pp], ctypes.c_int),
+        "dsssp_tree_count": ([p, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+        "dsssp_forest_read": ([p, i32, p, p], ctypes.c_int),
         "dsssp_update": ([p, p, p, i64, ctypes.POINTER(Stats)], ctypes.c_int),
         "dsssp_batch_upload": ([p, p, i64, pp], ctypes.c_int),
         "dsssp_batch_destroy": ([p], ctypes.c_int),
         "dsssp_update_batch": ([p, p, p, ctypes.POINTER(Stats)], ctypes.c_int),
+        "dsssp_server_create": ([ctypes.POINTER(ServerConfig), pp], ctypes.c_int),
+        "dsssp_server_add_instance": ([p, i64, i64, p, p, p, i32, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+        "dsssp_server_start": ([p], ctypes.c_int),
+        "dsssp_server_submit": ([p, i32, p, i64, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+        "dsssp_server_wait": ([p, ctypes.c_uint64, ctypes.POINTER(RequestStats)], ctypes.c_int),
+        "dsssp_server_read": ([p, i32, p, p], ctypes.c_int),
+        "dsssp_server_info": ([p, ctypes.POINTER(ServerTotals)], ctypes.c_int),
+        "dsssp_server_destroy": ([p], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -224,6 +260,15 @@ def gen_changes(n: int, a, b, w, k: int, ins_frac: float, wlo: int, whi: int, se
     return out
 
 
+def gen_changes_grid(side: int, a, b, w, parent, k: int, del_frac: float = 0.7, tree_frac: float = 0.5,
+                     max_manhattan: int = 3, wlo: int = 1, whi: int = 10000, seed: int = 5) -> np.ndarray:
+    a, b, w, p = _i32(a), _i32(b), _i32(w), _i32(parent)
+    out = np.empty((k, 4), np.int32)
+    _check(lib().dsssp_gen_changes_grid(side, len(a), _ptr(a), _ptr(b), _ptr(w), _ptr(p), k, del_frac, tree_frac,
+                                        max_manhattan, wlo, whi, seed, _ptr(out)))
+    return out
+
+
 # ----------------------------------------------------------------------------- tuning
 
 def get_tuning() -> dict:
@@ -293,16 +338,27 @@ class Graph:
 
 
 class Tree:
-    """SSSP tree of a Graph, resident in HBM.
+    """SSSP tree(s) of a Graph, resident in HBM.
 
     ``Tree(g, src)`` computes it on the GPU from ``src``; ``Tree(g, parent=p,
-    dist=d)`` loads a given tree (``canonical=True`` re-derives parents)."""
+    dist=d)`` loads a given tree (``canonical=True`` re-derives parents);
+    ``Tree(g, roots=[...])`` grows one forest from several roots;
+    ``Tree(g, sources=[...])`` holds one tree per source (batched form)."""
 
-    def __init__(self, graph: Graph, src: int = 0, parent=None, dist=None, canonical: bool = False):
+    def __init__(self, graph: Graph, src: int = 0, parent=None, dist=None, canonical: bool = False,
+                 roots=None, sources=None):
         self.graph = graph
         h = ctypes.c_void_p()
         flags = TREE_CANONICAL if canonical else 0
-        if parent is None:
+        self.ntrees = 1
+        if sources is not None:
+            s = _i32(sources)
+            _check(lib().dsssp_forest_create(graph._h, len(s), _ptr(s), flags, ctypes.byref(h)))
+            self.ntrees = len(s)
+        elif roots is not None:
+            r = _i32(roots)
+            _check(lib().dsssp_tree_create_roots(graph._h, len(r), _ptr(r), flags, ctypes.byref(h)))
+        elif parent is None:
             _check(lib().dsssp_tree_create(graph._h, int(src), None, None, flags, ctypes.byref(h)))
         else:
             p = _i32(parent)
@@ -319,10 +375,10 @@ class Tree:
 
     __del__ = close
 
-    def read(self):
+    def read(self, tree: int = 0):
         par = np.empty(self.graph.n, np.int32)
         dist = np.empty(self.graph.n, np.int64)
-        _check(lib().dsssp_tree_read(self._h, _ptr(par), _ptr(dist)))
+        _check(lib().dsssp_forest_read(self._h, tree, _ptr(par), _ptr(dist)))
         return par, dist
 
     def copy_from(self, other: "Tree") -> None:
@@ -343,6 +399,57 @@ class Batch:
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:
             lib().dsssp_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    __del__ = close
+
+
+class Server:
+    """Served form: resident instances + continuous-batching request loop."""
+
+    def __init__(self, mem_budget_bytes: int, max_merge: int = 0, max_records: int = 0):
+        cfg = ServerConfig(mem_budget_bytes, max_merge, 0, max_records)
+        h = ctypes.c_void_p()
+        _check(lib().dsssp_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self.sizes = []
+
+    def add_instance(self, n: int, a, b, w, src: int) -> int:
+        a, b, w = _i32(a), _i32(b), _i32(w)
+        i = ctypes.c_int32()
+        _check(lib().dsssp_server_add_instance(self._h, n, len(a), _ptr(a), _ptr(b), _ptr(w), src, ctypes.byref(i)))
+        self.sizes.append(n)
+        return i.value
+
+    def start(self) -> None:
+        _check(lib().dsssp_server_start(self._h))
+
+    def submit(self, instance: int, changes) -> int:
+        rec = as_changes(changes)
+        t = ctypes.c_uint64()
+        _check(lib().dsssp_server_submit(self._h, instance, _ptr(rec), len(rec), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int) -> dict:
+        st = RequestStats()
+        _check(lib().dsssp_server_wait(self._h, ticket, ctypes.byref(st)))
+        return st.as_dict()
+
+    def read(self, instance: int):
+        n = self.sizes[instance]
+        par = np.empty(n, np.int32)
+        dist = np.empty(n, np.int64)
+        _check(lib().dsssp_server_read(self._h, instance, _ptr(par), _ptr(dist)))
+        return par, dist
+
+    def info(self) -> dict:
+        t = ServerTotals()
+        _check(lib().dsssp_server_info(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            lib().dsssp_server_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     __del__ = close

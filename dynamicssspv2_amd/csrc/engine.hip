@@ -115,6 +115,12 @@ struct dsssp_graph {
     bool zero_w = false;          // some weight was ever 0: parents need the tight-BFS repair
     hipEvent_t ev[8] = {};
     std::vector<hipEvent_t> kev;  // per-launch timing events (kernel_timing)
+    // Trees and batches attached to this graph.  dsssp_graph_destroy while any
+    // remain only marks the graph; the last detach releases it, so objects may
+    // be destroyed in any order.
+    int attached = 0;
+    bool destroy_pending = false;
+    std::mutex mu;
 
     GView view() const { return GView{rp, adj, dbl, pool}; }
 };
@@ -122,6 +128,9 @@ struct dsssp_graph {
 struct dsssp_tree {
     dsssp_graph *g = nullptr;
     int32_t src = -1;
+    int32_t ntrees = 1;         // trees sharing the graph (a forest of K sources)
+    uint32_t kshift = 0;        // slot = vertex << kshift | tree
+    uint64_t nslots = 0;        // n << kshift
     uint64_t *st = nullptr;
     uint32_t *stamp = nullptr;
     uint32_t *q[2] = {nullptr, nullptr};
@@ -163,7 +172,8 @@ FArgs base_args(dsssp_tree *t)
     a.stamp = t->stamp;
     a.lvl = t->lvl;
     a.best = t->best;
-    a.out_cap = (uint32_t)t->g->n;
+    a.out_cap = (uint32_t)t->nslots;
+    a.kshift = t->kshift;
     a.out_hcap = t->hcap;
     a.inv_total = &t->c->inv_total;
     a.invl = t->invl;
@@ -176,7 +186,7 @@ FArgs base_args(dsssp_tree *t)
 uint32_t next_epoch(dsssp_tree *t)
 {
     if (t->epoch >= 0xF0000000u) {  // stamps restart well before the counter wraps
-        launch_fill_u32(t->stamp, 0u, (uint64_t)t->g->n, t->g->stream);
+        launch_fill_u32(t->stamp, 0u, t->nslots, t->g->stream);
         t->epoch = 1;
     }
     return ++t->epoch;
@@ -260,10 +270,10 @@ int run_rounds(dsssp_tree *t, int mode, Ring &r, int64_t &rounds, double *kernel
 int ensure_pick_buffers(dsssp_tree *t)
 {
     if (!t->lvl) {
-        if (int s = dev_alloc((void **)&t->lvl, sizeof(uint32_t) * (size_t)t->g->n, "levels")) return s;
+        if (int s = dev_alloc((void **)&t->lvl, sizeof(uint32_t) * (size_t)t->nslots, "levels")) return s;
     }
     if (!t->best) {
-        if (int s = dev_alloc((void **)&t->best, sizeof(uint32_t) * (size_t)t->g->n, "parent picks")) return s;
+        if (int s = dev_alloc((void **)&t->best, sizeof(uint32_t) * (size_t)t->nslots, "parent picks")) return s;
     }
     return DSSSP_OK;
 }
@@ -275,28 +285,28 @@ int repick_parents(dsssp_tree *t, bool use_lvl, int64_t &rounds)
 {
     dsssp_graph *g = t->g;
     if (int s = ensure_pick_buffers(t)) return s;
-    const uint32_t n = (uint32_t)g->n;
+    const uint32_t ns = (uint32_t)t->nslots;
     const dsssp_tuning tu = tuning();
     if (use_lvl) {
         Ring r;
         zero_ring(t, r);
         uint32_t *oq, *oc;
         producer_out(t, r, oq, oc);
-        launch_lvl_init(n, t->st, t->lvl, t->best, t->stamp, next_epoch(t), oq, oc, n, t->c, g->stream);
+        launch_lvl_init(ns, t->kshift, t->st, t->lvl, t->best, t->stamp, next_epoch(t), oq, oc, ns, t->c, g->stream);
         advance(r);
         if (int s = run_rounds(t, M_TBFS, r, rounds, nullptr)) return s;
     } else {
-        launch_fill_u32(t->best, 0xFFFFFFFFu, n, g->stream);
+        launch_fill_u32(t->best, 0xFFFFFFFFu, ns, g->stream);
     }
     FArgs a = base_args(t);
     a.lvl = t->lvl;
     a.best = t->best;
     a.in_q = nullptr;
     a.in_cnt = nullptr;
-    a.in_n = n;
+    a.in_n = ns;
     a.use_lvl = use_lvl ? 1u : 0u;
     launch_frontier(M_PICK, a, std::max(1, tu.grid), g->stream);
-    launch_apply_best(n, t->st, t->best, t->c, g->stream);
+    launch_apply_best(ns, t->kshift, t->st, t->best, t->c, g->stream);
     HIP_TRY(hipGetLastError());
     return DSSSP_OK;
 }
@@ -304,7 +314,7 @@ int repick_parents(dsssp_tree *t, bool use_lvl, int64_t &rounds)
 int ensure_hq(dsssp_tree *t, uint64_t total_entries)
 {
     const dsssp_tuning tu = tuning();
-    const uint64_t need = 2 * total_entries / (uint64_t)std::max(64, tu.chunk) + 1024;
+    const uint64_t need = ((2 * total_entries / (uint64_t)std::max(64, tu.chunk)) << t->kshift) + 1024;
     if (need > 0xFFFFFFFFull) return fail(DSSSP_ERR_ARG, "heavy queue would exceed 2^32 entries; raise chunk");
     if (t->hcap >= need) return DSSSP_OK;
     dev_free(t->hq[0]);
@@ -381,6 +391,7 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
     DevGuard dg(g->device);
     hipStream_t s = g->stream;
     const uint32_t n = (uint32_t)g->n;
+    const uint32_t ns = (uint32_t)t->nslots;
     if (b->n_ins > 0) {
         if (int e = ensure_pool(g, 2 * (uint64_t)b->n_ins)) return e;
     }
@@ -408,7 +419,7 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
     if (b->n_del > 0) {
         uint32_t *oq, *oc;
         producer_out(t, r, oq, oc);
-        launch_classify(b->rec, b->k, g->view(), t->st, oq, oc, t->invl, &t->c->inv_total, n, t->c, s);
+        launch_classify(b->rec, b->k, t->kshift, g->view(), t->st, oq, oc, t->invl, &t->c->inv_total, ns, t->c, s);
         advance(r);
     }
     HIP_TRY(hipEventRecord(g->ev[2], s));
@@ -431,7 +442,8 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
             a.epoch = ep;
             launch_frontier(M_SEED, a, std::max(1, tu.grid), s);
         }
-        if (b->n_ins > 0) launch_ins_offer(b->rec, b->k, b->pos, g->pool, t->st, t->stamp, ep, oq, oc, n, t->c, s);
+        if (b->n_ins > 0)
+            launch_ins_offer(b->rec, b->k, t->kshift, b->pos, g->pool, t->st, t->stamp, ep, oq, oc, ns, t->c, s);
         advance(r);
     }
     HIP_TRY(hipEventRecord(g->ev[4], s));
@@ -555,6 +567,24 @@ extern "C" int dsssp_set_tuning(const dsssp_tuning *t)
     return DSSSP_OK;
 }
 
+static void graph_release(dsssp_graph *g);
+
+static void graph_attach(dsssp_graph *g)
+{
+    std::lock_guard<std::mutex> lk(g->mu);
+    ++g->attached;
+}
+
+static void graph_detach(dsssp_graph *g)
+{
+    bool release;
+    {
+        std::lock_guard<std::mutex> lk(g->mu);
+        release = --g->attached == 0 && g->destroy_pending;
+    }
+    if (release) graph_release(g);
+}
+
 static void graph_release(dsssp_graph *g)
 {
     if (!g) return;
@@ -632,7 +662,14 @@ extern "C" int dsssp_graph_create(int64_t n, int64_t m, const int32_t *a, const 
 extern "C" int dsssp_graph_destroy(dsssp_graph *g)
 {
     clear_error();
-    graph_release(g);
+    if (!g) return DSSSP_OK;
+    bool release;
+    {
+        std::lock_guard<std::mutex> lk(g->mu);
+        g->destroy_pending = true;
+        release = g->attached == 0;
+    }
+    if (release) graph_release(g);
     return DSSSP_OK;
 }
 
@@ -741,6 +778,7 @@ extern "C" int dsssp_graph_copy(dsssp_graph *dst, dsssp_graph *src)
 static void tree_release(dsssp_tree *t)
 {
     if (!t) return;
+    dsssp_graph *g = t->g;
     DevGuard dg(t->g->device);
     (void)hipStreamSynchronize(t->g->stream);
     dev_free(t->st);
@@ -755,13 +793,14 @@ static void tree_release(dsssp_tree *t)
     dev_free(t->c);
     if (t->c_host) (void)hipHostFree(t->c_host);
     delete t;
+    graph_detach(g);
 }
 
 static int tree_alloc(dsssp_tree *t)
 {
     dsssp_graph *g = t->g;
     DevGuard dg(g->device);
-    const size_t n = (size_t)g->n;
+    const size_t n = (size_t)t->nslots;
     if (int s = dev_alloc((void **)&t->st, sizeof(uint64_t) * n, "tree state")) return s;
     if (int s = dev_alloc((void **)&t->stamp, sizeof(uint32_t) * n, "stamps")) return s;
     if (int s = dev_alloc((void **)&t->q[0], sizeof(uint32_t) * n, "frontier")) return s;
@@ -776,14 +815,19 @@ static int tree_alloc(dsssp_tree *t)
     return DSSSP_OK;
 }
 
-extern "C" int dsssp_tree_create(dsssp_graph *g, int32_t src, const int32_t *parent, const int64_t *dist,
-                                 uint32_t flags, dsssp_tree **out)
+// Creates a tree object of `ntrees` trees (slot stride 2^kshift) and fills it:
+// from host arrays (one tree), or from scratch on the GPU from `roots`
+// (per_tree: root j belongs to tree j; otherwise all roots share tree 0).
+static int tree_make(dsssp_graph *g, int32_t ntrees, const std::vector<uint32_t> &roots, bool per_tree,
+                     const int32_t *parent, const int64_t *dist, uint32_t flags, dsssp_tree **out)
 {
-    clear_error();
-    if (!g || !out) return fail(DSSSP_ERR_ARG, "null argument");
     *out = nullptr;
     if (flags & ~DSSSP_TREE_CANONICAL) return fail(DSSSP_ERR_ARG, "unknown flags");
     const int64_t n = g->n;
+    uint32_t kshift = 0;
+    while ((1 << kshift) < ntrees) ++kshift;
+    if (((uint64_t)n << kshift) >= 0xFFFFFFFFull)
+        return fail(DSSSP_ERR_ARG, "n * trees must stay below 2^32 (n=%lld, trees=%d)", (long long)n, ntrees);
     std::vector<uint64_t> packed;
     if (parent) {
         if (!dist) return fail(DSSSP_ERR_ARG, "parent given without dist");
@@ -802,12 +846,17 @@ extern "C" int dsssp_tree_create(dsssp_graph *g, int32_t src, const int32_t *par
                 return fail(DSSSP_ERR_TREE, "root %lld has distance %lld, not 0", (long long)v, (long long)dist[v]);
             packed[v] = pack_state((uint32_t)dist[v], (uint32_t)p);
         }
-    } else if (src < 0 || src >= n) {
-        return fail(DSSSP_ERR_RANGE, "source %d outside [0,%lld)", src, (long long)n);
+    } else {
+        for (uint32_t r : roots)
+            if ((int64_t)r >= n) return fail(DSSSP_ERR_RANGE, "source %u outside [0,%lld)", r, (long long)n);
     }
     dsssp_tree *t = new dsssp_tree();
     t->g = g;
-    t->src = src;
+    graph_attach(g);
+    t->src = roots.empty() ? -1 : (int32_t)roots[0];
+    t->ntrees = ntrees;
+    t->kshift = kshift;
+    t->nslots = (uint64_t)n << kshift;
     if (int s = tree_alloc(t)) {
         tree_release(t);
         return s;
@@ -815,6 +864,7 @@ extern "C" int dsssp_tree_create(dsssp_graph *g, int32_t src, const int32_t *par
     DevGuard dg(g->device);
     int64_t rounds = 0;
     int s = DSSSP_OK;
+    uint32_t *d_roots = nullptr;
     launch_reset_counters(t->c, g->stream);
     if (parent) {
         if (hipMemcpyAsync(t->st, packed.data(), sizeof(uint64_t) * (size_t)n, hipMemcpyHostToDevice, g->stream) !=
@@ -822,15 +872,23 @@ extern "C" int dsssp_tree_create(dsssp_graph *g, int32_t src, const int32_t *par
             s = fail(DSSSP_ERR_HIP, "tree upload failed");
         if (!s && (flags & DSSSP_TREE_CANONICAL)) s = repick_parents(t, g->zero_w, rounds);
     } else {
-        // From scratch: every vertex unreachable but the source, then relax.
-        launch_fill_u64(t->st, ST_INF, (uint64_t)n, g->stream);
-        Ring r;
-        zero_ring(t, r);
-        uint32_t *oq, *oc;
-        producer_out(t, r, oq, oc);
-        launch_set_source(t->st, (uint32_t)src, t->stamp, next_epoch(t), oq, oc, g->stream);
-        advance(r);
-        s = run_rounds(t, M_RELAX, r, rounds, nullptr);
+        // From scratch: every slot unreachable but the roots, then relax.
+        launch_fill_u64(t->st, ST_INF, t->nslots, g->stream);
+        s = dev_alloc((void **)&d_roots, sizeof(uint32_t) * std::max<size_t>(roots.size(), 1), "roots");
+        if (!s && !roots.empty() &&
+            hipMemcpyAsync(d_roots, roots.data(), sizeof(uint32_t) * roots.size(), hipMemcpyHostToDevice,
+                           g->stream) != hipSuccess)
+            s = fail(DSSSP_ERR_HIP, "root upload failed");
+        if (!s) {
+            Ring r;
+            zero_ring(t, r);
+            uint32_t *oq, *oc;
+            producer_out(t, r, oq, oc);
+            launch_set_sources(t->st, d_roots, (uint32_t)roots.size(), t->kshift, per_tree, t->stamp, next_epoch(t),
+                               oq, oc, (uint32_t)t->nslots, t->c, g->stream);
+            advance(r);
+            s = run_rounds(t, M_RELAX, r, rounds, nullptr);
+        }
         if (!s && g->zero_w) s = repick_parents(t, true, rounds);
     }
     if (!s) {
@@ -840,11 +898,58 @@ extern "C" int dsssp_tree_create(dsssp_graph *g, int32_t src, const int32_t *par
         else
             s = finish_errors(t, 0);
     }
+    dev_free(d_roots);
     if (s) {
         tree_release(t);
         return s;
     }
     *out = t;
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_tree_create(dsssp_graph *g, int32_t src, const int32_t *parent, const int64_t *dist,
+                                 uint32_t flags, dsssp_tree **out)
+{
+    clear_error();
+    if (!g || !out) return fail(DSSSP_ERR_ARG, "null argument");
+    if (!parent && (src < 0 || src >= g->n))
+        return fail(DSSSP_ERR_RANGE, "source %d outside [0,%lld)", src, (long long)g->n);
+    std::vector<uint32_t> roots;
+    if (!parent) roots.push_back((uint32_t)src);
+    return tree_make(g, 1, roots, false, parent, dist, flags, out);
+}
+
+extern "C" int dsssp_tree_create_roots(dsssp_graph *g, int32_t nroots, const int32_t *roots, uint32_t flags,
+                                       dsssp_tree **out)
+{
+    clear_error();
+    if (!g || !out || nroots < 1 || !roots) return fail(DSSSP_ERR_ARG, "bad argument");
+    std::vector<uint32_t> r((size_t)nroots);
+    for (int32_t i = 0; i < nroots; ++i) {
+        if (roots[i] < 0 || roots[i] >= g->n) return fail(DSSSP_ERR_RANGE, "root %d outside [0,n)", roots[i]);
+        r[i] = (uint32_t)roots[i];
+    }
+    return tree_make(g, 1, r, false, nullptr, nullptr, flags, out);
+}
+
+extern "C" int dsssp_forest_create(dsssp_graph *g, int32_t ntrees, const int32_t *srcs, uint32_t flags,
+                                   dsssp_tree **out)
+{
+    clear_error();
+    if (!g || !out || ntrees < 1 || ntrees > 1024 || !srcs) return fail(DSSSP_ERR_ARG, "bad argument");
+    std::vector<uint32_t> r((size_t)ntrees);
+    for (int32_t i = 0; i < ntrees; ++i) {
+        if (srcs[i] < 0 || srcs[i] >= g->n) return fail(DSSSP_ERR_RANGE, "source %d outside [0,n)", srcs[i]);
+        r[i] = (uint32_t)srcs[i];
+    }
+    return tree_make(g, ntrees, r, true, nullptr, nullptr, flags, out);
+}
+
+extern "C" int dsssp_tree_count(const dsssp_tree *t, int32_t *ntrees)
+{
+    clear_error();
+    if (!t || !ntrees) return fail(DSSSP_ERR_ARG, "null argument");
+    *ntrees = t->ntrees;
     return DSSSP_OK;
 }
 
@@ -855,15 +960,21 @@ extern "C" int dsssp_tree_destroy(dsssp_tree *t)
     return DSSSP_OK;
 }
 
-extern "C" int dsssp_tree_read(dsssp_tree *t, int32_t *parent, int64_t *dist)
+extern "C" int dsssp_forest_read(dsssp_tree *t, int32_t tree, int32_t *parent, int64_t *dist)
 {
     clear_error();
     if (!t || !parent || !dist) return fail(DSSSP_ERR_ARG, "null argument");
+    if (tree < 0 || tree >= t->ntrees) return fail(DSSSP_ERR_ARG, "tree %d outside [0,%d)", tree, t->ntrees);
     DevGuard dg(t->g->device);
     const size_t n = (size_t)t->g->n;
-    std::vector<uint64_t> st(n);
+    const size_t stride = (size_t)1 << t->kshift;
     HIP_TRY(hipStreamSynchronize(t->g->stream));
-    HIP_TRY(hipMemcpy(st.data(), t->st, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+    std::vector<uint64_t> st(n);
+    if (stride == 1)
+        HIP_TRY(hipMemcpy(st.data(), t->st, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+    else
+        HIP_TRY(hipMemcpy2D(st.data(), sizeof(uint64_t), t->st + tree, sizeof(uint64_t) * stride, sizeof(uint64_t), n,
+                            hipMemcpyDeviceToHost));
     for (size_t v = 0; v < n; ++v) {
         if (st[v] == ST_INF) {
             parent[v] = -1;
@@ -876,16 +987,21 @@ extern "C" int dsssp_tree_read(dsssp_tree *t, int32_t *parent, int64_t *dist)
     return DSSSP_OK;
 }
 
+extern "C" int dsssp_tree_read(dsssp_tree *t, int32_t *parent, int64_t *dist)
+{
+    return dsssp_forest_read(t, 0, parent, dist);
+}
+
 extern "C" int dsssp_tree_copy(dsssp_tree *dst, dsssp_tree *src)
 {
     clear_error();
     if (!dst || !src) return fail(DSSSP_ERR_ARG, "null argument");
-    if (dst->g->n != src->g->n) return fail(DSSSP_ERR_ARG, "trees differ in size");
+    if (dst->nslots != src->nslots || dst->kshift != src->kshift) return fail(DSSSP_ERR_ARG, "trees differ in shape");
     DevGuard dg(dst->g->device);
     HIP_TRY(hipStreamSynchronize(src->g->stream));
-    const size_t n = (size_t)src->g->n;
-    HIP_TRY(hipMemcpyAsync(dst->st, src->st, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, dst->g->stream));
-    HIP_TRY(hipMemcpyAsync(dst->stamp, src->stamp, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, dst->g->stream));
+    const size_t ns = (size_t)src->nslots;
+    HIP_TRY(hipMemcpyAsync(dst->st, src->st, sizeof(uint64_t) * ns, hipMemcpyDeviceToDevice, dst->g->stream));
+    HIP_TRY(hipMemcpyAsync(dst->stamp, src->stamp, sizeof(uint32_t) * ns, hipMemcpyDeviceToDevice, dst->g->stream));
     HIP_TRY(hipStreamSynchronize(dst->g->stream));
     dst->epoch = std::max(dst->epoch, src->epoch);
     dst->src = src->src;
@@ -903,6 +1019,7 @@ extern "C" int dsssp_batch_upload(dsssp_graph *g, const dsssp_change *chg, int64
     DevGuard dg(g->device);
     dsssp_batch *b = new dsssp_batch();
     b->g = g;
+    graph_attach(g);
     b->k = k;
     b->n_ins = ni;
     b->n_del = nd;
@@ -927,13 +1044,17 @@ extern "C" int dsssp_batch_upload(dsssp_graph *g, const dsssp_change *chg, int64
 extern "C" int dsssp_batch_destroy(dsssp_batch *b)
 {
     if (!b) return DSSSP_OK;
-    DevGuard dg(b->g->device);
-    (void)hipStreamSynchronize(b->g->stream);
-    dev_free(b->rec);
-    dev_free(b->slot);
-    dev_free(b->pos);
-    dev_free(b->touched);
-    delete b;
+    dsssp_graph *g = b->g;
+    {
+        DevGuard dg(g->device);
+        (void)hipStreamSynchronize(g->stream);
+        dev_free(b->rec);
+        dev_free(b->slot);
+        dev_free(b->pos);
+        dev_free(b->touched);
+        delete b;
+    }
+    graph_detach(g);
     return DSSSP_OK;
 }
 

@@ -345,3 +345,78 @@ extern "C" int dsssp_gen_changes(int64_t n, int64_t m, const int32_t *a, const i
     }
     return DSSSP_OK;
 }
+
+extern "C" int dsssp_gen_changes_grid(int32_t side, int64_t m, const int32_t *a, const int32_t *b, const int32_t *w,
+                                      const int32_t *parent, int64_t k, double del_frac, double tree_frac,
+                                      int32_t max_manhattan, int32_t wlo, int32_t whi, uint64_t seed,
+                                      dsssp_change *out)
+{
+    clear_error();
+    if (side < 2 || m < 1 || !a || !b || !w || !parent || k < 0 || (k > 0 && !out) || del_frac < 0 ||
+        del_frac > 1 || tree_frac < 0 || tree_frac > 1 || max_manhattan < 1)
+        return fail(DSSSP_ERR_ARG, "bad argument");
+    if (int s = check_weights(wlo, whi)) return s;
+    const int64_t n = (int64_t)side * side;
+    const int64_t nd = llround((double)k * del_frac);
+    const int64_t ntree = llround((double)nd * tree_frac);
+    const int64_t ni = k - nd;
+    // edges of the current tree: (v, parent v) for v with a parent other than itself
+    std::vector<int64_t> tree_e, other_e;
+    {
+        std::vector<uint64_t> tk;
+        for (int64_t v = 0; v < n; ++v)
+            if (parent[v] >= 0 && parent[v] != v) tk.push_back(pair_key((uint32_t)v, (uint32_t)parent[v]));
+        std::sort(tk.begin(), tk.end());
+        for (int64_t i = 0; i < m; ++i) {
+            if (std::binary_search(tk.begin(), tk.end(), pair_key((uint32_t)a[i], (uint32_t)b[i])))
+                tree_e.push_back(i);
+            else
+                other_e.push_back(i);
+        }
+    }
+    if (ntree > (int64_t)tree_e.size() || nd - ntree > (int64_t)other_e.size())
+        return fail(DSSSP_ERR_ARG, "not enough edges to delete");
+    int64_t o = 0;
+    auto pick = [&](std::vector<int64_t> &pool, int64_t cnt, uint64_t stream) {
+        // partial Fisher-Yates: cnt distinct picks
+        for (int64_t i = 0; i < cnt; ++i) {
+            const int64_t j = i + (int64_t)bounded(rng_at(seed, stream, (uint64_t)i), (uint64_t)(pool.size() - i));
+            std::swap(pool[i], pool[j]);
+            const int64_t e = pool[i];
+            out[o++] = dsssp_change{a[e], b[e], w[e], 0};
+        }
+    };
+    pick(tree_e, ntree, 40);
+    pick(other_e, nd - ntree, 41);
+    // insertions: distinct absent pairs at Manhattan distance 1..max_manhattan
+    std::vector<uint64_t> ek((size_t)m);
+    for (int64_t i = 0; i < m; ++i) ek[i] = pair_key((uint32_t)a[i], (uint32_t)b[i]);
+    std::sort(ek.begin(), ek.end());
+    KeySet mine((size_t)std::max<int64_t>(ni, 1));
+    int64_t got = 0;
+    for (uint64_t d = 0; got < ni; ++d) {
+        if (d > (uint64_t)ni * 1000 + 100000) return fail(DSSSP_ERR_ARG, "could not find enough absent pairs");
+        const uint64_t r = rng_at(seed, 42, d);
+        const int64_t v = (int64_t)bounded(r, (uint64_t)n);
+        const int64_t r0 = v / side, c0 = v % side;
+        const int32_t L = max_manhattan;
+        const uint64_t r2 = mix64(r);
+        const int64_t dr = (int64_t)bounded(r2, (uint64_t)(2 * L + 1)) - L;
+        const int64_t rem = L - (dr < 0 ? -dr : dr);
+        const int64_t dc = (int64_t)bounded(mix64(r2), (uint64_t)(2 * rem + 1)) - rem;
+        if (dr == 0 && dc == 0) continue;
+        const int64_t r1 = r0 + dr, c1 = c0 + dc;
+        if (r1 < 0 || r1 >= side || c1 < 0 || c1 >= side) continue;
+        const uint32_t x = (uint32_t)v, y = (uint32_t)(r1 * side + c1);
+        const uint64_t key = pair_key(x, y);
+        if (std::binary_search(ek.begin(), ek.end(), key)) continue;
+        if (!mine.insert(key + 1)) continue;
+        out[o++] = dsssp_change{(int32_t)x, (int32_t)y, draw_weight(rng_at(seed, 43, (uint64_t)got), wlo, whi), 1};
+        ++got;
+    }
+    for (int64_t i = k - 1; i > 0; --i) {
+        const int64_t j = (int64_t)bounded(rng_at(seed, 44, (uint64_t)i), (uint64_t)i + 1);
+        std::swap(out[i], out[j]);
+    }
+    return DSSSP_OK;
+}

@@ -51,6 +51,7 @@ struct FArgs {
     uint32_t heavy;   // rows longer than this are deferred as chunks (loop modes)
     uint32_t chunk;   // entries per chunk, multiple of 64
     uint32_t use_lvl; // M_PICK: require lvl[parent] + 1 == lvl[v]
+    uint32_t kshift;  // log2 of the trees per vertex: state slot = vertex << kshift | tree
 };
 
 constexpr int FRONTIER_BLOCK = 256;
@@ -66,19 +67,22 @@ void launch_ins_scatter(const int4 *rec, int64_t k, const uint64_t *slot, const 
                         const uint32_t *tcnt, uint64_t *pool, uint64_t *pos, hipStream_t s);
 void launch_ins_reset(const uint32_t *touched, uint32_t max_touched, uint32_t *tcnt, GraphCtl *gc,
                       hipStream_t s);
-void launch_classify(const int4 *rec, int64_t k, GView g, uint64_t *st, uint32_t *out_q, uint32_t *out_cnt,
-                     uint32_t *invl, uint32_t *inv_total, uint32_t cap, Counters *c, hipStream_t s);
-void launch_ins_offer(const int4 *rec, int64_t k, const uint64_t *pos, const uint64_t *pool, uint64_t *st,
-                      uint32_t *stamp, uint32_t epoch, uint32_t *out_q, uint32_t *out_cnt, uint32_t cap,
+void launch_classify(const int4 *rec, int64_t k, uint32_t kshift, GView g, uint64_t *st, uint32_t *out_q,
+                     uint32_t *out_cnt, uint32_t *invl, uint32_t *inv_total, uint32_t cap, Counters *c, hipStream_t s);
+void launch_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint64_t *pos, const uint64_t *pool,
+                      uint64_t *st, uint32_t *stamp, uint32_t epoch, uint32_t *out_q, uint32_t *out_cnt, uint32_t cap,
                       Counters *c, hipStream_t s);
-void launch_set_source(uint64_t *st, uint32_t src, uint32_t *stamp, uint32_t epoch, uint32_t *q,
-                       uint32_t *cnt, hipStream_t s);
+// Roots: for j < count, vertex src[j] becomes a root (distance 0, parent itself)
+// of tree (per_tree ? j : 0) and is queued.
+void launch_set_sources(uint64_t *st, const uint32_t *src, uint32_t count, uint32_t kshift, bool per_tree,
+                        uint32_t *stamp, uint32_t epoch, uint32_t *q, uint32_t *cnt, uint32_t cap, Counters *c,
+                        hipStream_t s);
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
-void launch_lvl_init(uint32_t n, const uint64_t *st, uint32_t *lvl, uint32_t *best, uint32_t *stamp,
-                     uint32_t epoch, uint32_t *out_q, uint32_t *out_cnt, uint32_t cap, Counters *c,
+void launch_lvl_init(uint32_t nslots, uint32_t kshift, const uint64_t *st, uint32_t *lvl, uint32_t *best,
+                     uint32_t *stamp, uint32_t epoch, uint32_t *out_q, uint32_t *out_cnt, uint32_t cap, Counters *c,
                      hipStream_t s);
-void launch_apply_best(uint32_t n, uint64_t *st, const uint32_t *best, Counters *c, hipStream_t s);
-void launch_export_mark(uint32_t n, GView g, uint32_t *counts, hipStream_t s);
+void launch_apply_best(uint32_t nslots, uint32_t kshift, uint64_t *st, const uint32_t *best, Counters *c,
+                       hipStream_t s);
 
 }  // namespace dsssp

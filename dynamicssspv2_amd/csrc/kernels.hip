@@ -76,17 +76,20 @@ struct WaveStats {
     uint32_t enq;       // per lane
 };
 
-// One edge (owner u, entry ent).  Returns true when outv should be enqueued.
+// One edge of the owner slot os (vertex os >> kshift, tree os & mask) with
+// adjacency entry ent.  Returns true when slot outv should be enqueued.
 template <int MODE>
-__device__ __forceinline__ bool edge_op(const FArgs &A, uint32_t u, uint32_t aux0, uint32_t aux1, uint64_t ent,
+__device__ __forceinline__ bool edge_op(const FArgs &A, uint32_t os, uint32_t aux0, uint32_t aux1, uint64_t ent,
                                         uint32_t &outv, WaveStats &S)
 {
+    const uint32_t u = os >> A.kshift;
     const uint32_t x = e_col(ent);
     const uint32_t w = e_w(ent);
     if (x == u) return false;  // a self-loop never carries a shortest path
     if constexpr (MODE != M_INVAL) {
         if (w & W_DEAD) return false;
     }
+    const uint32_t xs = (x << A.kshift) | (os & ((1u << A.kshift) - 1u));
     if constexpr (MODE == M_RELAX) {
         const uint64_t nd = (uint64_t)aux0 + w;
         if (nd >= DIST_INF) {
@@ -94,25 +97,28 @@ __device__ __forceinline__ bool edge_op(const FArgs &A, uint32_t u, uint32_t aux
             return false;
         }
         const uint64_t cand = pack_state((uint32_t)nd, u);
-        if (cand >= A.st[x]) return false;  // possibly stale, only ever too large
-        const uint64_t old = atomicMin((unsigned long long *)&A.st[x], (unsigned long long)cand);
+        const uint64_t cur = A.st[xs];  // possibly stale, only ever too large
+        // A root (parent itself, distance 0) never changes: with zero weights a
+        // candidate (0, smaller id) would otherwise take its place.
+        if (cand >= cur || st_parent(cur) == x) return false;
+        const uint64_t old = atomicMin((unsigned long long *)&A.st[xs], (unsigned long long)cand);
         if (cand >= old) return false;
         S.improved++;
-        outv = x;
+        outv = xs;
         return true;
     } else if constexpr (MODE == M_INVAL) {
         // x is a child of u in the pre-update tree: reset it and descend.
         // Tombstoned entries are scanned too, so a child hanging off a deleted
         // parallel copy is still found.
-        const uint64_t sx = A.st[x];
+        const uint64_t sx = A.st[xs];
         if (sx == ST_INF || st_parent(sx) != u) return false;
-        if (atomicCAS((unsigned long long *)&A.st[x], (unsigned long long)sx, (unsigned long long)ST_INF) != sx)
+        if (atomicCAS((unsigned long long *)&A.st[xs], (unsigned long long)sx, (unsigned long long)ST_INF) != sx)
             return false;
-        outv = x;
+        outv = xs;
         return true;
     } else if constexpr (MODE == M_SEED) {
         // u was invalidated: take the best still-valid neighbour.
-        const uint64_t sx = A.st[x];
+        const uint64_t sx = A.st[xs];
         if (sx == ST_INF) return false;
         const uint64_t nd = (uint64_t)st_dist(sx) + w;
         if (nd >= DIST_INF) {
@@ -120,26 +126,26 @@ __device__ __forceinline__ bool edge_op(const FArgs &A, uint32_t u, uint32_t aux
             return false;
         }
         const uint64_t cand = pack_state((uint32_t)nd, x);
-        if (cand >= A.st[u]) return false;
-        const uint64_t old = atomicMin((unsigned long long *)&A.st[u], (unsigned long long)cand);
+        if (cand >= A.st[os]) return false;
+        const uint64_t old = atomicMin((unsigned long long *)&A.st[os], (unsigned long long)cand);
         if (cand >= old) return false;
-        outv = u;
+        outv = os;
         return true;
     } else if constexpr (MODE == M_TBFS) {
-        const uint64_t sx = A.st[x];
+        const uint64_t sx = A.st[xs];
         if (sx == ST_INF || st_parent(sx) == x) return false;
         if ((uint64_t)aux0 + w != (uint64_t)st_dist(sx)) return false;
         const uint32_t nl = aux1 + 1;
-        if (nl >= A.lvl[x]) return false;
-        if (nl >= atomicMin(&A.lvl[x], nl)) return false;
-        outv = x;
+        if (nl >= A.lvl[xs]) return false;
+        if (nl >= atomicMin(&A.lvl[xs], nl)) return false;
+        outv = xs;
         return true;
-    } else {  // M_PICK: u is the vertex whose parent is being chosen
-        const uint64_t sx = A.st[x];
+    } else {  // M_PICK: os is the slot whose parent is being chosen
+        const uint64_t sx = A.st[xs];
         if (sx == ST_INF) return false;
         if ((uint64_t)st_dist(sx) + w != (uint64_t)aux0) return false;
-        if (A.use_lvl && A.lvl[x] + 1 != aux1) return false;
-        if (x < A.best[u]) atomicMin(&A.best[u], x);
+        if (A.use_lvl && A.lvl[xs] + 1 != aux1) return false;
+        if (x < A.best[os]) atomicMin(&A.best[os], x);
         return false;
     }
 }
@@ -173,24 +179,24 @@ struct TileSmem {
     uint64_t b0[FW][64];
 };
 
-// Owner-side fields; returns false when the owner has nothing to scan.
+// Owner-side fields of slot os; returns false when the owner has nothing to scan.
 template <int MODE>
-__device__ __forceinline__ bool owner_prep(const FArgs &A, uint32_t u, uint32_t &aux0, uint32_t &aux1)
+__device__ __forceinline__ bool owner_prep(const FArgs &A, uint32_t os, uint32_t &aux0, uint32_t &aux1)
 {
     if constexpr (MODE == M_RELAX) {
-        const uint64_t su = A.st[u];
+        const uint64_t su = A.st[os];
         aux0 = st_dist(su);
         return su != ST_INF;
     } else if constexpr (MODE == M_TBFS) {
-        const uint64_t su = A.st[u];
+        const uint64_t su = A.st[os];
         aux0 = st_dist(su);
-        aux1 = A.lvl[u];
+        aux1 = A.lvl[os];
         return su != ST_INF && aux1 != LVL_INF;
     } else if constexpr (MODE == M_PICK) {
-        const uint64_t su = A.st[u];
+        const uint64_t su = A.st[os];
         aux0 = st_dist(su);
-        aux1 = A.use_lvl ? A.lvl[u] : 0u;
-        return su != ST_INF && st_parent(su) != u;
+        aux1 = A.use_lvl ? A.lvl[os] : 0u;
+        return su != ST_INF && st_parent(su) != (os >> A.kshift);
     } else {
         return true;
     }
@@ -202,22 +208,23 @@ __device__ void do_tile(const FArgs &A, uint32_t t, uint32_t L, TileSmem &sm, ui
     const uint32_t lane = lane_id();
     const uint32_t idx = t * 64u + lane;
     const bool valid = idx < L;
-    uint32_t u = 0, deg = 0, blen = 0, dbeg = 0, aux0 = 0, aux1 = 0;
+    uint32_t os = 0, deg = 0, blen = 0, dbeg = 0, aux0 = 0, aux1 = 0;
     uint64_t b0 = 0;
     if (valid) {
-        u = A.in_q ? A.in_q[idx] : idx;
+        os = A.in_q ? A.in_q[idx] : idx;
+        const uint32_t u = os >> A.kshift;
         b0 = A.g.rp[u];
         blen = (uint32_t)(A.g.rp[u + 1] - b0);
         const uint64_t dd = A.g.dbl[u];
         dbeg = (uint32_t)(dd >> 32);
         deg = blen + (uint32_t)dd;
-        if (!owner_prep<MODE>(A, u, aux0, aux1)) deg = 0;
+        if (!owner_prep<MODE>(A, os, aux0, aux1)) deg = 0;
         if constexpr (defers(MODE)) {
             if (deg > A.heavy) {
                 const uint32_t nch = (deg + A.chunk - 1) / A.chunk;
                 const uint32_t base = atomicAdd(A.out_hcnt, nch);
                 if (base + nch <= A.out_hcap) {
-                    for (uint32_t c = 0; c < nch; ++c) A.out_hq[base + c] = ((uint64_t)u << 32) | c;
+                    for (uint32_t c = 0; c < nch; ++c) A.out_hq[base + c] = ((uint64_t)os << 32) | c;
                 } else {
                     atomicOr(&A.c->err, ERR_QUEUE);
                 }
@@ -235,7 +242,7 @@ __device__ void do_tile(const FArgs &A, uint32_t t, uint32_t L, TileSmem &sm, ui
     const uint32_t total = __shfl(incl, 63);
     if (total == 0) return;
     sm.off[wv][lane] = incl - deg;
-    sm.u[wv][lane] = u;
+    sm.u[wv][lane] = os;
     sm.aux0[wv][lane] = aux0;
     sm.aux1[wv][lane] = aux1;
     sm.blen[wv][lane] = blen;
@@ -271,7 +278,8 @@ __device__ void do_chunk(const FArgs &A, uint32_t item, WaveStats &S)
 {
     const uint32_t lane = lane_id();
     const uint64_t hv = A.in_hq[item];
-    const uint32_t u = (uint32_t)(hv >> 32);
+    const uint32_t os = (uint32_t)(hv >> 32);
+    const uint32_t u = os >> A.kshift;
     const uint32_t c = (uint32_t)hv;
     const uint64_t b0 = A.g.rp[u];
     const uint32_t blen = (uint32_t)(A.g.rp[u + 1] - b0);
@@ -280,7 +288,7 @@ __device__ void do_chunk(const FArgs &A, uint32_t item, WaveStats &S)
     const uint32_t deg = blen + (uint32_t)dd;
     uint32_t aux0 = 0, aux1 = 0;
     if (c == 0) S.front += 1;
-    if (!owner_prep<MODE>(A, u, aux0, aux1)) return;
+    if (!owner_prep<MODE>(A, os, aux0, aux1)) return;
     const uint32_t start = c * A.chunk;
     const uint32_t end = min(start + A.chunk, deg);
     for (uint32_t jb = start; jb < end; jb += 64) {
@@ -289,7 +297,7 @@ __device__ void do_chunk(const FArgs &A, uint32_t item, WaveStats &S)
         uint32_t outv = 0;
         if (j < end) {
             const uint64_t ent = j < blen ? A.g.adj[b0 + j] : A.g.pool[(uint64_t)dbeg + (j - blen)];
-            want = edge_op<MODE>(A, u, aux0, aux1, ent, outv, S);
+            want = edge_op<MODE>(A, os, aux0, aux1, ent, outv, S);
         }
         finish_edge<MODE>(A, want, outv, S);
     }
@@ -440,35 +448,41 @@ __device__ __forceinline__ uint32_t tomb_row(GView g, uint32_t r, uint32_t c, ui
     return probes + len;
 }
 
-__global__ __launch_bounds__(RB) void k_classify(const int4 *rec, int64_t k, GView g, uint64_t *st,
+// Thread i handles record i >> kshift for tree i & mask; tree 0 also tombstones.
+__global__ __launch_bounds__(RB) void k_classify(const int4 *rec, int64_t k, uint32_t kshift, GView g, uint64_t *st,
                                                  uint32_t *out_q, uint32_t *out_cnt, uint32_t *invl,
                                                  uint32_t *inv_total, uint32_t cap, Counters *c)
 {
     const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
+    const int64_t ri = i >> kshift;
+    const uint32_t tree = (uint32_t)i & ((1u << kshift) - 1u);
     bool want = false;
     uint32_t victim = 0, probes = 0;
-    if (i < k) {
-        const int4 r = rec[i];
+    if (ri < k) {
+        const int4 r = rec[ri];
         if (r.w == 0) {
             const uint32_t a = (uint32_t)r.x, b = (uint32_t)r.y;
             if (a != b) {
                 // deleteEdge's tree-edge test (gpuFunctions_undir.cuh:189-198): the
                 // child endpoint of a tree edge roots an invalidated subtree.
-                const uint64_t sb = st[b];
-                if (sb != ST_INF && st_parent(sb) == a) {
-                    victim = b;
+                const uint32_t sa = (a << kshift) | tree, sb = (b << kshift) | tree;
+                const uint64_t vb = st[sb];
+                if (vb != ST_INF && st_parent(vb) == a) {
+                    victim = sb;
                     want = true;
                 } else {
-                    const uint64_t sa = st[a];
-                    if (sa != ST_INF && st_parent(sa) == b) {
-                        victim = a;
+                    const uint64_t va = st[sa];
+                    if (va != ST_INF && st_parent(va) == b) {
+                        victim = sa;
                         want = true;
                     }
                 }
                 if (want) want = atomicExch((unsigned long long *)&st[victim], (unsigned long long)ST_INF) != ST_INF;
             }
-            probes += tomb_row(g, a, b, (uint32_t)r.z);
-            if (a != b) probes += tomb_row(g, b, a, (uint32_t)r.z);
+            if (tree == 0) {
+                probes += tomb_row(g, a, b, (uint32_t)r.z);
+                if (a != b) probes += tomb_row(g, b, a, (uint32_t)r.z);
+            }
         }
     }
     wave_append(want, victim, out_q, out_cnt, cap, c);
@@ -479,47 +493,52 @@ __global__ __launch_bounds__(RB) void k_classify(const int4 *rec, int64_t k, GVi
 
 // Offers every inserted edge that survived the batch's deletions in both
 // directions (the role of insertEdge, gpuFunctions_undir.cuh:217-251).
-__global__ __launch_bounds__(RB) void k_ins_offer(const int4 *rec, int64_t k, const uint64_t *pos,
+__global__ __launch_bounds__(RB) void k_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint64_t *pos,
                                                   const uint64_t *pool, uint64_t *st, uint32_t *stamp,
                                                   uint32_t epoch, uint32_t *out_q, uint32_t *out_cnt,
                                                   uint32_t cap, Counters *c)
 {
     const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
+    const int64_t ri = i >> kshift;
+    const uint32_t tree = (uint32_t)i & ((1u << kshift) - 1u);
     bool wa = false, wb = false;
-    uint32_t a = 0, b = 0;
-    if (i < k) {
-        const int4 r = rec[i];
-        if (r.w == 1 && r.x != r.y && pos[i] != ~0ull && !(pool[pos[i] >> 32] & E_DEAD)) {
-            a = (uint32_t)r.x;
-            b = (uint32_t)r.y;
+    uint32_t sa_slot = 0, sb_slot = 0;
+    if (ri < k) {
+        const int4 r = rec[ri];
+        if (r.w == 1 && r.x != r.y && pos[ri] != ~0ull && !(pool[pos[ri] >> 32] & E_DEAD)) {
+            const uint32_t a = (uint32_t)r.x, b = (uint32_t)r.y;
+            sa_slot = (a << kshift) | tree;
+            sb_slot = (b << kshift) | tree;
             const uint32_t w = (uint32_t)r.z;
-            const uint64_t sa = st[a], sb = st[b];
-            if (sa != ST_INF) {
+            const uint64_t sa = st[sa_slot], sb = st[sb_slot];
+            const bool a_root = sa != ST_INF && st_parent(sa) == a;  // roots never change
+            const bool b_root = sb != ST_INF && st_parent(sb) == b;
+            if (sa != ST_INF && !b_root) {
                 const uint64_t nd = (uint64_t)st_dist(sa) + w;
                 if (nd >= DIST_INF) {
                     atomicOr(&c->err, ERR_DIST_OVERFLOW);
                 } else {
                     const uint64_t cand = pack_state((uint32_t)nd, a);
-                    if (cand < sb && cand < atomicMin((unsigned long long *)&st[b], (unsigned long long)cand))
+                    if (cand < sb && cand < atomicMin((unsigned long long *)&st[sb_slot], (unsigned long long)cand))
                         wb = true;
                 }
             }
-            if (sb != ST_INF) {
+            if (sb != ST_INF && !a_root) {
                 const uint64_t nd = (uint64_t)st_dist(sb) + w;
                 if (nd >= DIST_INF) {
                     atomicOr(&c->err, ERR_DIST_OVERFLOW);
                 } else {
                     const uint64_t cand = pack_state((uint32_t)nd, b);
-                    if (cand < sa && cand < atomicMin((unsigned long long *)&st[a], (unsigned long long)cand))
+                    if (cand < sa && cand < atomicMin((unsigned long long *)&st[sa_slot], (unsigned long long)cand))
                         wa = true;
                 }
             }
-            if (wa && atomicExch(&stamp[a], epoch) == epoch) wa = false;
-            if (wb && atomicExch(&stamp[b], epoch) == epoch) wb = false;
+            if (wa && atomicExch(&stamp[sa_slot], epoch) == epoch) wa = false;
+            if (wb && atomicExch(&stamp[sb_slot], epoch) == epoch) wb = false;
         }
     }
-    wave_append(wa, a, out_q, out_cnt, cap, c);
-    wave_append(wb, b, out_q, out_cnt, cap, c);
+    wave_append(wa, sa_slot, out_q, out_cnt, cap, c);
+    wave_append(wb, sb_slot, out_q, out_cnt, cap, c);
 }
 
 // ---------------------------------------------------------------------------
@@ -542,14 +561,21 @@ __global__ void k_reset_graphctl(GraphCtl *gc)
     }
 }
 
-__global__ void k_set_source(uint64_t *st, uint32_t src, uint32_t *stamp, uint32_t epoch, uint32_t *q, uint32_t *cnt)
+__global__ __launch_bounds__(RB) void k_set_sources(uint64_t *st, const uint32_t *src, uint32_t count,
+                                                    uint32_t kshift, uint32_t per_tree, uint32_t *stamp,
+                                                    uint32_t epoch, uint32_t *q, uint32_t *cnt, uint32_t cap,
+                                                    Counters *c)
 {
-    if (threadIdx.x == 0) {
-        st[src] = pack_state(0u, src);
-        stamp[src] = epoch;
-        q[0] = src;
-        *cnt = 1;
+    const uint32_t j = blockIdx.x * RB + threadIdx.x;
+    bool want = false;
+    uint32_t slot = 0;
+    if (j < count) {
+        const uint32_t v = src[j];
+        slot = (v << kshift) | (per_tree ? j : 0u);
+        st[slot] = pack_state(0u, v);
+        want = atomicExch(&stamp[slot], epoch) != epoch;
     }
+    wave_append(want, slot, q, cnt, cap, c);
 }
 
 __global__ __launch_bounds__(RB) void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n)
@@ -563,34 +589,35 @@ __global__ __launch_bounds__(RB) void k_fill_u32(uint32_t *p, uint32_t v, uint64
 }
 
 // Roots (parent == self) start the tight-graph BFS at level 0.
-__global__ __launch_bounds__(RB) void k_lvl_init(uint32_t n, const uint64_t *st, uint32_t *lvl, uint32_t *best,
-                                                 uint32_t *stamp, uint32_t epoch, uint32_t *out_q,
+__global__ __launch_bounds__(RB) void k_lvl_init(uint32_t nslots, uint32_t kshift, const uint64_t *st, uint32_t *lvl,
+                                                 uint32_t *best, uint32_t *stamp, uint32_t epoch, uint32_t *out_q,
                                                  uint32_t *out_cnt, uint32_t cap, Counters *c)
 {
-    const uint32_t v = blockIdx.x * RB + threadIdx.x;
+    const uint32_t s = blockIdx.x * RB + threadIdx.x;
     bool root = false;
-    if (v < n) {
-        const uint64_t s = st[v];
-        root = s != ST_INF && st_parent(s) == v;
-        lvl[v] = root ? 0u : LVL_INF;
-        best[v] = 0xFFFFFFFFu;
-        if (root) stamp[v] = epoch;
+    if (s < nslots) {
+        const uint64_t v = st[s];
+        root = v != ST_INF && st_parent(v) == (s >> kshift);
+        lvl[s] = root ? 0u : LVL_INF;
+        best[s] = 0xFFFFFFFFu;
+        if (root) stamp[s] = epoch;
     }
-    wave_append(root, v, out_q, out_cnt, cap, c);
+    wave_append(root, s, out_q, out_cnt, cap, c);
 }
 
-__global__ __launch_bounds__(RB) void k_apply_best(uint32_t n, uint64_t *st, const uint32_t *best, Counters *c)
+__global__ __launch_bounds__(RB) void k_apply_best(uint32_t nslots, uint32_t kshift, uint64_t *st, const uint32_t *best,
+                                                   Counters *c)
 {
-    const uint32_t v = blockIdx.x * RB + threadIdx.x;
-    if (v >= n) return;
-    const uint64_t s = st[v];
-    if (s == ST_INF || st_parent(s) == v) return;
-    const uint32_t b = best[v];
+    const uint32_t s = blockIdx.x * RB + threadIdx.x;
+    if (s >= nslots) return;
+    const uint64_t v = st[s];
+    if (v == ST_INF || st_parent(v) == (s >> kshift)) return;
+    const uint32_t b = best[s];
     if (b == 0xFFFFFFFFu) {
         atomicOr(&c->err, ERR_NO_TIGHT);
         return;
     }
-    st[v] = pack_state(st_dist(s), b);
+    st[s] = pack_state(st_dist(v), b);
 }
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + RB - 1) / RB); }
@@ -642,27 +669,30 @@ void launch_ins_reset(const uint32_t *touched, uint32_t max_touched, uint32_t *t
         hipLaunchKernelGGL(k_ins_reset, dim3(blocks_for(max_touched)), dim3(RB), 0, s, touched, max_touched, tcnt, gc);
 }
 
-void launch_classify(const int4 *rec, int64_t k, GView g, uint64_t *st, uint32_t *out_q, uint32_t *out_cnt,
-                     uint32_t *invl, uint32_t *inv_total, uint32_t cap, Counters *c, hipStream_t s)
+void launch_classify(const int4 *rec, int64_t k, uint32_t kshift, GView g, uint64_t *st, uint32_t *out_q,
+                     uint32_t *out_cnt, uint32_t *invl, uint32_t *inv_total, uint32_t cap, Counters *c, hipStream_t s)
 {
     if (k > 0)
-        hipLaunchKernelGGL(k_classify, dim3(blocks_for(k)), dim3(RB), 0, s, rec, k, g, st, out_q, out_cnt, invl,
-                           inv_total, cap, c);
+        hipLaunchKernelGGL(k_classify, dim3(blocks_for(k << kshift)), dim3(RB), 0, s, rec, k, kshift, g, st, out_q,
+                           out_cnt, invl, inv_total, cap, c);
 }
 
-void launch_ins_offer(const int4 *rec, int64_t k, const uint64_t *pos, const uint64_t *pool, uint64_t *st,
-                      uint32_t *stamp, uint32_t epoch, uint32_t *out_q, uint32_t *out_cnt, uint32_t cap, Counters *c,
-                      hipStream_t s)
+void launch_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint64_t *pos, const uint64_t *pool,
+                      uint64_t *st, uint32_t *stamp, uint32_t epoch, uint32_t *out_q, uint32_t *out_cnt, uint32_t cap,
+                      Counters *c, hipStream_t s)
 {
     if (k > 0)
-        hipLaunchKernelGGL(k_ins_offer, dim3(blocks_for(k)), dim3(RB), 0, s, rec, k, pos, pool, st, stamp, epoch, out_q,
-                           out_cnt, cap, c);
+        hipLaunchKernelGGL(k_ins_offer, dim3(blocks_for(k << kshift)), dim3(RB), 0, s, rec, k, kshift, pos, pool, st,
+                           stamp, epoch, out_q, out_cnt, cap, c);
 }
 
-void launch_set_source(uint64_t *st, uint32_t src, uint32_t *stamp, uint32_t epoch, uint32_t *q, uint32_t *cnt,
-                       hipStream_t s)
+void launch_set_sources(uint64_t *st, const uint32_t *src, uint32_t count, uint32_t kshift, bool per_tree,
+                        uint32_t *stamp, uint32_t epoch, uint32_t *q, uint32_t *cnt, uint32_t cap, Counters *c,
+                        hipStream_t s)
 {
-    hipLaunchKernelGGL(k_set_source, dim3(1), dim3(64), 0, s, st, src, stamp, epoch, q, cnt);
+    if (count > 0)
+        hipLaunchKernelGGL(k_set_sources, dim3(blocks_for(count)), dim3(RB), 0, s, st, src, count, kshift,
+                           per_tree ? 1u : 0u, stamp, epoch, q, cnt, cap, c);
 }
 
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s)
@@ -679,17 +709,19 @@ void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s)
     hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(RB), 0, s, p, v, n);
 }
 
-void launch_lvl_init(uint32_t n, const uint64_t *st, uint32_t *lvl, uint32_t *best, uint32_t *stamp, uint32_t epoch,
-                     uint32_t *out_q, uint32_t *out_cnt, uint32_t cap, Counters *c, hipStream_t s)
+void launch_lvl_init(uint32_t nslots, uint32_t kshift, const uint64_t *st, uint32_t *lvl, uint32_t *best,
+                     uint32_t *stamp, uint32_t epoch, uint32_t *out_q, uint32_t *out_cnt, uint32_t cap, Counters *c,
+                     hipStream_t s)
 {
-    if (n > 0)
-        hipLaunchKernelGGL(k_lvl_init, dim3(blocks_for(n)), dim3(RB), 0, s, n, st, lvl, best, stamp, epoch, out_q,
-                           out_cnt, cap, c);
+    if (nslots > 0)
+        hipLaunchKernelGGL(k_lvl_init, dim3(blocks_for(nslots)), dim3(RB), 0, s, nslots, kshift, st, lvl, best, stamp,
+                           epoch, out_q, out_cnt, cap, c);
 }
 
-void launch_apply_best(uint32_t n, uint64_t *st, const uint32_t *best, Counters *c, hipStream_t s)
+void launch_apply_best(uint32_t nslots, uint32_t kshift, uint64_t *st, const uint32_t *best, Counters *c,
+                       hipStream_t s)
 {
-    if (n > 0) hipLaunchKernelGGL(k_apply_best, dim3(blocks_for(n)), dim3(RB), 0, s, n, st, best, c);
+    if (nslots > 0) hipLaunchKernelGGL(k_apply_best, dim3(blocks_for(nslots)), dim3(RB), 0, s, nslots, kshift, st, best, c);
 }
 
 }  // namespace dsssp

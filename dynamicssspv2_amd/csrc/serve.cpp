@@ -1,0 +1,344 @@
+// serve.cpp -- served form: resident instances and a continuous-batching
+// request loop on one GPU.
+//
+// All admitted instances live in ONE device graph (vertex ranges laid end to
+// end) with ONE tree object holding one root per instance, so a merged update
+// that carries batches for several instances runs their work in the same
+// kernel launches.  A request loop thread takes, in arrival order, the oldest
+// pending request of every instance that has one (FIFO per instance), up to
+// max_merge requests / max_records records, concatenates them with vertex
+// offsets, uploads and applies them as one update, then wakes the waiters.
+// Requests that arrive while a merged update is on the GPU are picked up by
+// the next one.  Components never share vertices or edges, so each instance's
+// result is exactly what it would be alone.
+// The reference has no counterpart (one graph and one batch per process,
+// SSSPmainForUndirected.cu:27-335).
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/dsssp.h"
+#include "host.h"
+
+using namespace dsssp;
+using sclock = std::chrono::steady_clock;
+
+namespace {
+
+struct Instance {
+    int64_t n = 0, m = 0, offset = 0;
+    std::vector<int32_t> a, b, w;
+    int32_t src = 0;
+    uint64_t bytes = 0;
+};
+
+struct Request {
+    uint64_t ticket = 0;
+    int32_t inst = 0;
+    std::vector<dsssp_change> chg;
+    sclock::time_point submit, start, finish;
+    int status = DSSSP_OK;
+    std::string error;
+    int32_t merged = 0;
+    double update_ms = 0;
+    bool done = false;
+};
+
+// Device bytes of an instance: CSR + delta descriptors/counters + an equal
+// share of the delta pool + tree state, stamps, queues, invalidation list.
+uint64_t instance_bytes(int64_t n, int64_t m)
+{
+    const uint64_t E = 2 * (uint64_t)m;
+    return 8 * ((uint64_t)n + 1) + 8 * E + 12 * (uint64_t)n + 8 * std::max<uint64_t>(E / 4, 1024) +
+           24 * (uint64_t)n + 16 * (E / 512 + 1);
+}
+
+}  // namespace
+
+struct dsssp_server {
+    dsssp_server_config cfg{};
+    std::vector<Instance> inst;
+    uint64_t used = 0;
+    dsssp_graph *g = nullptr;
+    dsssp_tree *t = nullptr;
+    int device = 0;
+    bool started = false, stopping = false;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request *> pending;
+    std::map<uint64_t, Request *> all;
+    uint64_t next_ticket = 1;
+    std::thread loop;
+    // totals
+    int64_t launches = 0, records = 0, requests_done = 0;
+    double busy_ms = 0;
+};
+
+namespace {
+
+void serve_loop(dsssp_server *s)
+{
+    (void)dsssp_set_device(s->device);
+    for (;;) {
+        std::vector<Request *> pick;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [&] { return s->stopping || !s->pending.empty(); });
+            if (s->pending.empty() && s->stopping) return;
+            std::vector<char> taken(s->inst.size(), 0);
+            int64_t recs = 0;
+            for (auto it = s->pending.begin(); it != s->pending.end();) {
+                Request *r = *it;
+                const bool room = (s->cfg.max_merge <= 0 || (int32_t)pick.size() < s->cfg.max_merge) &&
+                                  (pick.empty() || s->cfg.max_records <= 0 ||
+                                   recs + (int64_t)r->chg.size() <= s->cfg.max_records);
+                if (!taken[r->inst] && room) {
+                    taken[r->inst] = 1;
+                    recs += (int64_t)r->chg.size();
+                    pick.push_back(r);
+                    it = s->pending.erase(it);
+                } else {
+                    taken[r->inst] = 1;  // keeps per-instance FIFO order
+                    ++it;
+                }
+            }
+        }
+        const auto t0 = sclock::now();
+        std::vector<dsssp_change> merged;
+        for (Request *r : pick) {
+            r->start = t0;
+            const int32_t off = (int32_t)s->inst[r->inst].offset;
+            for (dsssp_change c : r->chg) {
+                if (c.op == 0 || c.op == 1) {
+                    c.a += off;
+                    c.b += off;
+                }
+                merged.push_back(c);
+            }
+        }
+        dsssp_stats st{};
+        const int rc = dsssp_update(s->g, s->t, merged.data(), (int64_t)merged.size(), &st);
+        const std::string err = rc ? dsssp_last_error() : std::string();
+        const auto t1 = sclock::now();
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (Request *r : pick) {
+                r->finish = t1;
+                r->status = rc;
+                r->error = err;
+                r->merged = (int32_t)pick.size();
+                r->update_ms = st.t_update_ms;
+                r->done = true;
+            }
+            s->launches += 1;
+            s->records += (int64_t)merged.size();
+            s->requests_done += (int64_t)pick.size();
+            s->busy_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+        }
+        s->cv_done.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" int dsssp_server_create(const dsssp_server_config *cfg, dsssp_server **out)
+{
+    clear_error();
+    if (!cfg || !out) return fail(DSSSP_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (cfg->mem_budget_bytes == 0) return fail(DSSSP_ERR_ARG, "memory budget must be positive");
+    dsssp_server *s = new dsssp_server();
+    s->cfg = *cfg;
+    *out = s;
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_server_add_instance(dsssp_server *s, int64_t n, int64_t m, const int32_t *a, const int32_t *b,
+                                         const int32_t *w, int32_t src, int32_t *id)
+{
+    clear_error();
+    if (!s || !id || n < 1 || m < 0 || (m > 0 && (!a || !b || !w))) return fail(DSSSP_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->started) return fail(DSSSP_ERR_STATE, "instances are admitted before dsssp_server_start");
+    if (src < 0 || src >= n) return fail(DSSSP_ERR_RANGE, "source %d outside [0,%lld)", src, (long long)n);
+    const uint64_t bytes = instance_bytes(n, m);
+    if (s->used + bytes > s->cfg.mem_budget_bytes)
+        return fail(DSSSP_ERR_BUDGET, "instance needs %llu bytes, %llu of %llu left", (unsigned long long)bytes,
+                    (unsigned long long)(s->cfg.mem_budget_bytes - s->used),
+                    (unsigned long long)s->cfg.mem_budget_bytes);
+    int64_t total = 0;
+    for (auto &x : s->inst) total += x.n;
+    if (total + n > 0x7FFFFFFF) return fail(DSSSP_ERR_BUDGET, "more than 2^31-1 vertices in total");
+    Instance I;
+    I.n = n;
+    I.m = m;
+    I.offset = total;
+    I.a.assign(a, a + m);
+    I.b.assign(b, b + m);
+    I.w.assign(w, w + m);
+    I.src = src;
+    I.bytes = bytes;
+    s->used += bytes;
+    s->inst.push_back(std::move(I));
+    *id = (int32_t)s->inst.size() - 1;
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_server_start(dsssp_server *s)
+{
+    clear_error();
+    if (!s) return fail(DSSSP_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->started) return fail(DSSSP_ERR_STATE, "already started");
+    if (s->inst.empty()) return fail(DSSSP_ERR_STATE, "no instance admitted");
+    int64_t N = 0, M = 0;
+    for (auto &x : s->inst) N += x.n, M += x.m;
+    std::vector<int32_t> A((size_t)M), B((size_t)M), W((size_t)M), roots;
+    int64_t e = 0;
+    for (auto &x : s->inst) {
+        const int32_t off = (int32_t)x.offset;
+        for (int64_t i = 0; i < x.m; ++i, ++e) {
+            A[e] = x.a[i] + off;
+            B[e] = x.b[i] + off;
+            W[e] = x.w[i];
+        }
+        roots.push_back(x.src + off);
+        std::vector<int32_t>().swap(x.a);
+        std::vector<int32_t>().swap(x.b);
+        std::vector<int32_t>().swap(x.w);
+    }
+    int rc = dsssp_graph_create(N, M, A.data(), B.data(), W.data(), DSSSP_GRAPH_DEFAULT, &s->g);
+    if (rc) return rc;
+    rc = dsssp_tree_create_roots(s->g, (int32_t)roots.size(), roots.data(), DSSSP_TREE_DEFAULT, &s->t);
+    if (rc) {
+        std::string keep = dsssp_last_error();
+        dsssp_graph_destroy(s->g);
+        s->g = nullptr;
+        return fail(rc, "%s", keep.c_str());
+    }
+    s->started = true;
+    s->loop = std::thread(serve_loop, s);
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_server_submit(dsssp_server *s, int32_t id, const dsssp_change *chg, int64_t k, uint64_t *ticket)
+{
+    clear_error();
+    if (!s || !ticket || k < 0 || (k > 0 && !chg)) return fail(DSSSP_ERR_ARG, "bad argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (!s->started || s->stopping) return fail(DSSSP_ERR_STATE, "server not running");
+    if (id < 0 || id >= (int32_t)s->inst.size()) return fail(DSSSP_ERR_ARG, "unknown instance %d", id);
+    const int64_t n = s->inst[id].n;
+    for (int64_t i = 0; i < k; ++i) {
+        const dsssp_change &c = chg[i];
+        if (c.op != 0 && c.op != 1) continue;
+        if (c.a < 0 || c.a >= n || c.b < 0 || c.b >= n)
+            return fail(DSSSP_ERR_RANGE, "change %lld: vertex outside [0,%lld) of instance %d", (long long)i,
+                        (long long)n, id);
+        if (c.w < 0) return fail(DSSSP_ERR_WEIGHT, "change %lld: negative weight", (long long)i);
+    }
+    Request *r = new Request();
+    r->ticket = s->next_ticket++;
+    r->inst = id;
+    r->chg.assign(chg, chg + k);
+    r->submit = sclock::now();
+    s->pending.push_back(r);
+    s->all[r->ticket] = r;
+    *ticket = r->ticket;
+    lk.unlock();
+    s->cv_work.notify_one();
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_server_wait(dsssp_server *s, uint64_t ticket, dsssp_request_stats *st)
+{
+    clear_error();
+    if (!s) return fail(DSSSP_ERR_ARG, "null argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end()) return fail(DSSSP_ERR_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+    Request *r = it->second;
+    s->cv_done.wait(lk, [&] { return r->done; });
+    const int rc = r->status;
+    const std::string err = r->error;
+    if (st) {
+        st->instance = r->inst;
+        st->records = (int64_t)r->chg.size();
+        st->merged = r->merged;
+        st->queue_ms = std::chrono::duration<double, std::milli>(r->start - r->submit).count();
+        st->latency_ms = std::chrono::duration<double, std::milli>(r->finish - r->submit).count();
+        st->update_ms = r->update_ms;
+    }
+    s->all.erase(it);
+    delete r;
+    lk.unlock();
+    if (rc) return fail(rc, "%s", err.c_str());
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_server_read(dsssp_server *s, int32_t id, int32_t *parent, int64_t *dist)
+{
+    clear_error();
+    if (!s || !parent || !dist) return fail(DSSSP_ERR_ARG, "null argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (!s->started) return fail(DSSSP_ERR_STATE, "server not started");
+    if (id < 0 || id >= (int32_t)s->inst.size()) return fail(DSSSP_ERR_ARG, "unknown instance %d", id);
+    // wait until no request of this instance is pending or running
+    s->cv_done.wait(lk, [&] {
+        for (auto &kv : s->all)
+            if (kv.second->inst == id && !kv.second->done) return false;
+        return true;
+    });
+    const Instance &I = s->inst[id];
+    int64_t N = 0;
+    for (auto &x : s->inst) N += x.n;
+    std::vector<int32_t> par((size_t)N);
+    std::vector<int64_t> d((size_t)N);
+    const int rc = dsssp_tree_read(s->t, par.data(), d.data());
+    if (rc) return rc;
+    for (int64_t v = 0; v < I.n; ++v) {
+        const int32_t p = par[I.offset + v];
+        parent[v] = p < 0 ? -1 : p - (int32_t)I.offset;
+        dist[v] = d[I.offset + v];
+    }
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_server_info(dsssp_server *s, dsssp_server_totals *tot)
+{
+    clear_error();
+    if (!s || !tot) return fail(DSSSP_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    tot->instances = (int32_t)s->inst.size();
+    tot->budget_bytes = s->cfg.mem_budget_bytes;
+    tot->used_bytes = s->used;
+    tot->launches = s->launches;
+    tot->records = s->records;
+    tot->requests = s->requests_done;
+    tot->busy_ms = s->busy_ms;
+    tot->pending = (int64_t)s->pending.size();
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_server_destroy(dsssp_server *s)
+{
+    clear_error();
+    if (!s) return DSSSP_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+    }
+    s->cv_work.notify_all();
+    if (s->loop.joinable()) s->loop.join();
+    for (auto &kv : s->all) delete kv.second;
+    if (s->t) dsssp_tree_destroy(s->t);
+    if (s->g) dsssp_graph_destroy(s->g);
+    delete s;
+    return DSSSP_OK;
+}

@@ -155,6 +155,14 @@ int dsssp_gen_changes(int64_t n, int64_t m, const int32_t *a, const int32_t *b, 
                       int64_t k, double ins_frac, int32_t wlo, int32_t whi,
                       int64_t first_deletable, uint64_t seed, dsssp_change *out);
 
+/* Grid workload batch: round(k*del_frac) deletions, a fraction tree_frac of
+ * them drawn from edges of the current tree (parent[] of the tree before the
+ * batch), the rest from other edges; the remaining records insert distinct
+ * absent pairs at Manhattan distance 1..max_manhattan, weights U[wlo,whi]. */
+int dsssp_gen_changes_grid(int32_t side, int64_t m, const int32_t *a, const int32_t *b, const int32_t *w,
+                           const int32_t *parent, int64_t k, double del_frac, double tree_frac,
+                           int32_t max_manhattan, int32_t wlo, int32_t whi, uint64_t seed, dsssp_change *out);
+
 /* ---- device graph ----------------------------------------------------------
  * Replaces the host CSR flatten + UVM upload (SSSPmainForUndirected.cu:66-91):
  * the CSR is built on the host (rows sorted by (neighbour, weight)) and copied
@@ -186,6 +194,26 @@ int dsssp_tree_create(dsssp_graph *g, int32_t src, const int32_t *parent, const 
 int dsssp_tree_destroy(dsssp_tree *t);
 int dsssp_tree_read(dsssp_tree *t, int32_t *parent, int64_t *dist);
 int dsssp_tree_copy(dsssp_tree *dst, dsssp_tree *src);
+/* One shortest-path forest grown from several roots at once (each root has
+ * distance 0 and parent itself), computed on the GPU.  Used for many graphs
+ * held in one device graph, one root per component.  No reference
+ * counterpart: the reference handles one source per run
+ * (seqSSSPwithDist_undir.cpp:196). */
+int dsssp_tree_create_roots(dsssp_graph *g, int32_t nroots, const int32_t *roots, uint32_t flags,
+                            dsssp_tree **out);
+
+/* ---- batched form: K trees (sources) on one graph ---------------------------
+ * ntrees independent SSSP trees, tree j from srcs[j], computed on the GPU and
+ * updated together by dsssp_update / dsssp_update_batch: the batch is applied
+ * to the graph once and every tree is brought to the exact tree of the updated
+ * graph in the same launches (state of vertex v, tree j at slot v*K' + j with
+ * K' = ntrees rounded up to a power of two, so one vertex's trees share cache
+ * lines).  Each tree equals what the plain path gives for its source.  The
+ * reference runs one source per process (SSSPmainForUndirected.cu:146-165). */
+int dsssp_forest_create(dsssp_graph *g, int32_t ntrees, const int32_t *srcs, uint32_t flags,
+                        dsssp_tree **out);
+int dsssp_tree_count(const dsssp_tree *t, int32_t *ntrees);
+int dsssp_forest_read(dsssp_tree *t, int32_t tree, int32_t *parent, int64_t *dist);
 
 /* ---- the update --------------------------------------------------------------
  * Replaces STEP1 + STEP2 of op_main (SSSPmainForUndirected.cu:196-292):
@@ -201,6 +229,49 @@ int dsssp_update(dsssp_graph *g, dsssp_tree *t, const dsssp_change *chg, int64_t
 int dsssp_batch_upload(dsssp_graph *g, const dsssp_change *chg, int64_t k, dsssp_batch **out);
 int dsssp_batch_destroy(dsssp_batch *b);
 int dsssp_update_batch(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stats);
+
+/* ---- served form -------------------------------------------------------------
+ * Resident instances (each a graph with its own source) kept in HBM under a
+ * memory budget, and a request loop that merges batches arriving for different
+ * instances into shared launches (continuous batching).  Each instance's result
+ * is bit-identical to updating it alone.  No reference counterpart: the
+ * reference applies one batch to one graph per process
+ * (SSSPmainForUndirected.cu:27-335). */
+typedef struct dsssp_server dsssp_server;
+typedef struct {
+    uint64_t mem_budget_bytes;  /* device bytes all admitted instances may use      */
+    int32_t max_merge;          /* most requests merged into one launch (0: no cap) */
+    int32_t pad0;
+    int64_t max_records;        /* most records per merged launch (0: no cap)       */
+} dsssp_server_config;
+typedef struct {
+    int32_t instance, merged;   /* merged: requests that shared the launch          */
+    int64_t records;
+    double queue_ms;            /* submit -> picked by the loop                     */
+    double latency_ms;          /* submit -> result in HBM                          */
+    double update_ms;           /* device time of the merged update                 */
+} dsssp_request_stats;
+typedef struct {
+    int32_t instances, pad0;
+    uint64_t budget_bytes, used_bytes;
+    int64_t launches, records, requests, pending;
+    double busy_ms;
+} dsssp_server_totals;
+
+int dsssp_server_create(const dsssp_server_config *cfg, dsssp_server **out);
+/* Admits an instance before start; DSSSP_ERR_BUDGET if it does not fit. */
+int dsssp_server_add_instance(dsssp_server *s, int64_t n, int64_t m, const int32_t *a, const int32_t *b,
+                              const int32_t *w, int32_t src, int32_t *id);
+/* Uploads every admitted instance, computes their trees, starts the loop. */
+int dsssp_server_start(dsssp_server *s);
+/* Queues a batch for one instance (ids local to the instance); returns at once. */
+int dsssp_server_submit(dsssp_server *s, int32_t id, const dsssp_change *chg, int64_t k, uint64_t *ticket);
+/* Blocks until the request is applied; returns its status. */
+int dsssp_server_wait(dsssp_server *s, uint64_t ticket, dsssp_request_stats *st);
+/* Tree of one instance after all of its submitted requests. */
+int dsssp_server_read(dsssp_server *s, int32_t id, int32_t *parent, int64_t *dist);
+int dsssp_server_info(dsssp_server *s, dsssp_server_totals *tot);
+int dsssp_server_destroy(dsssp_server *s);
 
 #ifdef __cplusplus
 }

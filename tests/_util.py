@@ -56,7 +56,9 @@ GOLDEN_N = {"toy": 6, "r300": 300, "r2000": 2000, "sparse1500": 1500}
 
 
 def random_connected(rng, n, m, wlo, whi):
-    """Random recursive spanning tree + uniform distinct extra pairs."""
+    """Random recursive spanning tree + uniform distinct extra pairs (m is capped
+    at the number of vertex pairs)."""
+    m = min(m, n * (n - 1) // 2)
     a = [int(rng.integers(0, i)) for i in range(1, n)]
     b = list(range(1, n))
     seen = {(min(x, y), max(x, y)) for x, y in zip(a, b)}

@@ -163,3 +163,40 @@ def test_gpu_entry_points_fail_cleanly_without_gpu():
     with pytest.raises(ds.DssspError) as e:
         ds.Graph(3, [0], [1], [1])
     assert e.value.code in (ds.ERR_NODEV, ds.ERR_HIP)
+
+
+def test_server_admission_by_budget():
+    """Admission happens on the host before anything touches the GPU."""
+    a, b, w = ds.gen_tree_plus_random(10_000, 50_000, 1, 10, seed=1)
+    srv = ds.Server(mem_budget_bytes=2_000_000)
+    assert srv.add_instance(10_000, a, b, w, 0) == 0
+    with pytest.raises(ds.DssspError) as e:
+        srv.add_instance(10_000, a, b, w, 0)
+    assert e.value.code == ds.ERR_BUDGET
+    info = srv.info()
+    assert info["instances"] == 1 and 0 < info["used_bytes"] <= 2_000_000
+    with pytest.raises(ds.DssspError) as e:
+        srv.submit(0, [[0, 1, 1, 1]])
+    assert e.value.code == ds.ERR_STATE  # not started
+    with pytest.raises(ds.DssspError) as e:
+        srv.add_instance(10, [0], [1], [1], 99)
+    assert e.value.code == ds.ERR_RANGE
+    srv.close()
+
+
+def test_gen_changes_grid():
+    side = 80
+    a, b, w = ds.gen_grid(side, 0.01, 1, 10000, seed=3)
+    n = side * side
+    par, dist = oracle.dijkstra(oracle.CSR.from_edges(n, a, b, w), [0])
+    k = 1000
+    c = ds.gen_changes_grid(side, a, b, w, par, k, 0.7, 0.5, 3, 1, 10000, seed=4)
+    dele, ins = c[c[:, 3] == 0], c[c[:, 3] == 1]
+    assert len(dele) == 700 and len(ins) == 300
+    tree = {(min(v, int(p)), max(v, int(p))) for v, p in enumerate(par) if p >= 0 and p != v}
+    dk = [(min(x, y), max(x, y)) for x, y, _, _ in dele]
+    assert len(set(dk)) == 700 and sum(key in tree for key in dk) == 350
+    ek = {(min(x, y), max(x, y)) for x, y in zip(a, b)}
+    for x, y, z, _ in ins:
+        md = abs(x // side - y // side) + abs(x % side - y % side)
+        assert 1 <= md <= 3 and (min(x, y), max(x, y)) not in ek and 1 <= z <= 10000
