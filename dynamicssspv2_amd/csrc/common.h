@@ -50,6 +50,8 @@ struct Counters {
     uint32_t ring[3][2];  // (light, heavy) queue lengths, indexed by round % 3
     uint32_t inv_total;   // length of invl
     uint32_t err;
+    uint32_t aux_hcnt;    // heavy chunks queued by a single-pass stage
+    uint32_t pad0;
     unsigned long long relax_frontier, relax_edges, relax_improved, relax_enq;
     unsigned long long other_edges, probes, tombstones;
 };

@@ -38,7 +38,7 @@ using namespace dsssp;
 
 namespace {
 
-dsssp_tuning g_tuning = {FRONTIER_BLOCK, 2048, 1024, 1024, 4, 0};
+dsssp_tuning g_tuning = {FRONTIER_BLOCK, 2048, 256, 256, 4, 0};
 std::mutex g_tuning_mu;
 thread_local int t_device = 0;
 
@@ -211,6 +211,26 @@ void advance(Ring &r)
     r.qs ^= 1;
 }
 
+// One pass of a single-pass mode (seed, parent pick): rows longer than the
+// heavy threshold are queued as chunks by the first launch and processed by a
+// second, chunk-only launch.  Both heavy buffers are idle between loop phases.
+void single_pass(dsssp_tree *t, int mode, FArgs a)
+{
+    hipStream_t s = t->g->stream;
+    const int grid = std::max(1, tuning().grid);
+    launch_fill_u32(&t->c->aux_hcnt, 0u, 1, s);
+    a.out_hq = t->hq[0];
+    a.out_hcnt = &t->c->aux_hcnt;
+    launch_frontier(mode, a, grid, s);
+    FArgs b = a;
+    b.in_q = nullptr;
+    b.in_cnt = nullptr;
+    b.in_n = 0;
+    b.in_hq = t->hq[0];
+    b.in_hcnt = &t->c->aux_hcnt;
+    launch_frontier(mode, b, grid, s);
+}
+
 // Launches frontier rounds until the current ring slot is empty.
 int run_rounds(dsssp_tree *t, int mode, Ring &r, int64_t &rounds, double *kernel_ms)
 {
@@ -286,7 +306,6 @@ int repick_parents(dsssp_tree *t, bool use_lvl, int64_t &rounds)
     dsssp_graph *g = t->g;
     if (int s = ensure_pick_buffers(t)) return s;
     const uint32_t ns = (uint32_t)t->nslots;
-    const dsssp_tuning tu = tuning();
     if (use_lvl) {
         Ring r;
         zero_ring(t, r);
@@ -305,7 +324,7 @@ int repick_parents(dsssp_tree *t, bool use_lvl, int64_t &rounds)
     a.in_cnt = nullptr;
     a.in_n = ns;
     a.use_lvl = use_lvl ? 1u : 0u;
-    launch_frontier(M_PICK, a, std::max(1, tu.grid), g->stream);
+    single_pass(t, M_PICK, a);
     launch_apply_best(ns, t->kshift, t->st, t->best, t->c, g->stream);
     HIP_TRY(hipGetLastError());
     return DSSSP_OK;
@@ -314,7 +333,12 @@ int repick_parents(dsssp_tree *t, bool use_lvl, int64_t &rounds)
 int ensure_hq(dsssp_tree *t, uint64_t total_entries)
 {
     const dsssp_tuning tu = tuning();
-    const uint64_t need = ((2 * total_entries / (uint64_t)std::max(64, tu.chunk)) << t->kshift) + 1024;
+    // A row longer than `heavy` gives ceil(deg/chunk) <= deg/chunk + 1 chunks and
+    // at most entries/heavy rows are that long; every slot is queued at most
+    // once per round.
+    const uint64_t per_tree = total_entries / (uint64_t)std::max(64, tu.chunk) +
+                              total_entries / (uint64_t)std::max(64, tu.heavy_degree) + 1;
+    const uint64_t need = (per_tree << t->kshift) + 1024;
     if (need > 0xFFFFFFFFull) return fail(DSSSP_ERR_ARG, "heavy queue would exceed 2^32 entries; raise chunk");
     if (t->hcap >= need) return DSSSP_OK;
     dev_free(t->hq[0]);
@@ -396,7 +420,6 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
         if (int e = ensure_pool(g, 2 * (uint64_t)b->n_ins)) return e;
     }
     if (int e = ensure_hq(t, g->E + g->delta_entries + 2 * (uint64_t)b->n_ins)) return e;
-    const dsssp_tuning tu = tuning();
     int64_t inv_rounds = 0, relax_rounds = 0, repair_rounds = 0;
     double relax_kernel_ms = 0;
 
@@ -440,7 +463,7 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
             a.out_q = oq;
             a.out_cnt = oc;
             a.epoch = ep;
-            launch_frontier(M_SEED, a, std::max(1, tu.grid), s);
+            single_pass(t, M_SEED, a);
         }
         if (b->n_ins > 0)
             launch_ins_offer(b->rec, b->k, t->kshift, b->pos, g->pool, t->st, t->stamp, ep, oq, oc, ns, t->c, s);

@@ -167,7 +167,9 @@ __device__ __forceinline__ void finish_edge(const FArgs &A, bool want, uint32_t 
     }
 }
 
-constexpr bool defers(int mode) { return mode == M_RELAX || mode == M_INVAL || mode == M_TBFS; }
+// Every mode splits long rows into chunks: loop modes process them in the next
+// round, single-pass modes (seed, parent pick) in a second, chunk-only launch.
+constexpr bool defers(int) { return true; }
 
 struct TileSmem {
     uint32_t off[FW][64];
@@ -203,11 +205,12 @@ __device__ __forceinline__ bool owner_prep(const FArgs &A, uint32_t os, uint32_t
 }
 
 template <int MODE>
-__device__ void do_tile(const FArgs &A, uint32_t t, uint32_t L, TileSmem &sm, uint32_t wv, WaveStats &S)
+__device__ void do_tile(const FArgs &A, uint32_t t, uint32_t tv_shift, uint32_t L, TileSmem &sm, uint32_t wv,
+                        WaveStats &S)
 {
     const uint32_t lane = lane_id();
-    const uint32_t idx = t * 64u + lane;
-    const bool valid = idx < L;
+    const uint32_t idx = (t << tv_shift) + lane;
+    const bool valid = lane < (1u << tv_shift) && idx < L;
     uint32_t os = 0, deg = 0, blen = 0, dbeg = 0, aux0 = 0, aux1 = 0;
     uint64_t b0 = 0;
     if (valid) {
@@ -316,16 +319,21 @@ __global__ __launch_bounds__(FB) void k_frontier(FArgs A)
     // capacity (only after a reported overflow) is clamped so no read leaves them.
     const uint32_t L = min(A.in_cnt ? load_relaxed(A.in_cnt) : A.in_n, A.out_cap);
     const uint32_t H = min(A.in_hcnt ? load_relaxed(A.in_hcnt) : 0u, A.out_hcap);
-    const uint32_t T = (L + 63u) >> 6;
-    const uint32_t items = H + T;
     const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t nw = gridDim.x * FW;
+    // Vertices per tile: 64 when the frontier fills every wave several times,
+    // fewer (down to 1) for short frontiers, so that the rows of a short
+    // frontier are spread over all waves instead of walked by a few.
+    uint32_t tv_shift = 6;
+    while (tv_shift > 0 && (L >> tv_shift) < 2u * nw) --tv_shift;
+    const uint32_t T = (L + (1u << tv_shift) - 1u) >> tv_shift;
+    const uint32_t items = H + T;
     WaveStats S{0, 0, 0, 0};
     for (uint32_t it = blockIdx.x * FW + wv; it < items; it += nw) {
         if (it < H)
             do_chunk<MODE>(A, it, S);
         else
-            do_tile<MODE>(A, it - H, L, sm, wv, S);
+            do_tile<MODE>(A, it - H, tv_shift, L, sm, wv, S);
     }
     const uint32_t imp = wave_sum(S.improved);
     const uint32_t enq = wave_sum(S.enq);
@@ -346,18 +354,36 @@ __global__ __launch_bounds__(FB) void k_frontier(FArgs A)
 // ---------------------------------------------------------------------------
 
 // Counts new entries per endpoint; the first touch of a vertex lists it.
+__device__ __forceinline__ void wave_append_plain(bool want, uint32_t v, uint32_t *q, uint32_t *cnt)
+{
+    const unsigned long long mask = __ballot(want);
+    if (mask == 0) return;
+    const uint32_t leader = (uint32_t)__builtin_ctzll(mask);
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(cnt, (uint32_t)__popcll(mask));
+    base = __shfl(base, (int)leader);
+    if (want) q[base + (uint32_t)__popcll(mask & lanemask_lt())] = v;
+}
+
 __global__ __launch_bounds__(RB) void k_ins_count(const int4 *rec, int64_t k, uint32_t *tcnt, uint64_t *slot,
                                                   uint32_t *touched, GraphCtl *gc)
 {
     const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
-    if (i >= k) return;
-    const int4 r = rec[i];
-    if (r.w != 1) return;
-    const uint32_t ra = atomicAdd(&tcnt[r.x], 1u);
-    if (ra == 0) touched[atomicAdd(&gc->touched, 1u)] = (uint32_t)r.x;
-    const uint32_t rb = atomicAdd(&tcnt[r.y], 1u);
-    if (rb == 0) touched[atomicAdd(&gc->touched, 1u)] = (uint32_t)r.y;
-    slot[i] = ((uint64_t)ra << 32) | rb;
+    bool ta = false, tb = false;
+    int4 r = make_int4(0, 0, 0, -1);
+    if (i < k) {
+        r = rec[i];
+        if (r.w == 1) {
+            const uint32_t ra = atomicAdd(&tcnt[r.x], 1u);
+            const uint32_t rb = atomicAdd(&tcnt[r.y], 1u);
+            ta = ra == 0;
+            tb = rb == 0;
+            slot[i] = ((uint64_t)ra << 32) | rb;
+        }
+    }
+    // the touched list holds each vertex once (first touch); at most 2k entries
+    wave_append_plain(ta, (uint32_t)r.x, touched, &gc->touched);
+    wave_append_plain(tb, (uint32_t)r.y, touched, &gc->touched);
 }
 
 // Moves each touched vertex's delta row to a fresh block with room for its new
