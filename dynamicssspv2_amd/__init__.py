@@ -78,7 +78,7 @@ class ServerTotals(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [(f, ctypes.c_int32) for f in ("block", "grid", "heavy_degree", "chunk", "round_batch",
-                                              "kernel_timing")]
+                                              "kernel_timing", "use_graph")]
 
 
 def build(force: bool = True) -> None:

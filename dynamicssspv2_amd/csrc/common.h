@@ -51,7 +51,14 @@ struct Counters {
     uint32_t inv_total;   // length of invl
     uint32_t err;
     uint32_t aux_hcnt;    // heavy chunks queued by a single-pass stage
-    uint32_t pad0;
+    // Device-side round indexing of a loop phase: round i reads ring slot
+    // (ring_base + i) % 3 and queue buffer (q_base + i) & 1 and stamps with
+    // epoch_base + i; the last workgroup to finish a round advances iter.
+    uint32_t iter;
+    uint32_t blocks_done;
+    uint32_t ring_base;
+    uint32_t q_base;
+    uint32_t epoch_base;
     unsigned long long relax_frontier, relax_edges, relax_improved, relax_enq;
     unsigned long long other_edges, probes, tombstones;
 };

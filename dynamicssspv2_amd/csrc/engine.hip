@@ -38,7 +38,7 @@ using namespace dsssp;
 
 namespace {
 
-dsssp_tuning g_tuning = {FRONTIER_BLOCK, 2048, 256, 256, 4, 0};
+dsssp_tuning g_tuning = {FRONTIER_BLOCK, 2048, 256, 256, 4, 0, 0};
 std::mutex g_tuning_mu;
 thread_local int t_device = 0;
 
@@ -141,6 +141,14 @@ struct dsssp_tree {
     Counters *c = nullptr;
     Counters *c_host = nullptr;  // pinned
     uint32_t epoch = 1;
+    // Captured groups of relaxation/invalidation rounds (hipGraph), keyed by
+    // mode, group size, grid and the exact kernel arguments.
+    struct Captured {
+        int mode, group, grid;
+        FArgs args;
+        hipGraphExec_t exec;
+    };
+    std::vector<Captured> graphs;
 };
 
 struct dsssp_batch {
@@ -166,7 +174,8 @@ struct Ring {
 FArgs base_args(dsssp_tree *t)
 {
     const dsssp_tuning tu = tuning();
-    FArgs a{};
+    FArgs a;
+    memset(&a, 0, sizeof a);  // padding included: captured graphs are keyed by these bytes
     a.g = t->g->view();
     a.st = t->st;
     a.stamp = t->stamp;
@@ -231,45 +240,87 @@ void single_pass(dsssp_tree *t, int mode, FArgs a)
     launch_frontier(mode, b, grid, s);
 }
 
-// Launches frontier rounds until the current ring slot is empty.
+// Returns the instantiated hipGraph of `group` back-to-back rounds of `mode`
+// with arguments `a` (every round has the same arguments: round indexing is
+// on the device), capturing it on first use.
+int captured_rounds(dsssp_tree *t, int mode, int group, int grid, const FArgs &a, hipGraphExec_t *out)
+{
+    for (auto &e : t->graphs)
+        if (e.mode == mode && e.group == group && e.grid == grid && !memcmp(&e.args, &a, sizeof a)) {
+            *out = e.exec;
+            return DSSSP_OK;
+        }
+    if (t->graphs.size() >= 24) {
+        (void)hipGraphExecDestroy(t->graphs.front().exec);
+        t->graphs.erase(t->graphs.begin());
+    }
+    hipStream_t s = t->g->stream;
+    hipGraph_t gr = nullptr;
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < group; ++i) launch_frontier(mode, a, grid, s);
+    HIP_TRY(hipStreamEndCapture(s, &gr));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) return fail(DSSSP_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    t->graphs.push_back({mode, group, grid, a, exec});
+    *out = exec;
+    return DSSSP_OK;
+}
+
+// Launches frontier rounds until the current ring slot is empty.  Rounds run
+// back to back on the stream (as captured hipGraphs when use_graph is set);
+// the host reads one 8-byte counter pair per group of rounds.
 int run_rounds(dsssp_tree *t, int mode, Ring &r, int64_t &rounds, double *kernel_ms)
 {
     dsssp_graph *g = t->g;
+    hipStream_t s = g->stream;
     const dsssp_tuning tu = tuning();
     int group = std::max(1, tu.round_batch);
     const int max_group = 64;
-    FArgs base = base_args(t);
     const int grid = std::max(1, tu.grid);
     const bool timed = tu.kernel_timing && kernel_ms;
     const int64_t start_rounds = rounds;
+    if ((uint64_t)t->epoch + (uint64_t)g->n + 256 >= 0xF0000000ull) {  // stamps restart before the epoch wraps
+        launch_fill_u32(t->stamp, 0u, t->nslots, s);
+        t->epoch = 1;
+    }
+    FArgs a = base_args(t);
+    a.dev_iter = 1;
+    a.qv[0] = t->q[0];
+    a.qv[1] = t->q[1];
+    a.hqv[0] = t->hq[0];
+    a.hqv[1] = t->hq[1];
+    const uint32_t ep0 = t->epoch + 1;
+    launch_phase_init(t->c, (uint32_t)r.cs, (uint32_t)r.qs, ep0, s);
+    uint32_t launched = 0;
     for (;;) {
-        if (timed && g->kev.size() < (size_t)(2 * group)) {
-            size_t old = g->kev.size();
-            g->kev.resize((size_t)(2 * group));
-            for (size_t i = old; i < g->kev.size(); ++i) HIP_TRY(hipEventCreate(&g->kev[i]));
+        if (timed) {
+            if (g->kev.size() < (size_t)(2 * group)) {
+                const size_t old = g->kev.size();
+                g->kev.resize((size_t)(2 * group));
+                for (size_t i = old; i < g->kev.size(); ++i) HIP_TRY(hipEventCreate(&g->kev[i]));
+            }
+            for (int i = 0; i < group; ++i) {
+                HIP_TRY(hipEventRecord(g->kev[2 * i], s));
+                launch_frontier(mode, a, grid, s);
+                HIP_TRY(hipEventRecord(g->kev[2 * i + 1], s));
+            }
+        } else if (tu.use_graph) {
+            hipGraphExec_t exec = nullptr;
+            if (int e = captured_rounds(t, mode, group, grid, a, &exec)) return e;
+            HIP_TRY(hipGraphLaunch(exec, s));
+        } else {
+            for (int i = 0; i < group; ++i) launch_frontier(mode, a, grid, s);
         }
-        for (int i = 0; i < group; ++i) {
-            FArgs a = base;
-            a.in_q = t->q[r.qs];
-            a.in_cnt = &t->c->ring[r.cs][0];
-            a.in_hq = t->hq[r.qs];
-            a.in_hcnt = &t->c->ring[r.cs][1];
-            a.out_q = t->q[r.qs ^ 1];
-            a.out_cnt = &t->c->ring[(r.cs + 1) % 3][0];
-            a.out_hq = t->hq[r.qs ^ 1];
-            a.out_hcnt = &t->c->ring[(r.cs + 1) % 3][1];
-            a.zero_cnt = &t->c->ring[(r.cs + 2) % 3][0];
-            a.epoch = next_epoch(t);
-            if (timed) HIP_TRY(hipEventRecord(g->kev[2 * i], g->stream));
-            launch_frontier(mode, a, grid, g->stream);
-            if (timed) HIP_TRY(hipEventRecord(g->kev[2 * i + 1], g->stream));
-            advance(r);
-            ++rounds;
-        }
+        launched += (uint32_t)group;
+        rounds += group;
+        r.cs = (r.cs + group) % 3;
+        r.qs ^= group & 1;
+        t->epoch = ep0 + launched - 1;
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(t->c_host->ring[r.cs], t->c->ring[r.cs], 2 * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                               g->stream));
-        HIP_TRY(hipStreamSynchronize(g->stream));
+        HIP_TRY(hipMemcpyAsync(t->c_host->ring[r.cs], t->c->ring[r.cs], 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
         if (timed) {
             for (int i = 0; i < group; ++i) {
                 float ms = 0;
@@ -583,6 +634,7 @@ extern "C" int dsssp_set_tuning(const dsssp_tuning *t)
     if (t->chunk) nt.chunk = t->chunk;
     if (t->round_batch) nt.round_batch = t->round_batch;
     nt.kernel_timing = t->kernel_timing;
+    nt.use_graph = t->use_graph;
     if (nt.grid < 1 || nt.grid > 65536 || nt.heavy_degree < 64 || nt.chunk < 64 || nt.chunk % 64 ||
         nt.round_batch < 1 || nt.round_batch > 64)
         return fail(DSSSP_ERR_ARG, "tuning out of range");
@@ -815,6 +867,7 @@ static void tree_release(dsssp_tree *t)
     dev_free(t->best);
     dev_free(t->c);
     if (t->c_host) (void)hipHostFree(t->c_host);
+    for (auto &e : t->graphs) (void)hipGraphExecDestroy(e.exec);
     delete t;
     graph_detach(g);
 }

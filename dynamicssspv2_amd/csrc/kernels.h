@@ -52,6 +52,12 @@ struct FArgs {
     uint32_t chunk;   // entries per chunk, multiple of 64
     uint32_t use_lvl; // M_PICK: require lvl[parent] + 1 == lvl[v]
     uint32_t kshift;  // log2 of the trees per vertex: state slot = vertex << kshift | tree
+    // dev_iter = 1: in/out queues, counters and epoch come from Counters'
+    // round indexing (identical arguments for every round of a loop phase,
+    // so rounds can be replayed from a captured hipGraph)
+    uint32_t dev_iter;
+    uint32_t *qv[2];
+    uint64_t *hqv[2];
 };
 
 constexpr int FRONTIER_BLOCK = 256;
@@ -82,6 +88,8 @@ void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 void launch_lvl_init(uint32_t nslots, uint32_t kshift, const uint64_t *st, uint32_t *lvl, uint32_t *best,
                      uint32_t *stamp, uint32_t epoch, uint32_t *out_q, uint32_t *out_cnt, uint32_t cap, Counters *c,
                      hipStream_t s);
+// Starts a loop phase's round indexing (iter = 0).
+void launch_phase_init(Counters *c, uint32_t ring_base, uint32_t q_base, uint32_t epoch_base, hipStream_t s);
 void launch_apply_best(uint32_t nslots, uint32_t kshift, uint64_t *st, const uint32_t *best, Counters *c,
                        hipStream_t s);
 

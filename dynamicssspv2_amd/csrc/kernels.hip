@@ -311,6 +311,22 @@ template <int MODE>
 __global__ __launch_bounds__(FB) void k_frontier(FArgs A)
 {
     __shared__ TileSmem sm;
+    if (A.dev_iter) {
+        Counters *c = A.c;
+        const uint32_t it = load_relaxed(&c->iter);
+        const uint32_t cs = (load_relaxed(&c->ring_base) + it) % 3u;
+        const uint32_t qs = (load_relaxed(&c->q_base) + it) & 1u;
+        A.epoch = load_relaxed(&c->epoch_base) + it;
+        A.in_q = A.qv[qs];
+        A.in_cnt = &c->ring[cs][0];
+        A.in_hq = A.hqv[qs];
+        A.in_hcnt = &c->ring[cs][1];
+        A.out_q = A.qv[qs ^ 1u];
+        A.out_cnt = &c->ring[(cs + 1u) % 3u][0];
+        A.out_hq = A.hqv[qs ^ 1u];
+        A.out_hcnt = &c->ring[(cs + 1u) % 3u][1];
+        A.zero_cnt = &c->ring[(cs + 2u) % 3u][0];
+    }
     if (A.zero_cnt && blockIdx.x == 0 && threadIdx.x == 0) {
         A.zero_cnt[0] = 0;
         A.zero_cnt[1] = 0;
@@ -345,6 +361,19 @@ __global__ __launch_bounds__(FB) void k_frontier(FArgs A)
             atomicAdd(&A.c->relax_enq, (unsigned long long)enq);
         } else {
             atomicAdd(&A.c->other_edges, (unsigned long long)S.edges);
+        }
+    }
+    if (A.dev_iter) {
+        // The last workgroup to finish advances the round index; the next
+        // launch on the stream starts after this one has completed.
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __threadfence();
+            const uint32_t prev = atomicAdd(&A.c->blocks_done, 1u);
+            if (prev == gridDim.x - 1u) {
+                atomicExch(&A.c->blocks_done, 0u);
+                atomicAdd(&A.c->iter, 1u);
+            }
         }
     }
 }
@@ -387,26 +416,43 @@ __global__ __launch_bounds__(RB) void k_ins_count(const int4 *rec, int64_t k, ui
 }
 
 // Moves each touched vertex's delta row to a fresh block with room for its new
-// entries; leaves the old length in tcnt as the write base.
+// entries; leaves the old length in tcnt as the write base.  Blocks are carved
+// from the pool with one atomicAdd per wave (prefix sum of the lanes' sizes).
 __global__ __launch_bounds__(RB) void k_ins_alloc(const uint32_t *touched, uint32_t max_touched, uint64_t *dbl,
                                                   uint32_t *tcnt, uint64_t *pool, uint64_t pool_cap, GraphCtl *gc)
 {
     const uint32_t i = blockIdx.x * RB + threadIdx.x;
     const uint32_t nt = min(load_relaxed(&gc->touched), max_touched);
+    const uint32_t lane = lane_id();
+    uint32_t v = 0, len = 0, add = 0;
+    uint64_t beg = 0;
+    if (i < nt) {
+        v = touched[i];
+        const uint64_t d = dbl[v];
+        beg = d >> 32;
+        len = (uint32_t)d;
+        add = tcnt[v];
+    }
+    uint64_t size = (uint64_t)len + add, incl = size;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t y = __shfl_up(incl, d);
+        if ((int)lane >= d) incl += y;
+    }
+    const uint64_t total = __shfl(incl, 63);
+    if (total == 0) return;  // wave-uniform
+    unsigned long long base = 0;
+    if (lane == 63) base = atomicAdd(&gc->pool_top, (unsigned long long)total);
+    base = __shfl(base, 63);
     if (i >= nt) return;
-    const uint32_t v = touched[i];
-    const uint64_t d = dbl[v];
-    const uint64_t beg = d >> 32;
-    const uint32_t len = (uint32_t)d;
-    const uint32_t add = tcnt[v];
-    const unsigned long long nb = atomicAdd(&gc->pool_top, (unsigned long long)(len + add));
-    if (nb + len + add > pool_cap) {
+    const uint64_t nb = base + incl - size;
+    if (nb + size > pool_cap) {
         atomicOr(&gc->err, ERR_POOL);
         tcnt[v] = 0xFFFFFFFFu;
         return;
     }
     for (uint32_t j = 0; j < len; ++j) pool[nb + j] = pool[beg + j];
-    dbl[v] = ((uint64_t)nb << 32) | (uint64_t)(len + add);
+    dbl[v] = (nb << 32) | size;
     tcnt[v] = len;
 }
 
@@ -646,6 +692,17 @@ __global__ __launch_bounds__(RB) void k_apply_best(uint32_t nslots, uint32_t ksh
     st[s] = pack_state(st_dist(v), b);
 }
 
+__global__ void k_phase_init(Counters *c, uint32_t ring_base, uint32_t q_base, uint32_t epoch_base)
+{
+    if (threadIdx.x == 0) {
+        c->iter = 0;
+        c->blocks_done = 0;
+        c->ring_base = ring_base;
+        c->q_base = q_base;
+        c->epoch_base = epoch_base;
+    }
+}
+
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + RB - 1) / RB); }
 
 }  // namespace
@@ -742,6 +799,11 @@ void launch_lvl_init(uint32_t nslots, uint32_t kshift, const uint64_t *st, uint3
     if (nslots > 0)
         hipLaunchKernelGGL(k_lvl_init, dim3(blocks_for(nslots)), dim3(RB), 0, s, nslots, kshift, st, lvl, best, stamp,
                            epoch, out_q, out_cnt, cap, c);
+}
+
+void launch_phase_init(Counters *c, uint32_t ring_base, uint32_t q_base, uint32_t epoch_base, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_phase_init, dim3(1), dim3(64), 0, s, c, ring_base, q_base, epoch_base);
 }
 
 void launch_apply_best(uint32_t nslots, uint32_t kshift, uint64_t *st, const uint32_t *best, Counters *c,

@@ -103,6 +103,7 @@ typedef struct {
     int32_t chunk;            /* entries per heavy chunk (multiple of 64)              */
     int32_t round_batch;      /* relaxation rounds launched between convergence checks */
     int32_t kernel_timing;    /* 1: time every relaxation launch with events           */
+    int32_t use_graph;        /* 1: launch round groups as captured hipGraphs          */
 } dsssp_tuning;
 
 /* ---- library ---------------------------------------------------------- */

@@ -320,3 +320,53 @@ def test_cli_drop_in(tmp_path):
     assert "Total Time taken for SSSP update" in res.stdout
     p, d = ds.read_sssp(str(out), 6)
     assert d.tolist() == [0, 4, 6, 6, 4, 9]
+
+
+def test_hipgraph_rounds_match_eager():
+    """Round groups replayed from captured hipGraphs give the eager result on
+    every update of a sequence (each update replays several captured groups,
+    with host synchronisations and other kernels between the replays)."""
+    a, b, w = ds.gen_rmat(12, 16, seed=31)
+    n = 1 << 12
+    rng = np.random.default_rng(31)
+    batches, ca, cb, cw = [], a, b, w
+    for i in range(6):
+        c = ds.gen_changes(n, ca, cb, cw, 1500, [0.0, 0.5, 1.0][i % 3], 1, 1000, seed=int(rng.integers(1 << 30)))
+        batches.append(c)
+        ca, cb, cw = oracle.apply_changes(n, ca, cb, cw, c)
+    old = ds.get_tuning()
+    results = {}
+    try:
+        for use_graph in (0, 1):
+            ds.set_tuning(use_graph=use_graph, round_batch=2)
+            say(f"hipgraph use_graph={use_graph}")
+            g = ds.Graph(n, a, b, w)
+            t = ds.Tree(g, 0)
+            seq = [t.read()]
+            for c in batches:
+                ds.update(g, t, c)
+                seq.append(t.read())
+            results[use_graph] = seq
+    finally:
+        ds.set_tuning(**old)
+    for (p0, d0), (p1, d1) in zip(results[0], results[1]):
+        assert np.array_equal(p0, p1) and np.array_equal(d0, d1)
+    m = Mirror(n, a, b, w)
+    for c in batches:
+        m.apply(c)
+    check_tree(m, *results[1][-1], [0], "hipgraph")
+
+
+def test_op_tools_sssp_matches_reference_dijkstra(tmp_path):
+    """`op_tools sssp` (the op_ssspSeq role on the GPU) against the reference
+    Dijkstra's own output on the golden graphs: same distances, exact integers."""
+    tools = os.path.join(os.path.dirname(ds.LIB_PATH), "op_tools")
+    for name in ("toy", "r2000"):
+        n = GOLDEN_N[name]
+        say(f"op_tools sssp {name}")
+        out = tmp_path / f"{name}.txt"
+        out.write_bytes(subprocess.check_output([tools, "sssp", os.path.join(GOLDEN, f"{name}_graph.txt"), str(n)],
+                                                timeout=120))
+        p, d = ds.read_sssp(str(out), n)
+        _, ed = read_ref_sssp(os.path.join(GOLDEN, f"{name}_SSSP.txt"), n)
+        assert np.array_equal(d, ed)

@@ -8,7 +8,7 @@ import pytest
 
 import dynamicssspv2_amd as ds
 import oracle
-from tests._util import GOLDEN, GOLDEN_N, INF, read_changes, read_graph, read_ref_sssp
+from tests._util import GOLDEN, GOLDEN_N, INF, golden_case, read_changes, read_graph, read_ref_sssp
 
 
 def test_library_exports_every_header_symbol():
@@ -200,3 +200,22 @@ def test_gen_changes_grid():
     for x, y, z, _ in ins:
         md = abs(x // side - y // side) + abs(x % side - y % side)
         assert 1 <= md <= 3 and (min(x, y), max(x, y)) not in ek and 1 <= z <= 10000
+
+
+TOOLS = os.path.join(os.path.dirname(ds.LIB_PATH), "op_tools")
+
+
+def test_op_tools_apply_and_ce(tmp_path):
+    """`op_tools apply` gives the oracle's graph-after-batch; `op_tools ce` writes
+    op_cE-style change lines (createChangedEdges.cpp:104,116) that the reader takes."""
+    a, b, w, chg, n = golden_case("r300")
+    out = subprocess.check_output([TOOLS, "apply", os.path.join(GOLDEN, "r300_graph.txt"), "300",
+                                   os.path.join(GOLDEN, "r300_cE.txt")]).decode()
+    got = np.array([[int(x) for x in line.split()] for line in out.splitlines()], np.int32)
+    ea, eb, ew = oracle.apply_changes(300, a, b, w, chg)
+    assert np.array_equal(got, np.stack([ea, eb, ew], 1))
+    ce = tmp_path / "ce.txt"
+    ce.write_bytes(subprocess.check_output([TOOLS, "ce", os.path.join(GOLDEN, "r300_graph.txt"), "300", "100", "40",
+                                            "3"]))
+    c = ds.read_changes(str(ce), 300)
+    assert len(c) == 100 and int(np.sum(c[:, 3] == 1)) == 40 and c[:, 2].max() <= 99
