@@ -7,11 +7,13 @@
 //          dbl  uint64[n]     delta row descriptor (pool offset << 32 | length)
 //          pool uint64[cap]   delta entries appended by insertions (rows unsorted)
 //          tcnt uint32[n]     per-vertex insertion counters, zero between updates
-//   tree   st   uint64[n]     packed state (distance << 32 | parent)
-//          stamp uint32[n]    epoch of the last enqueue (frontier de-duplication)
-//          q    2 x uint32[n] light frontier queues (ping-pong)
-//          hq   2 x uint64[h] heavy-row chunk queues (vertex << 32 | chunk)
-//          invl uint32[n]     every vertex invalidated by the current update
+//          tbm  uint64[n/64]  bitmap of vertices touched by the batch's insertions
+//          wsum/wbase         per-bitmap-word sizes / pool offsets of one batch
+//   tree   st   uint64[S]     packed state (distance << 32 | parent), S = n * K'
+//                             slots (K' = trees rounded up to a power of two)
+//          fb   2 x uint64[S/64]  frontier bitmaps (ping-pong between rounds)
+//          ib   uint64[S/64]  bitmap of slots invalidated by the current update
+//          hq   2 x uint64[h] heavy-row chunks (slot << 32 | chunk)
 // The weight's bit 31 is the tombstone bit (weights are >= 0), playing the role
 // of the reference's wt = -1 (gpuFunctions_undir.cuh:203,210,321,384).
 #pragma once
@@ -39,35 +41,40 @@ DSSSP_HD uint32_t e_col(uint64_t e) { return (uint32_t)(e >> 32); }
 DSSSP_HD uint32_t e_w(uint64_t e) { return (uint32_t)e; }
 DSSSP_HD uint64_t e_key(uint64_t e) { return e & ~E_DEAD; }
 
-// Error bits raised by kernels (Counters::err).
+// Error bits raised by kernels (Counters::err / GraphCtl::err).
 constexpr uint32_t ERR_DIST_OVERFLOW = 1u;
 constexpr uint32_t ERR_POOL = 2u;
 constexpr uint32_t ERR_QUEUE = 4u;
 constexpr uint32_t ERR_NO_TIGHT = 8u;
 
-// Device-side control block of one tree: ring counters, statistics, errors.
+// Statistics, accumulated per workgroup slot (no single-address hot spot) and
+// summed into Counters::stats once per update.
+enum Stat : int {
+    ST_RELAX_FRONT = 0,  // frontier slots processed by relaxation
+    ST_RELAX_EDGES,      // adjacency entries scanned by relaxation
+    ST_RELAX_MARKS,      // successful improvements (bits set) by relaxation
+    ST_OTHER_EDGES,      // entries scanned by invalidation, seeding, repair
+    ST_INVALIDATED,      // slots seeded (= invalidated by the update)
+    ST_PROBES,           // entries read by deletion searches
+    NSTAT
+};
+constexpr int STAT_SLOTS = 2048;
+
+// Device-side control block of one tree.
 struct Counters {
-    uint32_t ring[3][2];  // (light, heavy) queue lengths, indexed by round % 3
-    uint32_t inv_total;   // length of invl
+    uint32_t act[3];    // a round marked its output bitmap; indexed by round % 3
+    uint32_t hcnt[3];   // heavy chunks queued for the next round; round % 3
+    uint32_t aux_hcnt;  // heavy chunks queued by a single-pass stage
     uint32_t err;
-    uint32_t aux_hcnt;    // heavy chunks queued by a single-pass stage
-    // Device-side round indexing of a loop phase: round i reads ring slot
-    // (ring_base + i) % 3 and queue buffer (q_base + i) & 1 and stamps with
-    // epoch_base + i; the last workgroup to finish a round advances iter.
-    uint32_t iter;
-    uint32_t blocks_done;
-    uint32_t ring_base;
-    uint32_t q_base;
-    uint32_t epoch_base;
-    unsigned long long relax_frontier, relax_edges, relax_improved, relax_enq;
-    unsigned long long other_edges, probes, tombstones;
+    unsigned long long stats[NSTAT];
 };
 
 // Device-side control block of one graph (delta pool allocation).
 struct GraphCtl {
-    unsigned long long pool_top;  // next free pool entry
-    uint32_t touched;             // vertices touched by the current batch's insertions
+    unsigned long long pool_top;   // next free pool entry
+    unsigned long long batch_new;  // pool entries placed by the current batch
     uint32_t err;
+    uint32_t pad;
 };
 
 }  // namespace dsssp

@@ -1,24 +1,26 @@
 // engine.hip -- device objects and the update pipeline behind the C ABI.
 //
 // Update pipeline (one HIP stream per graph, no host round-trip per round):
-//   1. insert   k_ins_count / k_ins_alloc / k_ins_scatter / k_ins_reset:
-//               the batch's insertions become delta-row entries in HBM
-//               (the reference appends them to its host adjacency before upload,
+//   1. insert   k_ins_count / wordsum / scan / alloc / scatter / reset: the
+//               batch's insertions become delta-row entries in HBM (the
+//               reference appends them to its host adjacency before upload,
 //               all_structure_undir.cuh:106-117)
-//   2. classify k_classify: tree-edge test + tombstones, roots of invalidated
-//               subtrees queued (deleteEdge, gpuFunctions_undir.cuh:179-215)
-//   3. invalidate rounds of k_frontier<M_INVAL> over the queue, descending the
-//               pre-update tree (updateNeighbors_del + filterAffectedNodes loop,
+//   2. classify k_classify: tree-edge test + tombstones; roots of invalidated
+//               subtrees marked (deleteEdge, gpuFunctions_undir.cuh:179-215)
+//   3. invalidate rounds of k_frontier<M_INVAL>, descending the pre-update tree
+//               (updateNeighbors_del + filterAffectedNodes loop,
 //               SSSPmainForUndirected.cu:219-239, gpuFunctions_undir.cuh:125-144,312-349)
-//   4. seed     k_frontier<M_SEED> over all invalidated vertices + k_ins_offer
+//   4. seed     k_frontier<M_SEED> over every invalidated slot + k_ins_offer
 //               (insertEdge, gpuFunctions_undir.cuh:217-251)
-//   5. relax    rounds of k_frontier<M_RELAX> until the frontier is empty
+//   5. relax    rounds of k_frontier<M_RELAX> until no slot is marked
 //               (updateNeighbors + filterAllAffected loop, :162-174,370-433,
 //               SSSPmainForUndirected.cu:263-281)
 //   6. repair   only for graphs that ever held a zero weight: tight-graph BFS
 //               levels and parent re-pick so the tree stays acyclic
-// Rounds are launched in groups; the host reads one 8-byte counter per group to
-// decide whether to continue (the reference syncs and reads UVM every round).
+// Round i of a loop phase reads frontier bitmap i % 2 and flag / chunk-count
+// slot i % 3 and writes the next ones, so a group of rounds has fixed arguments
+// and can be replayed from a captured hipGraph.  The host reads one flag pair
+// per group of rounds (the reference syncs and reads UVM every round).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,6 +29,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/dsssp.h"
@@ -38,7 +41,7 @@ using namespace dsssp;
 
 namespace {
 
-dsssp_tuning g_tuning = {FRONTIER_BLOCK, 2048, 256, 256, 4, 0, 0};
+dsssp_tuning g_tuning = {FRONTIER_BLOCK, 2048, 256, 256, 4, 0, 0, 4};
 std::mutex g_tuning_mu;
 thread_local int t_device = 0;
 
@@ -94,6 +97,8 @@ int check_device(int dev)
     return DSSSP_OK;
 }
 
+inline uint32_t words_of(uint64_t bits) { return (uint32_t)((bits + 63) / 64); }
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -104,12 +109,16 @@ struct dsssp_graph {
     int device = 0;
     hipStream_t stream = nullptr;
     int64_t n = 0;
-    uint64_t E = 0;  // base entries
+    uint32_t nw = 0;  // words of a vertex bitmap
+    uint64_t E = 0;   // base entries
     uint64_t *rp = nullptr, *adj = nullptr, *dbl = nullptr, *pool = nullptr;
     uint64_t pool_cap = 0;
     uint64_t pool_top = 0;       // host copy, valid between updates
     uint64_t delta_entries = 0;  // sum of delta row lengths
     uint32_t *tcnt = nullptr;
+    uint64_t *tbm = nullptr;  // vertices touched by the batch's insertions
+    uint32_t *wsum = nullptr;
+    uint64_t *wbase = nullptr;
     GraphCtl *gc = nullptr;
     GraphCtl *gc_host = nullptr;  // pinned
     bool zero_w = false;          // some weight was ever 0: parents need the tight-BFS repair
@@ -128,23 +137,23 @@ struct dsssp_graph {
 struct dsssp_tree {
     dsssp_graph *g = nullptr;
     int32_t src = -1;
-    int32_t ntrees = 1;         // trees sharing the graph (a forest of K sources)
-    uint32_t kshift = 0;        // slot = vertex << kshift | tree
-    uint64_t nslots = 0;        // n << kshift
+    int32_t ntrees = 1;   // trees sharing the graph (a forest of K sources)
+    uint32_t kshift = 0;  // slot = vertex << kshift | tree
+    uint64_t nslots = 0;  // n << kshift
+    uint32_t nw = 0;      // words of a slot bitmap
     uint64_t *st = nullptr;
-    uint32_t *stamp = nullptr;
-    uint32_t *q[2] = {nullptr, nullptr};
+    uint64_t *fb[2] = {nullptr, nullptr};  // frontier bitmaps
+    uint64_t *ib = nullptr;                // slots invalidated by the current update
     uint64_t *hq[2] = {nullptr, nullptr};
     uint32_t hcap = 0;
-    uint32_t *invl = nullptr;
     uint32_t *lvl = nullptr, *best = nullptr;
+    unsigned long long *bstats = nullptr;
     Counters *c = nullptr;
     Counters *c_host = nullptr;  // pinned
-    uint32_t epoch = 1;
-    // Captured groups of relaxation/invalidation rounds (hipGraph), keyed by
-    // mode, group size, grid and the exact kernel arguments.
+    // Captured groups of rounds (hipGraph), keyed by mode, first round index
+    // mod 6, group size, grid and the exact base arguments.
     struct Captured {
-        int mode, group, grid;
+        int mode, phase, group, grid;
         FArgs args;
         hipGraphExec_t exec;
     };
@@ -157,18 +166,12 @@ struct dsssp_batch {
     bool zero_w = false;
     int4 *rec = nullptr;
     uint64_t *slot = nullptr, *pos = nullptr;
-    uint32_t *touched = nullptr;
 };
 
 namespace {
 
 struct DevGuard {
     explicit DevGuard(int d) { (void)hipSetDevice(d); }
-};
-
-// Ring of frontier buffers: counters cycle mod 3, queues mod 2.
-struct Ring {
-    int cs = 0, qs = 0;
 };
 
 FArgs base_args(dsssp_tree *t)
@@ -178,100 +181,102 @@ FArgs base_args(dsssp_tree *t)
     memset(&a, 0, sizeof a);  // padding included: captured graphs are keyed by these bytes
     a.g = t->g->view();
     a.st = t->st;
-    a.stamp = t->stamp;
     a.lvl = t->lvl;
     a.best = t->best;
-    a.out_cap = (uint32_t)t->nslots;
-    a.kshift = t->kshift;
-    a.out_hcap = t->hcap;
-    a.inv_total = &t->c->inv_total;
-    a.invl = t->invl;
+    a.inv_bm = t->ib;
     a.c = t->c;
+    a.bstats = t->bstats;
+    a.nslots = (uint32_t)t->nslots;
+    a.nwords = t->nw;
+    a.hcap = t->hcap;
+    a.tiles = 1;
+    a.consume = 1;
     a.heavy = (uint32_t)tu.heavy_degree;
     a.chunk = (uint32_t)tu.chunk;
+    a.kshift = t->kshift;
+    a.ept = (uint32_t)tu.edges_per_thread;
     return a;
 }
 
-uint32_t next_epoch(dsssp_tree *t)
+// Arguments of round i of a loop phase; the producer stage before the loop
+// wrote bitmap 0, flag 0 and chunk count 0.
+FArgs round_args(dsssp_tree *t, const FArgs &base, int64_t i)
 {
-    if (t->epoch >= 0xF0000000u) {  // stamps restart well before the counter wraps
-        launch_fill_u32(t->stamp, 0u, t->nslots, t->g->stream);
-        t->epoch = 1;
-    }
-    return ++t->epoch;
+    FArgs a = base;
+    const int fi = (int)(i & 1), ci = (int)(i % 3);
+    a.in_bm = t->fb[fi];
+    a.in_hq = t->hq[fi];
+    a.in_hcnt = &t->c->hcnt[ci];
+    a.out_bm = t->fb[fi ^ 1];
+    a.out_act = &t->c->act[(ci + 1) % 3];
+    a.out_hq = t->hq[fi ^ 1];
+    a.out_hcnt = &t->c->hcnt[(ci + 1) % 3];
+    a.zero_act = &t->c->act[(ci + 2) % 3];
+    a.zero_hcnt = &t->c->hcnt[(ci + 2) % 3];
+    return a;
 }
 
-void zero_ring(dsssp_tree *t, Ring &r)
-{
-    launch_fill_u32(&t->c->ring[0][0], 0u, 6, t->g->stream);
-    r.cs = 0;
-    r.qs = 0;
-}
+// Clears the round flags and chunk counts (act[3], hcnt[3]) before a producer.
+void zero_ring(dsssp_tree *t) { launch_fill_u32(&t->c->act[0], 0u, 6, t->g->stream); }
 
-// Producer stage outputs go to ring slot cs+1 / queue qs^1.
-void producer_out(dsssp_tree *t, const Ring &r, uint32_t *&q, uint32_t *&cnt)
-{
-    q = t->q[r.qs ^ 1];
-    cnt = &t->c->ring[(r.cs + 1) % 3][0];
-}
-void advance(Ring &r)
-{
-    r.cs = (r.cs + 1) % 3;
-    r.qs ^= 1;
-}
-
-// One pass of a single-pass mode (seed, parent pick): rows longer than the
-// heavy threshold are queued as chunks by the first launch and processed by a
-// second, chunk-only launch.  Both heavy buffers are idle between loop phases.
+// One pass of a single-pass mode (seed, parent pick) into bitmap 0 / flag 0.
+// Rows longer than the heavy threshold are queued as chunks on list 1 (idle
+// here) by the first launch and processed by a second, chunk-only launch.
 void single_pass(dsssp_tree *t, int mode, FArgs a)
 {
     hipStream_t s = t->g->stream;
     const int grid = std::max(1, tuning().grid);
     launch_fill_u32(&t->c->aux_hcnt, 0u, 1, s);
-    a.out_hq = t->hq[0];
+    a.out_bm = t->fb[0];
+    a.out_act = &t->c->act[0];
+    a.out_hq = t->hq[1];
     a.out_hcnt = &t->c->aux_hcnt;
+    a.in_hq = nullptr;
+    a.in_hcnt = nullptr;
     launch_frontier(mode, a, grid, s);
     FArgs b = a;
-    b.in_q = nullptr;
-    b.in_cnt = nullptr;
-    b.in_n = 0;
-    b.in_hq = t->hq[0];
+    b.tiles = 0;
+    b.in_hq = t->hq[1];
     b.in_hcnt = &t->c->aux_hcnt;
+    b.out_hq = nullptr;
+    b.out_hcnt = nullptr;
     launch_frontier(mode, b, grid, s);
 }
 
-// Returns the instantiated hipGraph of `group` back-to-back rounds of `mode`
-// with arguments `a` (every round has the same arguments: round indexing is
-// on the device), capturing it on first use.
-int captured_rounds(dsssp_tree *t, int mode, int group, int grid, const FArgs &a, hipGraphExec_t *out)
+// Instantiated hipGraph of rounds first .. first+group-1 of `mode`.
+int captured_rounds(dsssp_tree *t, int mode, int64_t first, int group, int grid, const FArgs &base,
+                    hipGraphExec_t *out)
 {
+    const int phase = (int)(first % 6);
     for (auto &e : t->graphs)
-        if (e.mode == mode && e.group == group && e.grid == grid && !memcmp(&e.args, &a, sizeof a)) {
+        if (e.mode == mode && e.phase == phase && e.group == group && e.grid == grid &&
+            !memcmp(&e.args, &base, sizeof base)) {
             *out = e.exec;
             return DSSSP_OK;
         }
-    if (t->graphs.size() >= 24) {
+    if (t->graphs.size() >= 32) {
         (void)hipGraphExecDestroy(t->graphs.front().exec);
         t->graphs.erase(t->graphs.begin());
     }
     hipStream_t s = t->g->stream;
     hipGraph_t gr = nullptr;
     HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < group; ++i) launch_frontier(mode, a, grid, s);
+    for (int i = 0; i < group; ++i) launch_frontier(mode, round_args(t, base, first + i), grid, s);
     HIP_TRY(hipStreamEndCapture(s, &gr));
     hipGraphExec_t exec = nullptr;
     const hipError_t e = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
     (void)hipGraphDestroy(gr);
     if (e != hipSuccess) return fail(DSSSP_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
-    t->graphs.push_back({mode, group, grid, a, exec});
+    t->graphs.push_back({mode, phase, group, grid, base, exec});
     *out = exec;
     return DSSSP_OK;
 }
 
-// Launches frontier rounds until the current ring slot is empty.  Rounds run
-// back to back on the stream (as captured hipGraphs when use_graph is set);
-// the host reads one 8-byte counter pair per group of rounds.
-int run_rounds(dsssp_tree *t, int mode, Ring &r, int64_t &rounds, double *kernel_ms)
+// Runs rounds of `mode` until a round marks nothing and queues no chunk.
+// Rounds run back to back on the stream (replayed from captured hipGraphs when
+// use_graph is set); the host reads one flag pair per group of rounds.  A round
+// after convergence reads an empty bitmap and is a no-op.
+int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms)
 {
     dsssp_graph *g = t->g;
     hipStream_t s = g->stream;
@@ -280,59 +285,45 @@ int run_rounds(dsssp_tree *t, int mode, Ring &r, int64_t &rounds, double *kernel
     const int max_group = 64;
     const int grid = std::max(1, tu.grid);
     const bool timed = tu.kernel_timing && kernel_ms;
-    const int64_t start_rounds = rounds;
-    if ((uint64_t)t->epoch + (uint64_t)g->n + 256 >= 0xF0000000ull) {  // stamps restart before the epoch wraps
-        launch_fill_u32(t->stamp, 0u, t->nslots, s);
-        t->epoch = 1;
-    }
-    FArgs a = base_args(t);
-    a.dev_iter = 1;
-    a.qv[0] = t->q[0];
-    a.qv[1] = t->q[1];
-    a.hqv[0] = t->hq[0];
-    a.hqv[1] = t->hq[1];
-    const uint32_t ep0 = t->epoch + 1;
-    launch_phase_init(t->c, (uint32_t)r.cs, (uint32_t)r.qs, ep0, s);
-    uint32_t launched = 0;
+    const FArgs base = base_args(t);
+    int64_t i = 0;
     for (;;) {
         if (timed) {
             if (g->kev.size() < (size_t)(2 * group)) {
                 const size_t old = g->kev.size();
                 g->kev.resize((size_t)(2 * group));
-                for (size_t i = old; i < g->kev.size(); ++i) HIP_TRY(hipEventCreate(&g->kev[i]));
+                for (size_t j = old; j < g->kev.size(); ++j) HIP_TRY(hipEventCreate(&g->kev[j]));
             }
-            for (int i = 0; i < group; ++i) {
-                HIP_TRY(hipEventRecord(g->kev[2 * i], s));
-                launch_frontier(mode, a, grid, s);
-                HIP_TRY(hipEventRecord(g->kev[2 * i + 1], s));
+            for (int j = 0; j < group; ++j) {
+                HIP_TRY(hipEventRecord(g->kev[2 * j], s));
+                launch_frontier(mode, round_args(t, base, i + j), grid, s);
+                HIP_TRY(hipEventRecord(g->kev[2 * j + 1], s));
             }
         } else if (tu.use_graph) {
             hipGraphExec_t exec = nullptr;
-            if (int e = captured_rounds(t, mode, group, grid, a, &exec)) return e;
+            if (int e = captured_rounds(t, mode, i, group, grid, base, &exec)) return e;
             HIP_TRY(hipGraphLaunch(exec, s));
         } else {
-            for (int i = 0; i < group; ++i) launch_frontier(mode, a, grid, s);
+            for (int j = 0; j < group; ++j) launch_frontier(mode, round_args(t, base, i + j), grid, s);
         }
-        launched += (uint32_t)group;
+        i += group;
         rounds += group;
-        r.cs = (r.cs + group) % 3;
-        r.qs ^= group & 1;
-        t->epoch = ep0 + launched - 1;
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(t->c_host->ring[r.cs], t->c->ring[r.cs], 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        const int ci = (int)(i % 3);  // flag / chunk slot written by the last round
+        HIP_TRY(hipMemcpyAsync(&t->c_host->act[ci], &t->c->act[ci], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(&t->c_host->hcnt[ci], &t->c->hcnt[ci], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         if (timed) {
-            for (int i = 0; i < group; ++i) {
+            for (int j = 0; j < group; ++j) {
                 float ms = 0;
-                HIP_TRY(hipEventElapsedTime(&ms, g->kev[2 * i], g->kev[2 * i + 1]));
+                HIP_TRY(hipEventElapsedTime(&ms, g->kev[2 * j], g->kev[2 * j + 1]));
                 *kernel_ms += ms;
             }
         }
-        if (t->c_host->ring[r.cs][0] == 0 && t->c_host->ring[r.cs][1] == 0) break;
+        if (t->c_host->act[ci] == 0 && t->c_host->hcnt[ci] == 0) break;
         // Every round settles at least one more hop of every shortest path, so
         // n + 64 rounds can only be exceeded by an internal error.
-        if (rounds - start_rounds > g->n + 64)
-            return fail(DSSSP_ERR_STATE, "frontier did not empty after %lld rounds", (long long)(rounds - start_rounds));
+        if (i > g->n + 64) return fail(DSSSP_ERR_STATE, "frontier did not empty after %lld rounds", (long long)i);
         group = std::min(group * 2, max_group);
     }
     return DSSSP_OK;
@@ -358,23 +349,17 @@ int repick_parents(dsssp_tree *t, bool use_lvl, int64_t &rounds)
     if (int s = ensure_pick_buffers(t)) return s;
     const uint32_t ns = (uint32_t)t->nslots;
     if (use_lvl) {
-        Ring r;
-        zero_ring(t, r);
-        uint32_t *oq, *oc;
-        producer_out(t, r, oq, oc);
-        launch_lvl_init(ns, t->kshift, t->st, t->lvl, t->best, t->stamp, next_epoch(t), oq, oc, ns, t->c, g->stream);
-        advance(r);
-        if (int s = run_rounds(t, M_TBFS, r, rounds, nullptr)) return s;
+        zero_ring(t);
+        launch_lvl_init(ns, t->kshift, t->st, t->lvl, t->best, t->fb[0], &t->c->act[0], g->stream);
+        if (int s = run_rounds(t, M_TBFS, rounds, nullptr)) return s;
     } else {
         launch_fill_u32(t->best, 0xFFFFFFFFu, ns, g->stream);
     }
     FArgs a = base_args(t);
-    a.lvl = t->lvl;
-    a.best = t->best;
-    a.in_q = nullptr;
-    a.in_cnt = nullptr;
-    a.in_n = ns;
+    a.identity = 1;
+    a.consume = 0;
     a.use_lvl = use_lvl ? 1u : 0u;
+    zero_ring(t);
     single_pass(t, M_PICK, a);
     launch_apply_best(ns, t->kshift, t->st, t->best, t->c, g->stream);
     HIP_TRY(hipGetLastError());
@@ -385,20 +370,20 @@ int ensure_hq(dsssp_tree *t, uint64_t total_entries)
 {
     const dsssp_tuning tu = tuning();
     // A row longer than `heavy` gives ceil(deg/chunk) <= deg/chunk + 1 chunks and
-    // at most entries/heavy rows are that long; every slot is queued at most
+    // at most entries/heavy rows are that long; a slot is processed at most
     // once per round.
     const uint64_t per_tree = total_entries / (uint64_t)std::max(64, tu.chunk) +
                               total_entries / (uint64_t)std::max(64, tu.heavy_degree) + 1;
     const uint64_t need = (per_tree << t->kshift) + 1024;
-    if (need > 0xFFFFFFFFull) return fail(DSSSP_ERR_ARG, "heavy queue would exceed 2^32 entries; raise chunk");
+    if (need > 0xFFFFFFFFull) return fail(DSSSP_ERR_ARG, "heavy chunk list would exceed 2^32 entries; raise chunk");
     if (t->hcap >= need) return DSSSP_OK;
     dev_free(t->hq[0]);
     dev_free(t->hq[1]);
     t->hcap = 0;
-    const uint64_t cap = need + need / 4;
-    if (int s = dev_alloc((void **)&t->hq[0], sizeof(uint64_t) * cap, "heavy queue")) return s;
-    if (int s = dev_alloc((void **)&t->hq[1], sizeof(uint64_t) * cap, "heavy queue")) return s;
-    t->hcap = (uint32_t)std::min<uint64_t>(cap, 0xFFFFFFFFull);
+    const uint64_t cap = std::min<uint64_t>(need + need / 4, 0xFFFFFFFFull);
+    if (int s = dev_alloc((void **)&t->hq[0], sizeof(uint64_t) * cap, "heavy chunk list")) return s;
+    if (int s = dev_alloc((void **)&t->hq[1], sizeof(uint64_t) * cap, "heavy chunk list")) return s;
+    t->hcap = (uint32_t)cap;
     return DSSSP_OK;
 }
 
@@ -410,7 +395,7 @@ int ensure_pool(dsssp_graph *g, uint64_t add)
     uint64_t cap = std::max<uint64_t>(g->pool_cap, 2 * (g->delta_entries + add) + (1u << 20));
     if (cap > 0xFFFFFFFFull) cap = 0xFFFFFFFFull;
     if (g->delta_entries + add > cap) return fail(DSSSP_ERR_NOMEM, "delta pool would exceed 2^32 entries");
-    // Compact on the host: rare, and keeps the kernels free of pool bookkeeping.
+    // Compacted on the host: rare, and keeps the kernels free of pool bookkeeping.
     std::vector<uint64_t> dbl((size_t)g->n), pool((size_t)g->pool_top), out;
     HIP_TRY(hipMemcpyAsync(dbl.data(), g->dbl, sizeof(uint64_t) * (size_t)g->n, hipMemcpyDeviceToHost, g->stream));
     if (g->pool_top)
@@ -456,17 +441,29 @@ int finish_errors(dsssp_tree *t, uint32_t gerr)
     const uint32_t err = t->c_host->err | gerr;
     if (err & ERR_DIST_OVERFLOW) return fail(DSSSP_ERR_OVERFLOW, "a distance reached 2^32-1; the tree is not exact");
     if (err & ERR_POOL) return fail(DSSSP_ERR_STATE, "delta pool overflow (internal)");
-    if (err & ERR_QUEUE) return fail(DSSSP_ERR_STATE, "frontier queue overflow (internal)");
+    if (err & ERR_QUEUE) return fail(DSSSP_ERR_STATE, "heavy chunk list overflow (internal)");
     if (err & ERR_NO_TIGHT) return fail(DSSSP_ERR_TREE, "a reachable vertex has no tight in-edge: distances not exact");
     return DSSSP_OK;
+}
+
+// After a failed update or setup: every bitmap, flag and insertion counter is
+// cleared so later calls start from a consistent state.
+void scrub(dsssp_tree *t)
+{
+    hipStream_t s = t->g->stream;
+    launch_fill_u64(t->fb[0], 0ull, t->nw, s);
+    launch_fill_u64(t->fb[1], 0ull, t->nw, s);
+    launch_fill_u64(t->ib, 0ull, t->nw, s);
+    launch_fill_u64(t->g->tbm, 0ull, t->g->nw, s);
+    launch_fill_u32(t->g->tcnt, 0u, (uint64_t)t->g->n, s);
+    zero_ring(t);
+    (void)hipStreamSynchronize(s);
 }
 
 int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stats, double h2d_ms)
 {
     DevGuard dg(g->device);
     hipStream_t s = g->stream;
-    const uint32_t n = (uint32_t)g->n;
-    const uint32_t ns = (uint32_t)t->nslots;
     if (b->n_ins > 0) {
         if (int e = ensure_pool(g, 2 * (uint64_t)b->n_ins)) return e;
     }
@@ -475,60 +472,47 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
     double relax_kernel_ms = 0;
 
     HIP_TRY(hipEventRecord(g->ev[0], s));
-    launch_reset_counters(t->c, s);
+    launch_fill_u32((uint32_t *)t->c, 0u, sizeof(Counters) / 4, s);
     // 1. insertions into the resident CSR
     if (b->n_ins > 0) {
-        const uint32_t max_touched = (uint32_t)std::min<uint64_t>((uint64_t)n, 2 * (uint64_t)b->n_ins);
-        launch_reset_graphctl(g->gc, s);
-        launch_ins_count(b->rec, b->k, g->tcnt, b->slot, b->touched, g->gc, s);
-        launch_ins_alloc(b->touched, max_touched, g->dbl, g->tcnt, g->pool, g->pool_cap, g->gc, s);
-        launch_ins_scatter(b->rec, b->k, b->slot, g->dbl, g->tcnt, g->pool, b->pos, s);
-        launch_ins_reset(b->touched, max_touched, g->tcnt, g->gc, s);
+        launch_fill_u32(&g->gc->err, 0u, 1, s);
+        launch_ins_count(b->rec, b->k, g->tcnt, b->slot, g->tbm, s);
+        launch_ins_wordsum(g->tbm, g->nw, g->dbl, g->tcnt, g->wsum, s);
+        launch_ins_scan(g->wsum, g->nw, g->wbase, g->pool_cap, g->gc, s);
+        launch_ins_alloc(g->tbm, g->nw, g->wbase, g->dbl, g->tcnt, g->pool, g->gc, s);
+        launch_ins_scatter(b->rec, b->k, b->slot, g->dbl, g->tcnt, g->pool, b->pos, g->gc, s);
+        launch_ins_reset(g->tbm, g->nw, g->tcnt, s);
         if (b->zero_w) g->zero_w = true;
     }
     HIP_TRY(hipEventRecord(g->ev[1], s));
     // 2-3. deletions: classify, then invalidate the subtrees
-    Ring r;
-    zero_ring(t, r);
+    int code = DSSSP_OK;
     if (b->n_del > 0) {
-        uint32_t *oq, *oc;
-        producer_out(t, r, oq, oc);
-        launch_classify(b->rec, b->k, t->kshift, g->view(), t->st, oq, oc, t->invl, &t->c->inv_total, ns, t->c, s);
-        advance(r);
+        zero_ring(t);
+        launch_classify(b->rec, b->k, t->kshift, g->view(), t->st, t->fb[0], t->ib, &t->c->act[0], t->bstats, s);
     }
     HIP_TRY(hipEventRecord(g->ev[2], s));
-    if (b->n_del > 0) {
-        if (int e = run_rounds(t, M_INVAL, r, inv_rounds, nullptr)) return e;
-    }
+    if (b->n_del > 0) code = run_rounds(t, M_INVAL, inv_rounds, nullptr);
     HIP_TRY(hipEventRecord(g->ev[3], s));
-    // 4. seed invalidated vertices from valid neighbours, offer inserted edges
-    zero_ring(t, r);
-    {
-        uint32_t *oq, *oc;
-        producer_out(t, r, oq, oc);
-        const uint32_t ep = next_epoch(t);
+    // 4. seed invalidated slots from valid neighbours, offer inserted edges
+    if (!code) {
+        zero_ring(t);
         if (b->n_del > 0) {
             FArgs a = base_args(t);
-            a.in_q = t->invl;
-            a.in_cnt = &t->c->inv_total;
-            a.out_q = oq;
-            a.out_cnt = oc;
-            a.epoch = ep;
+            a.in_bm = t->ib;  // consumed, so clean for the next update
             single_pass(t, M_SEED, a);
         }
         if (b->n_ins > 0)
-            launch_ins_offer(b->rec, b->k, t->kshift, b->pos, g->pool, t->st, t->stamp, ep, oq, oc, ns, t->c, s);
-        advance(r);
+            launch_ins_offer(b->rec, b->k, t->kshift, b->pos, g->pool, t->st, t->fb[0], &t->c->act[0], t->c, s);
     }
     HIP_TRY(hipEventRecord(g->ev[4], s));
     // 5. relaxation
-    if (int e = run_rounds(t, M_RELAX, r, relax_rounds, &relax_kernel_ms)) return e;
+    if (!code) code = run_rounds(t, M_RELAX, relax_rounds, &relax_kernel_ms);
     HIP_TRY(hipEventRecord(g->ev[5], s));
     // 6. zero-weight parent repair
-    if (g->zero_w) {
-        if (int e = repick_parents(t, true, repair_rounds)) return e;
-    }
+    if (!code && g->zero_w) code = repick_parents(t, true, repair_rounds);
     HIP_TRY(hipEventRecord(g->ev[6], s));
+    launch_reduce_stats(t->c, t->bstats, s);
     HIP_TRY(hipMemcpyAsync(t->c_host, t->c, sizeof(Counters), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(g->gc_host, g->gc, sizeof(GraphCtl), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -537,22 +521,27 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
         g->pool_top = g->gc_host->pool_top;
         g->delta_entries += 2 * (uint64_t)b->n_ins;
     }
-    const int st = finish_errors(t, b->n_ins > 0 ? g->gc_host->err : 0u);
+    if (!code) code = finish_errors(t, b->n_ins > 0 ? g->gc_host->err : 0u);
+    if (code) {
+        const std::string keep = last_error();
+        scrub(t);
+        return fail(code, "%s", keep.c_str());
+    }
     if (stats) {
         const Counters &c = *t->c_host;
         dsssp_stats S{};
         S.n_insert = b->n_ins;
         S.n_delete = b->n_del;
-        S.n_invalidated = c.inv_total;
+        S.n_invalidated = (int64_t)c.stats[ST_INVALIDATED];
         S.inv_rounds = inv_rounds;
         S.relax_rounds = relax_rounds;
         S.relax_launches = relax_rounds;
-        S.frontier_vertices = (int64_t)c.relax_frontier;
-        S.relax_edges = (int64_t)c.relax_edges;
-        S.relax_improved = (int64_t)c.relax_improved;
-        S.relax_enqueued = (int64_t)c.relax_enq;
-        S.other_edges = (int64_t)c.other_edges;
-        S.probes = (int64_t)c.probes;
+        S.frontier_vertices = (int64_t)c.stats[ST_RELAX_FRONT];
+        S.relax_edges = (int64_t)c.stats[ST_RELAX_EDGES];
+        S.relax_improved = (int64_t)c.stats[ST_RELAX_MARKS];
+        S.relax_enqueued = (int64_t)c.stats[ST_RELAX_MARKS];
+        S.other_edges = (int64_t)c.stats[ST_OTHER_EDGES];
+        S.probes = (int64_t)c.stats[ST_PROBES];
         S.t_h2d_ms = h2d_ms;
         S.t_insert_ms = elapsed(g->ev[0], g->ev[1]);
         S.t_classify_ms = elapsed(g->ev[1], g->ev[2]);
@@ -561,17 +550,21 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
         S.t_relax_ms = elapsed(g->ev[4], g->ev[5]) + elapsed(g->ev[5], g->ev[6]);
         S.t_update_ms = elapsed(g->ev[0], g->ev[6]);
         S.t_relax_kernel_ms = relax_kernel_ms;
-        // Algorithmic bytes: per frontier vertex 4 B id + 16 B row bounds +
+        // Algorithmic bytes of relaxation: per frontier slot 16 B row bounds +
         // 8 B delta descriptor + 8 B own state; per scanned entry 8 B entry +
-        // 8 B neighbour state; per improvement 8 B; per enqueue 4 B stamp + 4 B id.
-        const double relax = 36.0 * c.relax_frontier + 16.0 * c.relax_edges + 8.0 * c.relax_improved + 8.0 * c.relax_enq;
-        const double other = 16.0 * c.other_edges + 2 * 36.0 * c.inv_total + 16.0 * c.inv_total +
-                             8.0 * c.probes + 32.0 * b->n_del + 72.0 * b->n_ins;
+        // 8 B neighbour state; per improvement 8 B state + 8 B bitmap word.
+        // Each round reads (and clears) its input bitmap: 8 B per word.
+        const double fr = (double)c.stats[ST_RELAX_FRONT], ed = (double)c.stats[ST_RELAX_EDGES];
+        const double mk = (double)c.stats[ST_RELAX_MARKS];
+        const double relax = 32.0 * fr + 16.0 * ed + 16.0 * mk + 8.0 * t->nw * (double)relax_rounds;
+        const double other = 16.0 * c.stats[ST_OTHER_EDGES] + 2 * 32.0 * c.stats[ST_INVALIDATED] +
+                             8.0 * c.stats[ST_PROBES] + 32.0 * b->n_del + 72.0 * b->n_ins +
+                             8.0 * t->nw * (double)inv_rounds;
         S.alg_bytes_relax = relax;
         S.alg_bytes = relax + other;
         *stats = S;
     }
-    return st;
+    return DSSSP_OK;
 }
 
 int validate_changes(int64_t n, const dsssp_change *chg, int64_t k, int64_t &ni, int64_t &nd, bool &zero_w)
@@ -604,7 +597,7 @@ int validate_changes(int64_t n, const dsssp_change *chg, int64_t k, int64_t &ni,
 // ---------------------------------------------------------------------------
 
 extern "C" const char *dsssp_last_error(void) { return last_error(); }
-extern "C" const char *dsssp_version(void) { return "dsssp-mi355x 0.1 (gfx950)"; }
+extern "C" const char *dsssp_version(void) { return "dsssp-mi355x 0.2 (gfx950)"; }
 
 extern "C" int dsssp_set_device(int device)
 {
@@ -633,10 +626,12 @@ extern "C" int dsssp_set_tuning(const dsssp_tuning *t)
     if (t->heavy_degree) nt.heavy_degree = t->heavy_degree;
     if (t->chunk) nt.chunk = t->chunk;
     if (t->round_batch) nt.round_batch = t->round_batch;
+    if (t->edges_per_thread) nt.edges_per_thread = t->edges_per_thread;
     nt.kernel_timing = t->kernel_timing;
     nt.use_graph = t->use_graph;
     if (nt.grid < 1 || nt.grid > 65536 || nt.heavy_degree < 64 || nt.chunk < 64 || nt.chunk % 64 ||
-        nt.round_batch < 1 || nt.round_batch > 64)
+        nt.round_batch < 1 || nt.round_batch > 64 ||
+        (nt.edges_per_thread != 1 && nt.edges_per_thread != 2 && nt.edges_per_thread != 4))
         return fail(DSSSP_ERR_ARG, "tuning out of range");
     g_tuning = nt;
     return DSSSP_OK;
@@ -670,6 +665,9 @@ static void graph_release(dsssp_graph *g)
     dev_free(g->dbl);
     dev_free(g->pool);
     dev_free(g->tcnt);
+    dev_free(g->tbm);
+    dev_free(g->wsum);
+    dev_free(g->wbase);
     dev_free(g->gc);
     if (g->gc_host) (void)hipHostFree(g->gc_host);
     for (auto &e : g->ev)
@@ -685,15 +683,23 @@ static int graph_alloc(dsssp_graph *g, uint64_t pool_cap)
     HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     for (auto &e : g->ev) HIP_TRY(hipEventCreate(&e));
     const size_t n = (size_t)g->n;
+    g->nw = words_of(n);
     if (int s = dev_alloc((void **)&g->rp, sizeof(uint64_t) * (n + 1), "row offsets")) return s;
     if (int s = dev_alloc((void **)&g->adj, sizeof(uint64_t) * g->E, "adjacency")) return s;
     if (int s = dev_alloc((void **)&g->dbl, sizeof(uint64_t) * n, "delta rows")) return s;
     if (int s = dev_alloc((void **)&g->tcnt, sizeof(uint32_t) * n, "insert counters")) return s;
+    if (int s = dev_alloc((void **)&g->tbm, sizeof(uint64_t) * g->nw, "touched bitmap")) return s;
+    if (int s = dev_alloc((void **)&g->wsum, sizeof(uint32_t) * g->nw, "word sizes")) return s;
+    if (int s = dev_alloc((void **)&g->wbase, sizeof(uint64_t) * g->nw, "word bases")) return s;
     if (int s = dev_alloc((void **)&g->gc, sizeof(GraphCtl), "graph control")) return s;
     g->pool_cap = pool_cap;
     if (int s = dev_alloc((void **)&g->pool, sizeof(uint64_t) * pool_cap, "delta pool")) return s;
     HIP_TRY(hipHostMalloc((void **)&g->gc_host, sizeof(GraphCtl), hipHostMallocDefault));
     memset(g->gc_host, 0, sizeof(GraphCtl));
+    HIP_TRY(hipMemset(g->dbl, 0, sizeof(uint64_t) * n));
+    HIP_TRY(hipMemset(g->tcnt, 0, sizeof(uint32_t) * n));
+    HIP_TRY(hipMemset(g->tbm, 0, sizeof(uint64_t) * g->nw));
+    HIP_TRY(hipMemset(g->gc, 0, sizeof(GraphCtl)));
     return DSSSP_OK;
 }
 
@@ -719,14 +725,9 @@ extern "C" int dsssp_graph_create(int64_t n, int64_t m, const int32_t *a, const 
         return s;
     }
     DevGuard dg(g->device);
-    int s = DSSSP_OK;
     if (hipMemcpy(g->rp, rp.data(), sizeof(uint64_t) * rp.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        (g->E && hipMemcpy(g->adj, adj.data(), sizeof(uint64_t) * g->E, hipMemcpyHostToDevice) != hipSuccess) ||
-        hipMemset(g->dbl, 0, sizeof(uint64_t) * (size_t)n) != hipSuccess ||
-        hipMemset(g->tcnt, 0, sizeof(uint32_t) * (size_t)n) != hipSuccess ||
-        hipMemset(g->gc, 0, sizeof(GraphCtl)) != hipSuccess)
-        s = fail(DSSSP_ERR_HIP, "graph upload failed: %s", hipGetErrorString(hipGetLastError()));
-    if (s) {
+        (g->E && hipMemcpy(g->adj, adj.data(), sizeof(uint64_t) * g->E, hipMemcpyHostToDevice) != hipSuccess)) {
+        const int s = fail(DSSSP_ERR_HIP, "graph upload failed: %s", hipGetErrorString(hipGetLastError()));
         graph_release(g);
         return s;
     }
@@ -760,7 +761,8 @@ extern "C" int dsssp_graph_info(const dsssp_graph *g, int64_t *n, int64_t *m_liv
         *m_live = m;
     }
     if (device_bytes)
-        *device_bytes = 8 * ((uint64_t)g->n + 1) + 8 * g->E + 8 * (uint64_t)g->n + 8 * g->pool_cap + 4 * (uint64_t)g->n;
+        *device_bytes = 8 * ((uint64_t)g->n + 1) + 8 * g->E + 12 * (uint64_t)g->n + 20 * (uint64_t)g->nw +
+                        8 * g->pool_cap;
     return DSSSP_OK;
 }
 
@@ -843,6 +845,7 @@ extern "C" int dsssp_graph_copy(dsssp_graph *dst, dsssp_graph *src)
         HIP_TRY(hipMemcpyAsync(dst->pool, src->pool, sizeof(uint64_t) * src->pool_top, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemcpyAsync(dst->gc, src->gc, sizeof(GraphCtl), hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemsetAsync(dst->tcnt, 0, sizeof(uint32_t) * n, s));
+    HIP_TRY(hipMemsetAsync(dst->tbm, 0, sizeof(uint64_t) * dst->nw, s));
     HIP_TRY(hipStreamSynchronize(s));
     dst->pool_top = src->pool_top;
     dst->delta_entries = src->delta_entries;
@@ -854,21 +857,23 @@ static void tree_release(dsssp_tree *t)
 {
     if (!t) return;
     dsssp_graph *g = t->g;
-    DevGuard dg(t->g->device);
-    (void)hipStreamSynchronize(t->g->stream);
-    dev_free(t->st);
-    dev_free(t->stamp);
-    dev_free(t->q[0]);
-    dev_free(t->q[1]);
-    dev_free(t->hq[0]);
-    dev_free(t->hq[1]);
-    dev_free(t->invl);
-    dev_free(t->lvl);
-    dev_free(t->best);
-    dev_free(t->c);
-    if (t->c_host) (void)hipHostFree(t->c_host);
-    for (auto &e : t->graphs) (void)hipGraphExecDestroy(e.exec);
-    delete t;
+    {
+        DevGuard dg(g->device);
+        (void)hipStreamSynchronize(g->stream);
+        dev_free(t->st);
+        dev_free(t->fb[0]);
+        dev_free(t->fb[1]);
+        dev_free(t->ib);
+        dev_free(t->hq[0]);
+        dev_free(t->hq[1]);
+        dev_free(t->lvl);
+        dev_free(t->best);
+        dev_free(t->bstats);
+        dev_free(t->c);
+        if (t->c_host) (void)hipHostFree(t->c_host);
+        for (auto &e : t->graphs) (void)hipGraphExecDestroy(e.exec);
+        delete t;
+    }
     graph_detach(g);
 }
 
@@ -876,16 +881,21 @@ static int tree_alloc(dsssp_tree *t)
 {
     dsssp_graph *g = t->g;
     DevGuard dg(g->device);
-    const size_t n = (size_t)t->nslots;
-    if (int s = dev_alloc((void **)&t->st, sizeof(uint64_t) * n, "tree state")) return s;
-    if (int s = dev_alloc((void **)&t->stamp, sizeof(uint32_t) * n, "stamps")) return s;
-    if (int s = dev_alloc((void **)&t->q[0], sizeof(uint32_t) * n, "frontier")) return s;
-    if (int s = dev_alloc((void **)&t->q[1], sizeof(uint32_t) * n, "frontier")) return s;
-    if (int s = dev_alloc((void **)&t->invl, sizeof(uint32_t) * n, "invalidated list")) return s;
+    const size_t ns = (size_t)t->nslots;
+    t->nw = words_of(ns);
+    if (int s = dev_alloc((void **)&t->st, sizeof(uint64_t) * ns, "tree state")) return s;
+    if (int s = dev_alloc((void **)&t->fb[0], sizeof(uint64_t) * t->nw, "frontier bitmap")) return s;
+    if (int s = dev_alloc((void **)&t->fb[1], sizeof(uint64_t) * t->nw, "frontier bitmap")) return s;
+    if (int s = dev_alloc((void **)&t->ib, sizeof(uint64_t) * t->nw, "invalidated bitmap")) return s;
+    if (int s = dev_alloc((void **)&t->bstats, sizeof(unsigned long long) * STAT_SLOTS * NSTAT, "statistics"))
+        return s;
     if (int s = dev_alloc((void **)&t->c, sizeof(Counters), "counters")) return s;
     HIP_TRY(hipHostMalloc((void **)&t->c_host, sizeof(Counters), hipHostMallocDefault));
     memset(t->c_host, 0, sizeof(Counters));
-    HIP_TRY(hipMemsetAsync(t->stamp, 0, sizeof(uint32_t) * n, g->stream));
+    HIP_TRY(hipMemsetAsync(t->fb[0], 0, sizeof(uint64_t) * t->nw, g->stream));
+    HIP_TRY(hipMemsetAsync(t->fb[1], 0, sizeof(uint64_t) * t->nw, g->stream));
+    HIP_TRY(hipMemsetAsync(t->ib, 0, sizeof(uint64_t) * t->nw, g->stream));
+    HIP_TRY(hipMemsetAsync(t->bstats, 0, sizeof(unsigned long long) * STAT_SLOTS * NSTAT, g->stream));
     HIP_TRY(hipMemsetAsync(t->c, 0, sizeof(Counters), g->stream));
     if (int s = ensure_hq(t, g->E + g->delta_entries)) return s;
     return DSSSP_OK;
@@ -902,8 +912,9 @@ static int tree_make(dsssp_graph *g, int32_t ntrees, const std::vector<uint32_t>
     const int64_t n = g->n;
     uint32_t kshift = 0;
     while ((1 << kshift) < ntrees) ++kshift;
-    if (((uint64_t)n << kshift) >= 0xFFFFFFFFull)
-        return fail(DSSSP_ERR_ARG, "n * trees must stay below 2^32 (n=%lld, trees=%d)", (long long)n, ntrees);
+    // Slot ids, bitmap bit indices and chunk-list slot fields are 32-bit.
+    if (((uint64_t)n << kshift) > 0xFFFFFF00ull)
+        return fail(DSSSP_ERR_ARG, "n * trees must stay below 2^32 - 256 (n=%lld, trees=%d)", (long long)n, ntrees);
     std::vector<uint64_t> packed;
     if (parent) {
         if (!dist) return fail(DSSSP_ERR_ARG, "parent given without dist");
@@ -941,7 +952,7 @@ static int tree_make(dsssp_graph *g, int32_t ntrees, const std::vector<uint32_t>
     int64_t rounds = 0;
     int s = DSSSP_OK;
     uint32_t *d_roots = nullptr;
-    launch_reset_counters(t->c, g->stream);
+    launch_fill_u32((uint32_t *)t->c, 0u, sizeof(Counters) / 4, g->stream);
     if (parent) {
         if (hipMemcpyAsync(t->st, packed.data(), sizeof(uint64_t) * (size_t)n, hipMemcpyHostToDevice, g->stream) !=
             hipSuccess)
@@ -956,28 +967,25 @@ static int tree_make(dsssp_graph *g, int32_t ntrees, const std::vector<uint32_t>
                            g->stream) != hipSuccess)
             s = fail(DSSSP_ERR_HIP, "root upload failed");
         if (!s) {
-            Ring r;
-            zero_ring(t, r);
-            uint32_t *oq, *oc;
-            producer_out(t, r, oq, oc);
-            launch_set_sources(t->st, d_roots, (uint32_t)roots.size(), t->kshift, per_tree, t->stamp, next_epoch(t),
-                               oq, oc, (uint32_t)t->nslots, t->c, g->stream);
-            advance(r);
-            s = run_rounds(t, M_RELAX, r, rounds, nullptr);
+            zero_ring(t);
+            launch_set_sources(t->st, d_roots, (uint32_t)roots.size(), t->kshift, per_tree, t->fb[0], &t->c->act[0],
+                               g->stream);
+            s = run_rounds(t, M_RELAX, rounds, nullptr);
         }
         if (!s && g->zero_w) s = repick_parents(t, true, rounds);
     }
-    if (!s) {
-        if (hipMemcpyAsync(t->c_host, t->c, sizeof(Counters), hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
-            hipStreamSynchronize(g->stream) != hipSuccess)
-            s = fail(DSSSP_ERR_HIP, "tree setup failed: %s", hipGetErrorString(hipGetLastError()));
-        else
-            s = finish_errors(t, 0);
+    launch_reduce_stats(t->c, t->bstats, g->stream);
+    if (hipMemcpyAsync(t->c_host, t->c, sizeof(Counters), hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+        hipStreamSynchronize(g->stream) != hipSuccess) {
+        if (!s) s = fail(DSSSP_ERR_HIP, "tree setup failed: %s", hipGetErrorString(hipGetLastError()));
+    } else if (!s) {
+        s = finish_errors(t, 0);
     }
     dev_free(d_roots);
     if (s) {
+        const std::string keep = last_error();
         tree_release(t);
-        return s;
+        return fail(s, "%s", keep.c_str());
     }
     *out = t;
     return DSSSP_OK;
@@ -1077,9 +1085,7 @@ extern "C" int dsssp_tree_copy(dsssp_tree *dst, dsssp_tree *src)
     HIP_TRY(hipStreamSynchronize(src->g->stream));
     const size_t ns = (size_t)src->nslots;
     HIP_TRY(hipMemcpyAsync(dst->st, src->st, sizeof(uint64_t) * ns, hipMemcpyDeviceToDevice, dst->g->stream));
-    HIP_TRY(hipMemcpyAsync(dst->stamp, src->stamp, sizeof(uint32_t) * ns, hipMemcpyDeviceToDevice, dst->g->stream));
     HIP_TRY(hipStreamSynchronize(dst->g->stream));
-    dst->epoch = std::max(dst->epoch, src->epoch);
     dst->src = src->src;
     return DSSSP_OK;
 }
@@ -1103,15 +1109,14 @@ extern "C" int dsssp_batch_upload(dsssp_graph *g, const dsssp_change *chg, int64
     int s = dev_alloc((void **)&b->rec, sizeof(int4) * (size_t)std::max<int64_t>(k, 1), "batch");
     if (!s) s = dev_alloc((void **)&b->slot, sizeof(uint64_t) * (size_t)std::max<int64_t>(k, 1), "batch slots");
     if (!s) s = dev_alloc((void **)&b->pos, sizeof(uint64_t) * (size_t)std::max<int64_t>(k, 1), "batch positions");
-    if (!s)
-        s = dev_alloc((void **)&b->touched, sizeof(uint32_t) * (size_t)std::max<int64_t>(2 * ni, 1), "touched list");
     if (!s && k > 0 &&
         hipMemcpyAsync(b->rec, chg, sizeof(int4) * (size_t)k, hipMemcpyHostToDevice, g->stream) != hipSuccess)
         s = fail(DSSSP_ERR_HIP, "batch upload failed");
     if (!s && hipStreamSynchronize(g->stream) != hipSuccess) s = fail(DSSSP_ERR_HIP, "batch upload failed");
     if (s) {
+        const std::string keep = last_error();
         dsssp_batch_destroy(b);
-        return s;
+        return fail(s, "%s", keep.c_str());
     }
     *out = b;
     return DSSSP_OK;
@@ -1127,7 +1132,6 @@ extern "C" int dsssp_batch_destroy(dsssp_batch *b)
         dev_free(b->rec);
         dev_free(b->slot);
         dev_free(b->pos);
-        dev_free(b->touched);
         delete b;
     }
     graph_detach(g);
@@ -1157,7 +1161,7 @@ extern "C" int dsssp_update(dsssp_graph *g, dsssp_tree *t, const dsssp_change *c
     const double h2d = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     int s = b->k ? update_impl(g, t, b, stats, h2d) : DSSSP_OK;
     if (!b->k && stats) *stats = dsssp_stats{};
-    std::string keep = s ? last_error() : std::string();
+    const std::string keep = s ? last_error() : std::string();
     dsssp_batch_destroy(b);
     if (s) fail(s, "%s", keep.c_str());
     return s;

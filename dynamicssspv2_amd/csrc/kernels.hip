@@ -1,26 +1,32 @@
 // kernels.hip -- hand-written gfx950 kernels of the dynamic SSSP update.
 //
 // Work mapping (CDNA4, wave64):
-//  * Frontier kernels (k_frontier<MODE>) run a fixed grid of 256-thread
-//    workgroups; every wave takes work items in a grid-stride loop.  An item is
-//    either a "tile" of 64 light frontier vertices or one chunk of a heavy row.
-//    A tile is edge-balanced: each lane loads one vertex's row bounds, the wave
-//    forms an exclusive prefix sum of the degrees with DPP/shuffle steps, stages
-//    the per-vertex fields in LDS, and then lane l of the wave handles edge
-//    (base + l) of the tile's concatenated rows, finding its owner with a
-//    6-step binary search in LDS.  Adjacent lanes therefore read adjacent
-//    8-byte entries of the same row (coalesced) regardless of the degree mix.
-//  * Rows longer than `heavy` entries are not scanned inside a tile; they are
-//    split into `chunk`-entry pieces appended to the next round's heavy queue
-//    and processed one chunk per wave, so a hub never serialises a wave.
-//  * Vertices to visit next are compacted with one wave ballot + popcount and
-//    a single atomicAdd per wave; an epoch stamp per vertex removes duplicates.
+//  * A frontier is a bitmap over state slots.  Marking a slot is one
+//    non-returning 64-bit atomicOr at a random address, so there is no shared
+//    queue counter and duplicates vanish by construction.  A round's kernel
+//    consumes the input bitmap word by word (64 lanes load 64 words, a ballot
+//    finds the non-zero ones) and clears each word it reads, so the bitmap is
+//    clean again for use two rounds later.
+//  * One non-zero word is a tile of up to 64 consecutive slots.  Each lane loads
+//    one slot's row bounds, the wave forms an exclusive prefix sum of the
+//    degrees with shuffle steps, stages the per-slot fields in LDS, and lane l
+//    then handles edge base+l (and base+l+64*j for EPT edges in flight) of the
+//    tile's concatenated rows, finding its owner with a 6-step binary search in
+//    LDS.  Neighbouring lanes read neighbouring 8-byte entries (coalesced), and
+//    consecutive slots share cache lines of the row offsets and states.
+//  * Rows longer than `heavy` entries are not scanned inside a tile: they are
+//    split into `chunk`-entry pieces on a chunk list processed by the next
+//    launch, one chunk per wave, so an R-MAT hub never serialises a wave.
 //  * Distance and parent live in one uint64 (dist << 32 | parent) and change
 //    together through one 64-bit atomicMin, so the smaller distance wins and,
-//    among equal distances, the smaller parent id.
+//    among equal distances, the smaller parent id.  Every atomic in the edge
+//    loop is non-returning; the only returning atomics left are one per heavy
+//    row (chunk list) and the per-vertex insertion counters.
+//  * Statistics go to one slot per workgroup and are summed once per update;
+//    no kernel issues device-scope fences.
 // The reference does one thread per frontier vertex with unsynchronised
 // read-modify-writes (gpuFunctions_undir.cuh:370-433) and one full O(n) filter
-// pass per round (:125-174); none of that structure is reused here.
+// pass over 16-byte records per round (:125-174); none of that is reused here.
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -36,13 +42,6 @@ constexpr int RB = 256;      // block size of per-record / per-vertex kernels
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-__device__ __forceinline__ uint32_t load_relaxed(const uint32_t *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ unsigned long long lanemask_lt() { return (1ull << lane_id()) - 1ull; }
-
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 {
 #pragma unroll
@@ -50,126 +49,115 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
     return v;
 }
 
-// Appends v for every lane with want set: one atomicAdd per wave.
-__device__ __forceinline__ void wave_append(bool want, uint32_t v, uint32_t *q, uint32_t *cnt, uint32_t cap,
-                                            Counters *c)
+__device__ __forceinline__ void mark(uint64_t *bm, uint32_t s)
 {
-    const unsigned long long mask = __ballot(want);
-    if (mask == 0) return;
-    const uint32_t leader = (uint32_t)__builtin_ctzll(mask);
-    uint32_t base = 0;
-    if (lane_id() == leader) base = atomicAdd(cnt, (uint32_t)__popcll(mask));
-    base = __shfl(base, (int)leader);
-    if (want) {
-        const uint32_t pos = base + (uint32_t)__popcll(mask & lanemask_lt());
-        if (pos < cap)
-            q[pos] = v;
-        else
-            atomicOr(&c->err, ERR_QUEUE);
-    }
+    atomicOr((unsigned long long *)&bm[s >> 6], 1ull << (s & 63u));
+}
+
+__device__ __forceinline__ void add_stat(unsigned long long *bstats, int which, uint64_t v)
+{
+    if (v) atomicAdd(&bstats[(blockIdx.x % STAT_SLOTS) * NSTAT + which], (unsigned long long)v);
 }
 
 struct WaveStats {
-    uint32_t front;     // wave-uniform
-    uint64_t edges;     // wave-uniform
-    uint32_t improved;  // per lane
-    uint32_t enq;       // per lane
+    uint32_t front;  // wave-uniform
+    uint64_t edges;  // wave-uniform
+    uint32_t marks;  // per lane
+    bool any;        // per lane: this lane set an output bit
 };
 
-// One edge of the owner slot os (vertex os >> kshift, tree os & mask) with
-// adjacency entry ent.  Returns true when slot outv should be enqueued.
+// Each edge is handled in two steps so that a lane can keep several edges in
+// flight: gather() issues the loads of the neighbour's state, apply() decides
+// and writes.  The owner slot os is vertex os >> kshift of tree os & mask.
+struct Pre {
+    uint64_t sx;  // neighbour state
+    uint32_t lx;  // neighbour BFS level (TBFS / PICK)
+    bool skip;
+};
+
 template <int MODE>
-__device__ __forceinline__ bool edge_op(const FArgs &A, uint32_t os, uint32_t aux0, uint32_t aux1, uint64_t ent,
-                                        uint32_t &outv, WaveStats &S)
+__device__ __forceinline__ Pre gather(const FArgs &A, uint32_t os, uint64_t ent)
 {
+    Pre p{ST_INF, LVL_INF, true};
+    const uint32_t u = os >> A.kshift;
+    const uint32_t x = e_col(ent);
+    if (x == u) return p;  // a self-loop never carries a shortest path
+    if constexpr (MODE != M_INVAL) {
+        if (e_w(ent) & W_DEAD) return p;
+    }
+    const uint32_t xs = (x << A.kshift) | (os & ((1u << A.kshift) - 1u));
+    p.skip = false;
+    p.sx = A.st[xs];  // possibly stale within a round: only ever too large
+    if constexpr (MODE == M_TBFS) p.lx = A.lvl[xs];
+    if constexpr (MODE == M_PICK) {
+        if (A.use_lvl) p.lx = A.lvl[xs];
+    }
+    return p;
+}
+
+// Applies one edge; marks the slot to visit next.  A mark made on a state read
+// that was stale is harmless: that slot was changed in this round anyway.
+template <int MODE>
+__device__ __forceinline__ void apply(const FArgs &A, uint32_t os, uint32_t aux0, uint32_t aux1, uint64_t ent,
+                                      const Pre &p, WaveStats &S)
+{
+    if (p.skip) return;
     const uint32_t u = os >> A.kshift;
     const uint32_t x = e_col(ent);
     const uint32_t w = e_w(ent);
-    if (x == u) return false;  // a self-loop never carries a shortest path
-    if constexpr (MODE != M_INVAL) {
-        if (w & W_DEAD) return false;
-    }
     const uint32_t xs = (x << A.kshift) | (os & ((1u << A.kshift) - 1u));
     if constexpr (MODE == M_RELAX) {
         const uint64_t nd = (uint64_t)aux0 + w;
         if (nd >= DIST_INF) {
             atomicOr(&A.c->err, ERR_DIST_OVERFLOW);
-            return false;
+            return;
         }
         const uint64_t cand = pack_state((uint32_t)nd, u);
-        const uint64_t cur = A.st[xs];  // possibly stale, only ever too large
         // A root (parent itself, distance 0) never changes: with zero weights a
         // candidate (0, smaller id) would otherwise take its place.
-        if (cand >= cur || st_parent(cur) == x) return false;
-        const uint64_t old = atomicMin((unsigned long long *)&A.st[xs], (unsigned long long)cand);
-        if (cand >= old) return false;
-        S.improved++;
-        outv = xs;
-        return true;
+        if (cand >= p.sx || st_parent(p.sx) == x) return;
+        atomicMin((unsigned long long *)&A.st[xs], (unsigned long long)cand);
+        mark(A.out_bm, xs);
+        S.marks++;
+        S.any = true;
     } else if constexpr (MODE == M_INVAL) {
         // x is a child of u in the pre-update tree: reset it and descend.
         // Tombstoned entries are scanned too, so a child hanging off a deleted
-        // parallel copy is still found.
-        const uint64_t sx = A.st[xs];
-        if (sx == ST_INF || st_parent(sx) != u) return false;
-        if (atomicCAS((unsigned long long *)&A.st[xs], (unsigned long long)sx, (unsigned long long)ST_INF) != sx)
-            return false;
-        outv = xs;
-        return true;
+        // parallel copy is still found.  Only ST_INF is ever written in this
+        // phase, so a plain store is race-free.
+        if (p.sx == ST_INF || st_parent(p.sx) != u) return;
+        A.st[xs] = ST_INF;
+        mark(A.out_bm, xs);
+        mark(A.inv_bm, xs);
+        S.any = true;
     } else if constexpr (MODE == M_SEED) {
         // u was invalidated: take the best still-valid neighbour.
-        const uint64_t sx = A.st[xs];
-        if (sx == ST_INF) return false;
-        const uint64_t nd = (uint64_t)st_dist(sx) + w;
+        if (p.sx == ST_INF) return;
+        const uint64_t nd = (uint64_t)st_dist(p.sx) + w;
         if (nd >= DIST_INF) {
             atomicOr(&A.c->err, ERR_DIST_OVERFLOW);
-            return false;
+            return;
         }
         const uint64_t cand = pack_state((uint32_t)nd, x);
-        if (cand >= A.st[os]) return false;
-        const uint64_t old = atomicMin((unsigned long long *)&A.st[os], (unsigned long long)cand);
-        if (cand >= old) return false;
-        outv = os;
-        return true;
+        if (cand >= A.st[os]) return;
+        atomicMin((unsigned long long *)&A.st[os], (unsigned long long)cand);
+        mark(A.out_bm, os);
+        S.any = true;
     } else if constexpr (MODE == M_TBFS) {
-        const uint64_t sx = A.st[xs];
-        if (sx == ST_INF || st_parent(sx) == x) return false;
-        if ((uint64_t)aux0 + w != (uint64_t)st_dist(sx)) return false;
+        if (p.sx == ST_INF || st_parent(p.sx) == x) return;
+        if ((uint64_t)aux0 + w != (uint64_t)st_dist(p.sx)) return;
         const uint32_t nl = aux1 + 1;
-        if (nl >= A.lvl[xs]) return false;
-        if (nl >= atomicMin(&A.lvl[xs], nl)) return false;
-        outv = xs;
-        return true;
+        if (nl >= p.lx) return;
+        atomicMin(&A.lvl[xs], nl);
+        mark(A.out_bm, xs);
+        S.any = true;
     } else {  // M_PICK: os is the slot whose parent is being chosen
-        const uint64_t sx = A.st[xs];
-        if (sx == ST_INF) return false;
-        if ((uint64_t)st_dist(sx) + w != (uint64_t)aux0) return false;
-        if (A.use_lvl && A.lvl[xs] + 1 != aux1) return false;
+        if (p.sx == ST_INF) return;
+        if ((uint64_t)st_dist(p.sx) + w != (uint64_t)aux0) return;
+        if (A.use_lvl && p.lx + 1 != aux1) return;
         if (x < A.best[os]) atomicMin(&A.best[os], x);
-        return false;
     }
 }
-
-template <int MODE>
-__device__ __forceinline__ void finish_edge(const FArgs &A, bool want, uint32_t v, WaveStats &S)
-{
-    if constexpr (MODE == M_PICK) {
-        return;
-    } else {
-        if constexpr (MODE != M_INVAL) {
-            if (want) {
-                if (A.stamp[v] == A.epoch || atomicExch(&A.stamp[v], A.epoch) == A.epoch) want = false;
-            }
-        }
-        wave_append(want, v, A.out_q, A.out_cnt, A.out_cap, A.c);
-        if constexpr (MODE == M_INVAL) wave_append(want, v, A.invl, A.inv_total, A.out_cap, A.c);
-        S.enq += want ? 1u : 0u;
-    }
-}
-
-// Every mode splits long rows into chunks: loop modes process them in the next
-// round, single-pass modes (seed, parent pick) in a second, chunk-only launch.
-constexpr bool defers(int) { return true; }
 
 struct TileSmem {
     uint32_t off[FW][64];
@@ -204,17 +192,28 @@ __device__ __forceinline__ bool owner_prep(const FArgs &A, uint32_t os, uint32_t
     }
 }
 
-template <int MODE>
-__device__ void do_tile(const FArgs &A, uint32_t t, uint32_t tv_shift, uint32_t L, TileSmem &sm, uint32_t wv,
-                        WaveStats &S)
+// Queues the chunks of a row longer than A.heavy for the next launch.
+__device__ __forceinline__ void emit_chunks(const FArgs &A, uint32_t os, uint32_t deg)
+{
+    const uint32_t nch = (deg + A.chunk - 1) / A.chunk;
+    const uint32_t base = atomicAdd(A.out_hcnt, nch);
+    if (base + nch <= A.hcap) {
+        for (uint32_t c = 0; c < nch; ++c) A.out_hq[base + c] = ((uint64_t)os << 32) | c;
+    } else {
+        atomicOr(&A.c->err, ERR_QUEUE);
+    }
+}
+
+// One tile: the set bits of bitmap word `word`, slots base .. base+63.
+template <int MODE, int EPT>
+__device__ void do_tile(const FArgs &A, uint32_t base, uint64_t word, TileSmem &sm, uint32_t wv, WaveStats &S)
 {
     const uint32_t lane = lane_id();
-    const uint32_t idx = (t << tv_shift) + lane;
-    const bool valid = lane < (1u << tv_shift) && idx < L;
-    uint32_t os = 0, deg = 0, blen = 0, dbeg = 0, aux0 = 0, aux1 = 0;
+    const bool valid = (word >> lane) & 1ull;
+    const uint32_t os = base + lane;
+    uint32_t deg = 0, blen = 0, dbeg = 0, aux0 = 0, aux1 = 0;
     uint64_t b0 = 0;
     if (valid) {
-        os = A.in_q ? A.in_q[idx] : idx;
         const uint32_t u = os >> A.kshift;
         b0 = A.g.rp[u];
         blen = (uint32_t)(A.g.rp[u + 1] - b0);
@@ -222,20 +221,12 @@ __device__ void do_tile(const FArgs &A, uint32_t t, uint32_t tv_shift, uint32_t 
         dbeg = (uint32_t)(dd >> 32);
         deg = blen + (uint32_t)dd;
         if (!owner_prep<MODE>(A, os, aux0, aux1)) deg = 0;
-        if constexpr (defers(MODE)) {
-            if (deg > A.heavy) {
-                const uint32_t nch = (deg + A.chunk - 1) / A.chunk;
-                const uint32_t base = atomicAdd(A.out_hcnt, nch);
-                if (base + nch <= A.out_hcap) {
-                    for (uint32_t c = 0; c < nch; ++c) A.out_hq[base + c] = ((uint64_t)os << 32) | c;
-                } else {
-                    atomicOr(&A.c->err, ERR_QUEUE);
-                }
-                deg = 0;
-            }
+        if (deg > A.heavy) {
+            emit_chunks(A, os, deg);
+            deg = 0;
         }
     }
-    S.front += (uint32_t)__popcll(__ballot(valid));
+    S.front += (uint32_t)__popcll(word);
     uint32_t incl = deg;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -254,21 +245,34 @@ __device__ void do_tile(const FArgs &A, uint32_t t, uint32_t tv_shift, uint32_t 
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (uint32_t eb = 0; eb < total; eb += 64) {
-        const uint32_t e = eb + lane;
-        bool want = false;
-        uint32_t outv = 0;
-        if (e < total) {
-            uint32_t o = 0;
+    for (uint32_t eb = 0; eb < total; eb += 64 * EPT) {
+        uint64_t ent[EPT];
+        uint32_t ows[EPT], a0[EPT], a1[EPT];
+        bool act[EPT];
 #pragma unroll
-            for (uint32_t s = 32; s > 0; s >>= 1)
-                if (sm.off[wv][o + s] <= e) o += s;
-            const uint32_t j = e - sm.off[wv][o];
-            const uint32_t bl = sm.blen[wv][o];
-            const uint64_t ent = j < bl ? A.g.adj[sm.b0[wv][o] + j] : A.g.pool[(uint64_t)sm.dbeg[wv][o] + (j - bl)];
-            want = edge_op<MODE>(A, sm.u[wv][o], sm.aux0[wv][o], sm.aux1[wv][o], ent, outv, S);
+        for (int j = 0; j < EPT; ++j) {
+            const uint32_t e = eb + j * 64 + lane;
+            act[j] = e < total;
+            ent[j] = 0;
+            ows[j] = a0[j] = a1[j] = 0;
+            if (act[j]) {
+                uint32_t o = 0;
+#pragma unroll
+                for (uint32_t st = 32; st > 0; st >>= 1)
+                    if (sm.off[wv][o + st] <= e) o += st;
+                const uint32_t jj = e - sm.off[wv][o];
+                const uint32_t bl = sm.blen[wv][o];
+                ows[j] = sm.u[wv][o];
+                a0[j] = sm.aux0[wv][o];
+                a1[j] = sm.aux1[wv][o];
+                ent[j] = jj < bl ? A.g.adj[sm.b0[wv][o] + jj] : A.g.pool[(uint64_t)sm.dbeg[wv][o] + (jj - bl)];
+            }
         }
-        finish_edge<MODE>(A, want, outv, S);
+        Pre pre[EPT];
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) pre[j] = act[j] ? gather<MODE>(A, ows[j], ent[j]) : Pre{ST_INF, LVL_INF, true};
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) apply<MODE>(A, ows[j], a0[j], a1[j], ent[j], pre[j], S);
     }
     S.edges += total;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -276,7 +280,7 @@ __device__ void do_tile(const FArgs &A, uint32_t t, uint32_t tv_shift, uint32_t 
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int MODE>
+template <int MODE, int EPT>
 __device__ void do_chunk(const FArgs &A, uint32_t item, WaveStats &S)
 {
     const uint32_t lane = lane_id();
@@ -290,199 +294,227 @@ __device__ void do_chunk(const FArgs &A, uint32_t item, WaveStats &S)
     const uint32_t dbeg = (uint32_t)(dd >> 32);
     const uint32_t deg = blen + (uint32_t)dd;
     uint32_t aux0 = 0, aux1 = 0;
-    if (c == 0) S.front += 1;
     if (!owner_prep<MODE>(A, os, aux0, aux1)) return;
     const uint32_t start = c * A.chunk;
     const uint32_t end = min(start + A.chunk, deg);
-    for (uint32_t jb = start; jb < end; jb += 64) {
-        const uint32_t j = jb + lane;
-        bool want = false;
-        uint32_t outv = 0;
-        if (j < end) {
-            const uint64_t ent = j < blen ? A.g.adj[b0 + j] : A.g.pool[(uint64_t)dbeg + (j - blen)];
-            want = edge_op<MODE>(A, os, aux0, aux1, ent, outv, S);
+    for (uint32_t jb = start; jb < end; jb += 64 * EPT) {
+        uint64_t ent[EPT];
+        bool act[EPT];
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {
+            const uint32_t jj = jb + j * 64 + lane;
+            act[j] = jj < end;
+            ent[j] = act[j] ? (jj < blen ? A.g.adj[b0 + jj] : A.g.pool[(uint64_t)dbeg + (jj - blen)]) : 0ull;
         }
-        finish_edge<MODE>(A, want, outv, S);
+        Pre pre[EPT];
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) pre[j] = act[j] ? gather<MODE>(A, os, ent[j]) : Pre{ST_INF, LVL_INF, true};
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) apply<MODE>(A, os, aux0, aux1, ent[j], pre[j], S);
     }
     if (end > start) S.edges += end - start;
 }
 
-template <int MODE>
+template <int MODE, int EPT>
 __global__ __launch_bounds__(FB) void k_frontier(FArgs A)
 {
     __shared__ TileSmem sm;
-    if (A.dev_iter) {
-        Counters *c = A.c;
-        const uint32_t it = load_relaxed(&c->iter);
-        const uint32_t cs = (load_relaxed(&c->ring_base) + it) % 3u;
-        const uint32_t qs = (load_relaxed(&c->q_base) + it) & 1u;
-        A.epoch = load_relaxed(&c->epoch_base) + it;
-        A.in_q = A.qv[qs];
-        A.in_cnt = &c->ring[cs][0];
-        A.in_hq = A.hqv[qs];
-        A.in_hcnt = &c->ring[cs][1];
-        A.out_q = A.qv[qs ^ 1u];
-        A.out_cnt = &c->ring[(cs + 1u) % 3u][0];
-        A.out_hq = A.hqv[qs ^ 1u];
-        A.out_hcnt = &c->ring[(cs + 1u) % 3u][1];
-        A.zero_cnt = &c->ring[(cs + 2u) % 3u][0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (A.zero_act) *A.zero_act = 0;
+        if (A.zero_hcnt) *A.zero_hcnt = 0;
     }
-    if (A.zero_cnt && blockIdx.x == 0 && threadIdx.x == 0) {
-        A.zero_cnt[0] = 0;
-        A.zero_cnt[1] = 0;
-    }
-    // Input buffers have the same capacities as the output ones; a count above
-    // capacity (only after a reported overflow) is clamped so no read leaves them.
-    const uint32_t L = min(A.in_cnt ? load_relaxed(A.in_cnt) : A.in_n, A.out_cap);
-    const uint32_t H = min(A.in_hcnt ? load_relaxed(A.in_hcnt) : 0u, A.out_hcap);
+    const uint32_t lane = lane_id();
     const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t gw = blockIdx.x * FW + wv;
     const uint32_t nw = gridDim.x * FW;
-    // Vertices per tile: 64 when the frontier fills every wave several times,
-    // fewer (down to 1) for short frontiers, so that the rows of a short
-    // frontier are spread over all waves instead of walked by a few.
-    uint32_t tv_shift = 6;
-    while (tv_shift > 0 && (L >> tv_shift) < 2u * nw) --tv_shift;
-    const uint32_t T = (L + (1u << tv_shift) - 1u) >> tv_shift;
-    const uint32_t items = H + T;
-    WaveStats S{0, 0, 0, 0};
-    for (uint32_t it = blockIdx.x * FW + wv; it < items; it += nw) {
-        if (it < H)
-            do_chunk<MODE>(A, it, S);
-        else
-            do_tile<MODE>(A, it - H, tv_shift, L, sm, wv, S);
-    }
-    const uint32_t imp = wave_sum(S.improved);
-    const uint32_t enq = wave_sum(S.enq);
-    if (lane_id() == 0 && (S.edges | S.front | imp | enq)) {
-        if constexpr (MODE == M_RELAX) {
-            atomicAdd(&A.c->relax_frontier, (unsigned long long)S.front);
-            atomicAdd(&A.c->relax_edges, (unsigned long long)S.edges);
-            atomicAdd(&A.c->relax_improved, (unsigned long long)imp);
-            atomicAdd(&A.c->relax_enq, (unsigned long long)enq);
-        } else {
-            atomicAdd(&A.c->other_edges, (unsigned long long)S.edges);
-        }
-    }
-    if (A.dev_iter) {
-        // The last workgroup to finish advances the round index; the next
-        // launch on the stream starts after this one has completed.
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            __threadfence();
-            const uint32_t prev = atomicAdd(&A.c->blocks_done, 1u);
-            if (prev == gridDim.x - 1u) {
-                atomicExch(&A.c->blocks_done, 0u);
-                atomicAdd(&A.c->iter, 1u);
+    // Chunk count written by the previous launch; clamped to the list capacity.
+    const uint32_t H = A.in_hcnt ? min(*A.in_hcnt, A.hcap) : 0u;
+    WaveStats S{0, 0, 0, false};
+    for (uint32_t it = gw; it < H; it += nw) do_chunk<MODE, EPT>(A, it, S);
+    if (A.tiles) {
+        for (uint32_t wb = gw * 64; wb < A.nwords; wb += nw * 64) {
+            const uint32_t wi = wb + lane;
+            uint64_t word = 0;
+            if (wi < A.nwords) {
+                if (A.identity) {
+                    const uint32_t lo = wi * 64;
+                    word = A.nslots - lo >= 64 ? ~0ull : ((1ull << (A.nslots - lo)) - 1ull);
+                } else {
+                    word = A.in_bm[wi];
+                    if (word && A.consume) A.in_bm[wi] = 0;
+                }
+            }
+            unsigned long long nz = __ballot(word != 0);
+            while (nz) {
+                const uint32_t l = (uint32_t)__builtin_ctzll(nz);
+                nz &= nz - 1;
+                const uint64_t tw = __shfl(word, (int)l);
+                do_tile<MODE, EPT>(A, (wb + l) * 64, tw, sm, wv, S);
             }
         }
     }
-}
-
-// ---------------------------------------------------------------------------
-// Insertion of a batch into the resident CSR (delta rows in a pool).
-// ---------------------------------------------------------------------------
-
-// Counts new entries per endpoint; the first touch of a vertex lists it.
-__device__ __forceinline__ void wave_append_plain(bool want, uint32_t v, uint32_t *q, uint32_t *cnt)
-{
-    const unsigned long long mask = __ballot(want);
-    if (mask == 0) return;
-    const uint32_t leader = (uint32_t)__builtin_ctzll(mask);
-    uint32_t base = 0;
-    if (lane_id() == leader) base = atomicAdd(cnt, (uint32_t)__popcll(mask));
-    base = __shfl(base, (int)leader);
-    if (want) q[base + (uint32_t)__popcll(mask & lanemask_lt())] = v;
-}
-
-__global__ __launch_bounds__(RB) void k_ins_count(const int4 *rec, int64_t k, uint32_t *tcnt, uint64_t *slot,
-                                                  uint32_t *touched, GraphCtl *gc)
-{
-    const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
-    bool ta = false, tb = false;
-    int4 r = make_int4(0, 0, 0, -1);
-    if (i < k) {
-        r = rec[i];
-        if (r.w == 1) {
-            const uint32_t ra = atomicAdd(&tcnt[r.x], 1u);
-            const uint32_t rb = atomicAdd(&tcnt[r.y], 1u);
-            ta = ra == 0;
-            tb = rb == 0;
-            slot[i] = ((uint64_t)ra << 32) | rb;
+    const uint32_t marks = wave_sum(S.marks);
+    const bool any = __ballot(S.any) != 0;
+    if (lane == 0) {
+        if (any && A.out_act) *A.out_act = 1u;
+        if constexpr (MODE == M_RELAX) {
+            add_stat(A.bstats, ST_RELAX_FRONT, S.front);
+            add_stat(A.bstats, ST_RELAX_EDGES, S.edges);
+            add_stat(A.bstats, ST_RELAX_MARKS, marks);
+        } else {
+            add_stat(A.bstats, ST_OTHER_EDGES, S.edges);
+            if constexpr (MODE == M_SEED) add_stat(A.bstats, ST_INVALIDATED, S.front);
         }
     }
-    // the touched list holds each vertex once (first touch); at most 2k entries
-    wave_append_plain(ta, (uint32_t)r.x, touched, &gc->touched);
-    wave_append_plain(tb, (uint32_t)r.y, touched, &gc->touched);
 }
 
-// Moves each touched vertex's delta row to a fresh block with room for its new
-// entries; leaves the old length in tcnt as the write base.  Blocks are carved
-// from the pool with one atomicAdd per wave (prefix sum of the lanes' sizes).
-__global__ __launch_bounds__(RB) void k_ins_alloc(const uint32_t *touched, uint32_t max_touched, uint64_t *dbl,
-                                                  uint32_t *tcnt, uint64_t *pool, uint64_t pool_cap, GraphCtl *gc)
+// Sums the per-workgroup statistics into the counters and clears the slots.
+__global__ __launch_bounds__(256) void k_reduce_stats(Counters *c, unsigned long long *bstats)
 {
-    const uint32_t i = blockIdx.x * RB + threadIdx.x;
-    const uint32_t nt = min(load_relaxed(&gc->touched), max_touched);
-    const uint32_t lane = lane_id();
-    uint32_t v = 0, len = 0, add = 0;
-    uint64_t beg = 0;
-    if (i < nt) {
-        v = touched[i];
-        const uint64_t d = dbl[v];
-        beg = d >> 32;
-        len = (uint32_t)d;
-        add = tcnt[v];
-    }
-    uint64_t size = (uint64_t)len + add, incl = size;
+    __shared__ unsigned long long part[256][NSTAT];
+    unsigned long long acc[NSTAT] = {};
+    for (int i = threadIdx.x; i < STAT_SLOTS; i += 256) {
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint64_t y = __shfl_up(incl, d);
-        if ((int)lane >= d) incl += y;
+        for (int k = 0; k < NSTAT; ++k) {
+            acc[k] += bstats[i * NSTAT + k];
+            bstats[i * NSTAT + k] = 0;
+        }
     }
-    const uint64_t total = __shfl(incl, 63);
-    if (total == 0) return;  // wave-uniform
-    unsigned long long base = 0;
-    if (lane == 63) base = atomicAdd(&gc->pool_top, (unsigned long long)total);
-    base = __shfl(base, 63);
-    if (i >= nt) return;
-    const uint64_t nb = base + incl - size;
-    if (nb + size > pool_cap) {
-        atomicOr(&gc->err, ERR_POOL);
-        tcnt[v] = 0xFFFFFFFFu;
-        return;
+#pragma unroll
+    for (int k = 0; k < NSTAT; ++k) part[threadIdx.x][k] = acc[k];
+    __syncthreads();
+    if (threadIdx.x < NSTAT) {
+        unsigned long long s = 0;
+        for (int i = 0; i < 256; ++i) s += part[i][threadIdx.x];
+        c->stats[threadIdx.x] += s;
     }
-    for (uint32_t j = 0; j < len; ++j) pool[nb + j] = pool[beg + j];
-    dbl[v] = (nb << 32) | size;
-    tcnt[v] = len;
 }
 
-__global__ __launch_bounds__(RB) void k_ins_scatter(const int4 *rec, int64_t k, const uint64_t *slot,
-                                                    const uint64_t *dbl, const uint32_t *tcnt, uint64_t *pool,
-                                                    uint64_t *pos)
+// ---------------------------------------------------------------------------
+// Insertion of a batch into the resident CSR (delta rows in a pool).  Pool
+// space is carved deterministically: per-word sizes, one scan, per-word bases.
+// ---------------------------------------------------------------------------
+
+// Counts new entries per endpoint; the first touch of a vertex marks it.
+__global__ __launch_bounds__(RB) void k_ins_count(const int4 *rec, int64_t k, uint32_t *tcnt, uint64_t *slot,
+                                                  uint64_t *tbm)
 {
     const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
     if (i >= k) return;
     const int4 r = rec[i];
     if (r.w != 1) return;
-    const uint64_t s = slot[i];
-    const uint32_t ba = tcnt[r.x], bb = tcnt[r.y];
-    if (ba == 0xFFFFFFFFu || bb == 0xFFFFFFFFu) {  // pool overflow reported by k_ins_alloc
+    const uint32_t ra = atomicAdd(&tcnt[r.x], 1u);
+    const uint32_t rb = atomicAdd(&tcnt[r.y], 1u);
+    if (ra == 0) mark(tbm, (uint32_t)r.x);
+    if (rb == 0) mark(tbm, (uint32_t)r.y);
+    slot[i] = ((uint64_t)ra << 32) | rb;
+}
+
+// wsum[w] = entries the touched vertices of bitmap word w need (old + new).
+__global__ __launch_bounds__(RB) void k_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const uint64_t *dbl,
+                                                    const uint32_t *tcnt, uint32_t *wsum)
+{
+    const uint32_t w = blockIdx.x * RB + threadIdx.x;
+    if (w >= nwords) return;
+    uint64_t word = tbm[w];
+    uint32_t sum = 0;
+    while (word) {
+        const uint32_t v = w * 64 + (uint32_t)__builtin_ctzll(word);
+        word &= word - 1;
+        sum += (uint32_t)dbl[v] + tcnt[v];
+    }
+    wsum[w] = sum;
+}
+
+// Single-workgroup exclusive scan of wsum into pool offsets wbase (from the
+// current pool top); advances the pool top.
+__global__ __launch_bounds__(1024) void k_ins_scan(const uint32_t *wsum, uint32_t nwords, uint64_t *wbase,
+                                                   uint64_t pool_cap, GraphCtl *gc)
+{
+    __shared__ unsigned long long part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nwords + 1023) / 1024;
+    const uint32_t lo = min(t * per, nwords), hi = min(lo + per, nwords);
+    unsigned long long s = 0;
+    for (uint32_t i = lo; i < hi; ++i) s += wsum[i];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {
+        const unsigned long long y = t >= d ? part[t - d] : 0ull;
+        __syncthreads();
+        part[t] += y;
+        __syncthreads();
+    }
+    const unsigned long long top = gc->pool_top;
+    unsigned long long run = top + part[t] - s;
+    for (uint32_t i = lo; i < hi; ++i) {
+        wbase[i] = run;
+        run += wsum[i];
+    }
+    __syncthreads();
+    if (t == 1023) {
+        const unsigned long long total = part[1023];
+        gc->batch_new = total;
+        if (top + total > pool_cap)
+            gc->err |= ERR_POOL;
+        else
+            gc->pool_top = top + total;
+    }
+}
+
+// Moves each touched vertex's delta row to its new block (old entries first);
+// leaves the old length in tcnt as the write base of the new entries.
+__global__ __launch_bounds__(RB) void k_ins_alloc(const uint64_t *tbm, uint32_t nwords, const uint64_t *wbase,
+                                                  uint64_t *dbl, uint32_t *tcnt, uint64_t *pool, const GraphCtl *gc)
+{
+    const uint32_t w = blockIdx.x * RB + threadIdx.x;
+    if (w >= nwords || (gc->err & ERR_POOL)) return;
+    uint64_t word = tbm[w];
+    uint64_t nb = wbase[w];
+    while (word) {
+        const uint32_t v = w * 64 + (uint32_t)__builtin_ctzll(word);
+        word &= word - 1;
+        const uint64_t d = dbl[v];
+        const uint64_t beg = d >> 32;
+        const uint32_t len = (uint32_t)d, add = tcnt[v];
+        for (uint32_t j = 0; j < len; ++j) pool[nb + j] = pool[beg + j];
+        dbl[v] = (nb << 32) | (uint64_t)(len + add);
+        tcnt[v] = len;
+        nb += len + add;
+    }
+}
+
+__global__ __launch_bounds__(RB) void k_ins_scatter(const int4 *rec, int64_t k, const uint64_t *slot,
+                                                    const uint64_t *dbl, const uint32_t *tcnt, uint64_t *pool,
+                                                    uint64_t *pos, const GraphCtl *gc)
+{
+    const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
+    if (i >= k) return;
+    const int4 r = rec[i];
+    if (r.w != 1) return;
+    if (gc->err & ERR_POOL) {  // reported to the host; nothing was placed
         pos[i] = ~0ull;
         return;
     }
-    const uint64_t pa = (dbl[r.x] >> 32) + ba + (s >> 32);
-    const uint64_t pb = (dbl[r.y] >> 32) + bb + (uint32_t)s;
+    const uint64_t s = slot[i];
+    const uint64_t pa = (dbl[r.x] >> 32) + tcnt[r.x] + (s >> 32);
+    const uint64_t pb = (dbl[r.y] >> 32) + tcnt[r.y] + (uint32_t)s;
     pool[pa] = pack_entry((uint32_t)r.y, (uint32_t)r.z);
     pool[pb] = pack_entry((uint32_t)r.x, (uint32_t)r.z);
     pos[i] = (pa << 32) | pb;
 }
 
-__global__ __launch_bounds__(RB) void k_ins_reset(const uint32_t *touched, uint32_t max_touched, uint32_t *tcnt,
-                                                  GraphCtl *gc)
+__global__ __launch_bounds__(RB) void k_ins_reset(uint64_t *tbm, uint32_t nwords, uint32_t *tcnt)
 {
-    const uint32_t i = blockIdx.x * RB + threadIdx.x;
-    const uint32_t nt = min(load_relaxed(&gc->touched), max_touched);
-    if (i < nt) tcnt[touched[i]] = 0;
+    const uint32_t w = blockIdx.x * RB + threadIdx.x;
+    if (w >= nwords) return;
+    uint64_t word = tbm[w];
+    if (!word) return;
+    tbm[w] = 0;
+    while (word) {
+        tcnt[w * 64 + (uint32_t)__builtin_ctzll(word)] = 0;
+        word &= word - 1;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -522,14 +554,14 @@ __device__ __forceinline__ uint32_t tomb_row(GView g, uint32_t r, uint32_t c, ui
 
 // Thread i handles record i >> kshift for tree i & mask; tree 0 also tombstones.
 __global__ __launch_bounds__(RB) void k_classify(const int4 *rec, int64_t k, uint32_t kshift, GView g, uint64_t *st,
-                                                 uint32_t *out_q, uint32_t *out_cnt, uint32_t *invl,
-                                                 uint32_t *inv_total, uint32_t cap, Counters *c)
+                                                 uint64_t *out_bm, uint64_t *inv_bm, uint32_t *out_act,
+                                                 unsigned long long *bstats)
 {
     const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
     const int64_t ri = i >> kshift;
     const uint32_t tree = (uint32_t)i & ((1u << kshift) - 1u);
-    bool want = false;
-    uint32_t victim = 0, probes = 0;
+    bool hit = false;
+    uint32_t probes = 0;
     if (ri < k) {
         const int4 r = rec[ri];
         if (r.w == 0) {
@@ -539,17 +571,19 @@ __global__ __launch_bounds__(RB) void k_classify(const int4 *rec, int64_t k, uin
                 // child endpoint of a tree edge roots an invalidated subtree.
                 const uint32_t sa = (a << kshift) | tree, sb = (b << kshift) | tree;
                 const uint64_t vb = st[sb];
+                uint32_t victim = 0xFFFFFFFFu;
                 if (vb != ST_INF && st_parent(vb) == a) {
                     victim = sb;
-                    want = true;
                 } else {
                     const uint64_t va = st[sa];
-                    if (va != ST_INF && st_parent(va) == b) {
-                        victim = sa;
-                        want = true;
-                    }
+                    if (va != ST_INF && st_parent(va) == b) victim = sa;
                 }
-                if (want) want = atomicExch((unsigned long long *)&st[victim], (unsigned long long)ST_INF) != ST_INF;
+                if (victim != 0xFFFFFFFFu) {
+                    st[victim] = ST_INF;  // only ST_INF is written by this kernel
+                    mark(out_bm, victim);
+                    mark(inv_bm, victim);
+                    hit = true;
+                }
             }
             if (tree == 0) {
                 probes += tomb_row(g, a, b, (uint32_t)r.z);
@@ -557,30 +591,29 @@ __global__ __launch_bounds__(RB) void k_classify(const int4 *rec, int64_t k, uin
             }
         }
     }
-    wave_append(want, victim, out_q, out_cnt, cap, c);
-    wave_append(want, victim, invl, inv_total, cap, c);
+    const bool any = __ballot(hit) != 0;
     const uint32_t p = wave_sum(probes);
-    if (lane_id() == 0 && p) atomicAdd(&c->probes, (unsigned long long)p);
+    if (lane_id() == 0) {
+        if (any) *out_act = 1u;
+        add_stat(bstats, ST_PROBES, p);
+    }
 }
 
 // Offers every inserted edge that survived the batch's deletions in both
 // directions (the role of insertEdge, gpuFunctions_undir.cuh:217-251).
 __global__ __launch_bounds__(RB) void k_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint64_t *pos,
-                                                  const uint64_t *pool, uint64_t *st, uint32_t *stamp,
-                                                  uint32_t epoch, uint32_t *out_q, uint32_t *out_cnt,
-                                                  uint32_t cap, Counters *c)
+                                                  const uint64_t *pool, uint64_t *st, uint64_t *out_bm,
+                                                  uint32_t *out_act, Counters *c)
 {
     const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
     const int64_t ri = i >> kshift;
     const uint32_t tree = (uint32_t)i & ((1u << kshift) - 1u);
-    bool wa = false, wb = false;
-    uint32_t sa_slot = 0, sb_slot = 0;
+    bool hit = false;
     if (ri < k) {
         const int4 r = rec[ri];
         if (r.w == 1 && r.x != r.y && pos[ri] != ~0ull && !(pool[pos[ri] >> 32] & E_DEAD)) {
             const uint32_t a = (uint32_t)r.x, b = (uint32_t)r.y;
-            sa_slot = (a << kshift) | tree;
-            sb_slot = (b << kshift) | tree;
+            const uint32_t sa_slot = (a << kshift) | tree, sb_slot = (b << kshift) | tree;
             const uint32_t w = (uint32_t)r.z;
             const uint64_t sa = st[sa_slot], sb = st[sb_slot];
             const bool a_root = sa != ST_INF && st_parent(sa) == a;  // roots never change
@@ -591,8 +624,11 @@ __global__ __launch_bounds__(RB) void k_ins_offer(const int4 *rec, int64_t k, ui
                     atomicOr(&c->err, ERR_DIST_OVERFLOW);
                 } else {
                     const uint64_t cand = pack_state((uint32_t)nd, a);
-                    if (cand < sb && cand < atomicMin((unsigned long long *)&st[sb_slot], (unsigned long long)cand))
-                        wb = true;
+                    if (cand < sb) {
+                        atomicMin((unsigned long long *)&st[sb_slot], (unsigned long long)cand);
+                        mark(out_bm, sb_slot);
+                        hit = true;
+                    }
                 }
             }
             if (sb != ST_INF && !a_root) {
@@ -601,53 +637,33 @@ __global__ __launch_bounds__(RB) void k_ins_offer(const int4 *rec, int64_t k, ui
                     atomicOr(&c->err, ERR_DIST_OVERFLOW);
                 } else {
                     const uint64_t cand = pack_state((uint32_t)nd, b);
-                    if (cand < sa && cand < atomicMin((unsigned long long *)&st[sa_slot], (unsigned long long)cand))
-                        wa = true;
+                    if (cand < sa) {
+                        atomicMin((unsigned long long *)&st[sa_slot], (unsigned long long)cand);
+                        mark(out_bm, sa_slot);
+                        hit = true;
+                    }
                 }
             }
-            if (wa && atomicExch(&stamp[sa_slot], epoch) == epoch) wa = false;
-            if (wb && atomicExch(&stamp[sb_slot], epoch) == epoch) wb = false;
         }
     }
-    wave_append(wa, sa_slot, out_q, out_cnt, cap, c);
-    wave_append(wb, sb_slot, out_q, out_cnt, cap, c);
+    if (__ballot(hit) != 0 && lane_id() == 0) *out_act = 1u;
 }
 
 // ---------------------------------------------------------------------------
 // Small helpers.
 // ---------------------------------------------------------------------------
 
-__global__ void k_reset_counters(Counters *c)
-{
-    if (threadIdx.x == 0) {
-        Counters z{};
-        *c = z;
-    }
-}
-
-__global__ void k_reset_graphctl(GraphCtl *gc)
-{
-    if (threadIdx.x == 0) {
-        gc->touched = 0;
-        gc->err = 0;
-    }
-}
-
 __global__ __launch_bounds__(RB) void k_set_sources(uint64_t *st, const uint32_t *src, uint32_t count,
-                                                    uint32_t kshift, uint32_t per_tree, uint32_t *stamp,
-                                                    uint32_t epoch, uint32_t *q, uint32_t *cnt, uint32_t cap,
-                                                    Counters *c)
+                                                    uint32_t kshift, uint32_t per_tree, uint64_t *out_bm,
+                                                    uint32_t *out_act)
 {
     const uint32_t j = blockIdx.x * RB + threadIdx.x;
-    bool want = false;
-    uint32_t slot = 0;
-    if (j < count) {
-        const uint32_t v = src[j];
-        slot = (v << kshift) | (per_tree ? j : 0u);
-        st[slot] = pack_state(0u, v);
-        want = atomicExch(&stamp[slot], epoch) != epoch;
-    }
-    wave_append(want, slot, q, cnt, cap, c);
+    if (j >= count) return;
+    const uint32_t v = src[j];
+    const uint32_t slot = (v << kshift) | (per_tree ? j : 0u);
+    st[slot] = pack_state(0u, v);
+    mark(out_bm, slot);
+    *out_act = 1u;
 }
 
 __global__ __launch_bounds__(RB) void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n)
@@ -662,8 +678,7 @@ __global__ __launch_bounds__(RB) void k_fill_u32(uint32_t *p, uint32_t v, uint64
 
 // Roots (parent == self) start the tight-graph BFS at level 0.
 __global__ __launch_bounds__(RB) void k_lvl_init(uint32_t nslots, uint32_t kshift, const uint64_t *st, uint32_t *lvl,
-                                                 uint32_t *best, uint32_t *stamp, uint32_t epoch, uint32_t *out_q,
-                                                 uint32_t *out_cnt, uint32_t cap, Counters *c)
+                                                 uint32_t *best, uint64_t *out_bm, uint32_t *out_act)
 {
     const uint32_t s = blockIdx.x * RB + threadIdx.x;
     bool root = false;
@@ -672,9 +687,9 @@ __global__ __launch_bounds__(RB) void k_lvl_init(uint32_t nslots, uint32_t kshif
         root = v != ST_INF && st_parent(v) == (s >> kshift);
         lvl[s] = root ? 0u : LVL_INF;
         best[s] = 0xFFFFFFFFu;
-        if (root) stamp[s] = epoch;
+        if (root) mark(out_bm, s);
     }
-    wave_append(root, s, out_q, out_cnt, cap, c);
+    if (__ballot(root) != 0 && lane_id() == 0) *out_act = 1u;
 }
 
 __global__ __launch_bounds__(RB) void k_apply_best(uint32_t nslots, uint32_t kshift, uint64_t *st, const uint32_t *best,
@@ -692,18 +707,19 @@ __global__ __launch_bounds__(RB) void k_apply_best(uint32_t nslots, uint32_t ksh
     st[s] = pack_state(st_dist(v), b);
 }
 
-__global__ void k_phase_init(Counters *c, uint32_t ring_base, uint32_t q_base, uint32_t epoch_base)
+inline unsigned blocks_for(int64_t n) { return (unsigned)((n + RB - 1) / RB); }
+
+template <int EPT>
+void launch_frontier_ept(int mode, const FArgs &a, int grid, hipStream_t s)
 {
-    if (threadIdx.x == 0) {
-        c->iter = 0;
-        c->blocks_done = 0;
-        c->ring_base = ring_base;
-        c->q_base = q_base;
-        c->epoch_base = epoch_base;
+    switch (mode) {
+    case M_RELAX: hipLaunchKernelGGL((k_frontier<M_RELAX, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
+    case M_INVAL: hipLaunchKernelGGL((k_frontier<M_INVAL, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
+    case M_SEED: hipLaunchKernelGGL((k_frontier<M_SEED, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
+    case M_TBFS: hipLaunchKernelGGL((k_frontier<M_TBFS, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_frontier<M_PICK, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
     }
 }
-
-inline unsigned blocks_for(int64_t n) { return (unsigned)((n + RB - 1) / RB); }
 
 }  // namespace
 
@@ -713,69 +729,11 @@ inline unsigned blocks_for(int64_t n) { return (unsigned)((n + RB - 1) / RB); }
 
 void launch_frontier(int mode, const FArgs &a, int grid, hipStream_t s)
 {
-    switch (mode) {
-    case M_RELAX: hipLaunchKernelGGL(k_frontier<M_RELAX>, dim3(grid), dim3(FB), 0, s, a); break;
-    case M_INVAL: hipLaunchKernelGGL(k_frontier<M_INVAL>, dim3(grid), dim3(FB), 0, s, a); break;
-    case M_SEED: hipLaunchKernelGGL(k_frontier<M_SEED>, dim3(grid), dim3(FB), 0, s, a); break;
-    case M_TBFS: hipLaunchKernelGGL(k_frontier<M_TBFS>, dim3(grid), dim3(FB), 0, s, a); break;
-    default: hipLaunchKernelGGL(k_frontier<M_PICK>, dim3(grid), dim3(FB), 0, s, a); break;
+    switch (a.ept) {
+    case 1: launch_frontier_ept<1>(mode, a, grid, s); break;
+    case 2: launch_frontier_ept<2>(mode, a, grid, s); break;
+    default: launch_frontier_ept<4>(mode, a, grid, s); break;
     }
-}
-
-void launch_reset_counters(Counters *c, hipStream_t s) { hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, c); }
-
-void launch_reset_graphctl(GraphCtl *gc, hipStream_t s) { hipLaunchKernelGGL(k_reset_graphctl, dim3(1), dim3(64), 0, s, gc); }
-
-void launch_ins_count(const int4 *rec, int64_t k, uint32_t *tcnt, uint64_t *slot, uint32_t *touched, GraphCtl *gc,
-                      hipStream_t s)
-{
-    if (k > 0) hipLaunchKernelGGL(k_ins_count, dim3(blocks_for(k)), dim3(RB), 0, s, rec, k, tcnt, slot, touched, gc);
-}
-
-void launch_ins_alloc(const uint32_t *touched, uint32_t max_touched, uint64_t *dbl, uint32_t *tcnt, uint64_t *pool,
-                      uint64_t pool_cap, GraphCtl *gc, hipStream_t s)
-{
-    if (max_touched > 0)
-        hipLaunchKernelGGL(k_ins_alloc, dim3(blocks_for(max_touched)), dim3(RB), 0, s, touched, max_touched, dbl, tcnt,
-                           pool, pool_cap, gc);
-}
-
-void launch_ins_scatter(const int4 *rec, int64_t k, const uint64_t *slot, const uint64_t *dbl, const uint32_t *tcnt,
-                        uint64_t *pool, uint64_t *pos, hipStream_t s)
-{
-    if (k > 0) hipLaunchKernelGGL(k_ins_scatter, dim3(blocks_for(k)), dim3(RB), 0, s, rec, k, slot, dbl, tcnt, pool, pos);
-}
-
-void launch_ins_reset(const uint32_t *touched, uint32_t max_touched, uint32_t *tcnt, GraphCtl *gc, hipStream_t s)
-{
-    if (max_touched > 0)
-        hipLaunchKernelGGL(k_ins_reset, dim3(blocks_for(max_touched)), dim3(RB), 0, s, touched, max_touched, tcnt, gc);
-}
-
-void launch_classify(const int4 *rec, int64_t k, uint32_t kshift, GView g, uint64_t *st, uint32_t *out_q,
-                     uint32_t *out_cnt, uint32_t *invl, uint32_t *inv_total, uint32_t cap, Counters *c, hipStream_t s)
-{
-    if (k > 0)
-        hipLaunchKernelGGL(k_classify, dim3(blocks_for(k << kshift)), dim3(RB), 0, s, rec, k, kshift, g, st, out_q,
-                           out_cnt, invl, inv_total, cap, c);
-}
-
-void launch_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint64_t *pos, const uint64_t *pool,
-                      uint64_t *st, uint32_t *stamp, uint32_t epoch, uint32_t *out_q, uint32_t *out_cnt, uint32_t cap,
-                      Counters *c, hipStream_t s)
-{
-    if (k > 0)
-        hipLaunchKernelGGL(k_ins_offer, dim3(blocks_for(k << kshift)), dim3(RB), 0, s, rec, k, kshift, pos, pool, st,
-                           stamp, epoch, out_q, out_cnt, cap, c);
-}
-
-void launch_set_sources(uint64_t *st, const uint32_t *src, uint32_t count, uint32_t kshift, bool per_tree,
-                        uint32_t *stamp, uint32_t epoch, uint32_t *q, uint32_t *cnt, uint32_t cap, Counters *c,
-                        hipStream_t s)
-{
-    if (count > 0)
-        hipLaunchKernelGGL(k_set_sources, dim3(blocks_for(count)), dim3(RB), 0, s, st, src, count, kshift,
-                           per_tree ? 1u : 0u, stamp, epoch, q, cnt, cap, c);
 }
 
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s)
@@ -792,24 +750,86 @@ void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s)
     hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(RB), 0, s, p, v, n);
 }
 
-void launch_lvl_init(uint32_t nslots, uint32_t kshift, const uint64_t *st, uint32_t *lvl, uint32_t *best,
-                     uint32_t *stamp, uint32_t epoch, uint32_t *out_q, uint32_t *out_cnt, uint32_t cap, Counters *c,
-                     hipStream_t s)
+void launch_reduce_stats(Counters *c, unsigned long long *bstats, hipStream_t s)
 {
-    if (nslots > 0)
-        hipLaunchKernelGGL(k_lvl_init, dim3(blocks_for(nslots)), dim3(RB), 0, s, nslots, kshift, st, lvl, best, stamp,
-                           epoch, out_q, out_cnt, cap, c);
+    hipLaunchKernelGGL(k_reduce_stats, dim3(1), dim3(256), 0, s, c, bstats);
 }
 
-void launch_phase_init(Counters *c, uint32_t ring_base, uint32_t q_base, uint32_t epoch_base, hipStream_t s)
+void launch_ins_count(const int4 *rec, int64_t k, uint32_t *tcnt, uint64_t *slot, uint64_t *tbm, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_phase_init, dim3(1), dim3(64), 0, s, c, ring_base, q_base, epoch_base);
+    if (k > 0) hipLaunchKernelGGL(k_ins_count, dim3(blocks_for(k)), dim3(RB), 0, s, rec, k, tcnt, slot, tbm);
+}
+
+void launch_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const uint64_t *dbl, const uint32_t *tcnt,
+                        uint32_t *wsum, hipStream_t s)
+{
+    if (nwords > 0)
+        hipLaunchKernelGGL(k_ins_wordsum, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, dbl, tcnt, wsum);
+}
+
+void launch_ins_scan(const uint32_t *wsum, uint32_t nwords, uint64_t *wbase, uint64_t pool_cap, GraphCtl *gc,
+                     hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ins_scan, dim3(1), dim3(1024), 0, s, wsum, nwords, wbase, pool_cap, gc);
+}
+
+void launch_ins_alloc(const uint64_t *tbm, uint32_t nwords, const uint64_t *wbase, uint64_t *dbl, uint32_t *tcnt,
+                      uint64_t *pool, const GraphCtl *gc, hipStream_t s)
+{
+    if (nwords > 0)
+        hipLaunchKernelGGL(k_ins_alloc, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, wbase, dbl, tcnt, pool,
+                           gc);
+}
+
+void launch_ins_scatter(const int4 *rec, int64_t k, const uint64_t *slot, const uint64_t *dbl, const uint32_t *tcnt,
+                        uint64_t *pool, uint64_t *pos, const GraphCtl *gc, hipStream_t s)
+{
+    if (k > 0)
+        hipLaunchKernelGGL(k_ins_scatter, dim3(blocks_for(k)), dim3(RB), 0, s, rec, k, slot, dbl, tcnt, pool, pos, gc);
+}
+
+void launch_ins_reset(uint64_t *tbm, uint32_t nwords, uint32_t *tcnt, hipStream_t s)
+{
+    if (nwords > 0) hipLaunchKernelGGL(k_ins_reset, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, tcnt);
+}
+
+void launch_classify(const int4 *rec, int64_t k, uint32_t kshift, GView g, uint64_t *st, uint64_t *out_bm,
+                     uint64_t *inv_bm, uint32_t *out_act, unsigned long long *bstats, hipStream_t s)
+{
+    if (k > 0)
+        hipLaunchKernelGGL(k_classify, dim3(blocks_for(k << kshift)), dim3(RB), 0, s, rec, k, kshift, g, st, out_bm,
+                           inv_bm, out_act, bstats);
+}
+
+void launch_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint64_t *pos, const uint64_t *pool,
+                      uint64_t *st, uint64_t *out_bm, uint32_t *out_act, Counters *c, hipStream_t s)
+{
+    if (k > 0)
+        hipLaunchKernelGGL(k_ins_offer, dim3(blocks_for(k << kshift)), dim3(RB), 0, s, rec, k, kshift, pos, pool, st,
+                           out_bm, out_act, c);
+}
+
+void launch_set_sources(uint64_t *st, const uint32_t *src, uint32_t count, uint32_t kshift, bool per_tree,
+                        uint64_t *out_bm, uint32_t *out_act, hipStream_t s)
+{
+    if (count > 0)
+        hipLaunchKernelGGL(k_set_sources, dim3(blocks_for(count)), dim3(RB), 0, s, st, src, count, kshift,
+                           per_tree ? 1u : 0u, out_bm, out_act);
+}
+
+void launch_lvl_init(uint32_t nslots, uint32_t kshift, const uint64_t *st, uint32_t *lvl, uint32_t *best,
+                     uint64_t *out_bm, uint32_t *out_act, hipStream_t s)
+{
+    if (nslots > 0)
+        hipLaunchKernelGGL(k_lvl_init, dim3(blocks_for(nslots)), dim3(RB), 0, s, nslots, kshift, st, lvl, best, out_bm,
+                           out_act);
 }
 
 void launch_apply_best(uint32_t nslots, uint32_t kshift, uint64_t *st, const uint32_t *best, Counters *c,
                        hipStream_t s)
 {
-    if (nslots > 0) hipLaunchKernelGGL(k_apply_best, dim3(blocks_for(nslots)), dim3(RB), 0, s, nslots, kshift, st, best, c);
+    if (nslots > 0)
+        hipLaunchKernelGGL(k_apply_best, dim3(blocks_for(nslots)), dim3(RB), 0, s, nslots, kshift, st, best, c);
 }
 
 }  // namespace dsssp

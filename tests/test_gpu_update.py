@@ -230,10 +230,28 @@ def test_overflow_is_reported():
     assert e.value.code == ds.ERR_OVERFLOW
 
 
-def test_heavy_rows_and_chunks():
-    """Tiny heavy threshold so that most rows go through the chunk path."""
+def test_update_after_overflow_error():
+    """An update that overflows reports it and leaves the device scratch clean:
+    deleting the offending edge afterwards gives the exact tree again."""
+    n = 4
+    g = ds.Graph(n, [0], [1], [2_000_000_000])
+    t = ds.Tree(g, 0)
+    ds.update(g, t, np.array([[1, 2, 2_000_000_000, 1]], dtype=np.int32))
+    with pytest.raises(ds.DssspError) as e:
+        ds.update(g, t, np.array([[2, 3, 2_000_000_000, 1]], dtype=np.int32))
+    assert e.value.code == ds.ERR_OVERFLOW
+    ds.update(g, t, np.array([[2, 3, 2_000_000_000, 0]], dtype=np.int32))
+    p, d = t.read()
+    assert list(p) == [0, 0, 1, -1]
+    assert list(d[:3]) == [0, 2_000_000_000, 4_000_000_000] and d[3] == ds.INF
+
+
+@pytest.mark.parametrize("ept", [1, 2, 4])
+def test_heavy_rows_and_chunks(ept):
+    """Tiny heavy threshold so that most rows go through the chunk path; every
+    edges-in-flight variant of the frontier kernel."""
     old = ds.get_tuning()
-    ds.set_tuning(heavy_degree=64, chunk=64, grid=64)
+    ds.set_tuning(heavy_degree=64, chunk=64, grid=64, edges_per_thread=ept)
     try:
         a, b, w = ds.gen_rmat(13, 16, wlo=1, whi=1000, seed=77)
         n = 1 << 13
@@ -245,7 +263,7 @@ def test_heavy_rows_and_chunks():
             batches.append(chg)
             ca, cb, cw = oracle.apply_changes(n, ca, cb, cw, chg)
         deg = np.bincount(np.concatenate([a, b]), minlength=n)
-        run_sequence(n, a, b, w, batches, src=int(np.argmax(deg)), what="rmat13-heavy")
+        run_sequence(n, a, b, w, batches, src=int(np.argmax(deg)), what=f"rmat13-heavy-ept{ept}")
     finally:
         ds.set_tuning(**old)
 
