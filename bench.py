@@ -60,6 +60,8 @@ def parse():
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU Dijkstra baseline / exactness check")
     p.add_argument("--no-timing", action="store_true", help="no per-launch events (roofline unmeasured)")
+    for knob in ("grid", "heavy_degree", "chunk", "round_batch", "use_graph", "edges_per_thread"):
+        p.add_argument("--" + knob.replace("_", "-"), type=int, default=None, help="tuning knob (dsssp_tuning)")
     return p.parse_args()
 
 
@@ -133,6 +135,10 @@ def line(args, value, wall, steps, cfg, stats, check, cpu, unit="changed edges/s
         "dist_bitexact": bool(check.get("gpu_recompute_equal", True)
                               and check.get("dijkstra_mismatches", 0) == 0 and check.get("certificate", 0) == 0),
         "check": check, "roofline": roofline(stats), "cpu_baseline": cpu,
+        "phases_ms": {ph: sum(s[f"t_{ph}_ms"] for s in stats) / max(1, len(stats))
+                      for ph in ("insert", "classify", "invalidate", "seed", "relax")},
+        "rounds": {"invalidate": sum(s["inv_rounds"] for s in stats) / max(1, len(stats)),
+                   "relax": sum(s["relax_rounds"] for s in stats) / max(1, len(stats))},
     }
     if extra:
         out.update(extra)
@@ -412,6 +418,11 @@ def main():
 
     import dynamicssspv2_amd as ds
     ds.set_device(0)
+    knobs = {k: getattr(args, k) for k in ("grid", "heavy_degree", "chunk", "round_batch", "use_graph",
+                                           "edges_per_thread") if getattr(args, k) is not None}
+    if knobs:
+        ds.set_tuning(**knobs)
+    log(f"tuning {ds.get_tuning()}")
     {"1": config1, "2": config2, "3": config3, "4a": config4a, "4b": config4b, "5": config5}[args.config](args, ds, np)
 
 

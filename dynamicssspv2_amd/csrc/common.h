@@ -8,7 +8,8 @@
 //          pool uint64[cap]   delta entries appended by insertions (rows unsorted)
 //          tcnt uint32[n]     per-vertex insertion counters, zero between updates
 //          tbm  uint64[n/64]  bitmap of vertices touched by the batch's insertions
-//          wsum/wbase         per-bitmap-word sizes / pool offsets of one batch
+//          wsum/bsum/bbase    pool space per bitmap word / per 256 words, and the
+//                             pool offset of each 256-word group, for one batch
 //   tree   st   uint64[S]     packed state (distance << 32 | parent), S = n * K'
 //                             slots (K' = trees rounded up to a power of two)
 //          fb   2 x uint64[S/64]  frontier bitmaps (ping-pong between rounds)
@@ -46,6 +47,11 @@ constexpr uint32_t ERR_DIST_OVERFLOW = 1u;
 constexpr uint32_t ERR_POOL = 2u;
 constexpr uint32_t ERR_QUEUE = 4u;
 constexpr uint32_t ERR_NO_TIGHT = 8u;
+// A candidate of 2^32-1 or more was offered to an unreachable slot.  That is an
+// overflow only if the slot is still unreachable at convergence; a check pass
+// over the converged tree decides (a finite slot with a live edge to an
+// unreachable one proves it).
+constexpr uint32_t ERR_DIST_SUSPECT = 16u;
 
 // Statistics, accumulated per workgroup slot (no single-address hot spot) and
 // summed into Counters::stats once per update.

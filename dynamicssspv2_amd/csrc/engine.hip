@@ -117,8 +117,9 @@ struct dsssp_graph {
     uint64_t delta_entries = 0;  // sum of delta row lengths
     uint32_t *tcnt = nullptr;
     uint64_t *tbm = nullptr;  // vertices touched by the batch's insertions
-    uint32_t *wsum = nullptr;
-    uint64_t *wbase = nullptr;
+    uint32_t *wsum = nullptr;   // per bitmap word: pool entries needed by the batch
+    uint64_t *bsum = nullptr;   // per 256 words: sum of wsum
+    uint64_t *bbase = nullptr;  // per 256 words: pool offset
     GraphCtl *gc = nullptr;
     GraphCtl *gc_host = nullptr;  // pinned
     bool zero_w = false;          // some weight was ever 0: parents need the tight-BFS repair
@@ -446,6 +447,26 @@ int finish_errors(dsssp_tree *t, uint32_t gerr)
     return DSSSP_OK;
 }
 
+// Decides a suspected overflow (a candidate of 2^32-1 or more offered to an
+// unreachable slot): it is one iff some reachable slot still has a live edge to
+// an unreachable slot.  Runs only when the suspicion was raised; refreshes the
+// host copy of the counters.
+int confirm_overflow(dsssp_tree *t)
+{
+    const uint32_t err = t->c_host->err;
+    if (!(err & ERR_DIST_SUSPECT) || (err & ERR_DIST_OVERFLOW)) return DSSSP_OK;
+    hipStream_t s = t->g->stream;
+    FArgs a = base_args(t);
+    a.identity = 1;
+    a.consume = 0;
+    zero_ring(t);
+    single_pass(t, M_OFCHECK, a);
+    launch_reduce_stats(t->c, t->bstats, s);
+    HIP_TRY(hipMemcpyAsync(t->c_host, t->c, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return DSSSP_OK;
+}
+
 // After a failed update or setup: every bitmap, flag and insertion counter is
 // cleared so later calls start from a consistent state.
 void scrub(dsssp_tree *t)
@@ -477,9 +498,9 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
     if (b->n_ins > 0) {
         launch_fill_u32(&g->gc->err, 0u, 1, s);
         launch_ins_count(b->rec, b->k, g->tcnt, b->slot, g->tbm, s);
-        launch_ins_wordsum(g->tbm, g->nw, g->dbl, g->tcnt, g->wsum, s);
-        launch_ins_scan(g->wsum, g->nw, g->wbase, g->pool_cap, g->gc, s);
-        launch_ins_alloc(g->tbm, g->nw, g->wbase, g->dbl, g->tcnt, g->pool, g->gc, s);
+        launch_ins_wordsum(g->tbm, g->nw, g->dbl, g->tcnt, g->wsum, g->bsum, s);
+        launch_ins_scan(g->bsum, g->nw, g->bbase, g->pool_cap, g->gc, s);
+        launch_ins_alloc(g->tbm, g->nw, g->wsum, g->bbase, g->dbl, g->tcnt, g->pool, g->gc, s);
         launch_ins_scatter(b->rec, b->k, b->slot, g->dbl, g->tcnt, g->pool, b->pos, g->gc, s);
         launch_ins_reset(g->tbm, g->nw, g->tcnt, s);
         if (b->zero_w) g->zero_w = true;
@@ -521,6 +542,7 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
         g->pool_top = g->gc_host->pool_top;
         g->delta_entries += 2 * (uint64_t)b->n_ins;
     }
+    if (!code) code = confirm_overflow(t);
     if (!code) code = finish_errors(t, b->n_ins > 0 ? g->gc_host->err : 0u);
     if (code) {
         const std::string keep = last_error();
@@ -667,7 +689,8 @@ static void graph_release(dsssp_graph *g)
     dev_free(g->tcnt);
     dev_free(g->tbm);
     dev_free(g->wsum);
-    dev_free(g->wbase);
+    dev_free(g->bsum);
+    dev_free(g->bbase);
     dev_free(g->gc);
     if (g->gc_host) (void)hipHostFree(g->gc_host);
     for (auto &e : g->ev)
@@ -690,7 +713,9 @@ static int graph_alloc(dsssp_graph *g, uint64_t pool_cap)
     if (int s = dev_alloc((void **)&g->tcnt, sizeof(uint32_t) * n, "insert counters")) return s;
     if (int s = dev_alloc((void **)&g->tbm, sizeof(uint64_t) * g->nw, "touched bitmap")) return s;
     if (int s = dev_alloc((void **)&g->wsum, sizeof(uint32_t) * g->nw, "word sizes")) return s;
-    if (int s = dev_alloc((void **)&g->wbase, sizeof(uint64_t) * g->nw, "word bases")) return s;
+    const size_t nb = (g->nw + 255) / 256;
+    if (int s = dev_alloc((void **)&g->bsum, sizeof(uint64_t) * nb, "group sizes")) return s;
+    if (int s = dev_alloc((void **)&g->bbase, sizeof(uint64_t) * nb, "group bases")) return s;
     if (int s = dev_alloc((void **)&g->gc, sizeof(GraphCtl), "graph control")) return s;
     g->pool_cap = pool_cap;
     if (int s = dev_alloc((void **)&g->pool, sizeof(uint64_t) * pool_cap, "delta pool")) return s;
@@ -761,8 +786,8 @@ extern "C" int dsssp_graph_info(const dsssp_graph *g, int64_t *n, int64_t *m_liv
         *m_live = m;
     }
     if (device_bytes)
-        *device_bytes = 8 * ((uint64_t)g->n + 1) + 8 * g->E + 12 * (uint64_t)g->n + 20 * (uint64_t)g->nw +
-                        8 * g->pool_cap;
+        *device_bytes = 8 * ((uint64_t)g->n + 1) + 8 * g->E + 12 * (uint64_t)g->n + 12 * (uint64_t)g->nw +
+                        16 * (uint64_t)((g->nw + 255) / 256) + 8 * g->pool_cap;
     return DSSSP_OK;
 }
 
@@ -979,7 +1004,8 @@ static int tree_make(dsssp_graph *g, int32_t ntrees, const std::vector<uint32_t>
         hipStreamSynchronize(g->stream) != hipSuccess) {
         if (!s) s = fail(DSSSP_ERR_HIP, "tree setup failed: %s", hipGetErrorString(hipGetLastError()));
     } else if (!s) {
-        s = finish_errors(t, 0);
+        s = confirm_overflow(t);
+        if (!s) s = finish_errors(t, 0);
     }
     dev_free(d_roots);
     if (s) {

@@ -21,6 +21,7 @@ enum Mode : int {
     M_SEED = 2,   // pull the best live neighbour into an invalidated vertex
     M_TBFS = 3,   // BFS levels over tight edges (zero-weight parent repair)
     M_PICK = 4,   // smallest-id tight neighbour (optionally one level up) as parent
+    M_OFCHECK = 5,  // a reachable slot with a live edge to an unreachable one: overflow
 };
 
 // Arguments of one frontier launch.  The input frontier is a bitmap over
@@ -64,12 +65,14 @@ void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 void launch_reduce_stats(Counters *c, unsigned long long *bstats, hipStream_t s);
 // insertion of a batch into the delta rows
 void launch_ins_count(const int4 *rec, int64_t k, uint32_t *tcnt, uint64_t *slot, uint64_t *tbm, hipStream_t s);
+// Pool space per bitmap word (wsum) and per 256-word group (bsum), the scan of
+// the group sums (bbase, pool top), then the per-vertex moves.
 void launch_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const uint64_t *dbl, const uint32_t *tcnt,
-                        uint32_t *wsum, hipStream_t s);
-void launch_ins_scan(const uint32_t *wsum, uint32_t nwords, uint64_t *wbase, uint64_t pool_cap, GraphCtl *gc,
+                        uint32_t *wsum, uint64_t *bsum, hipStream_t s);
+void launch_ins_scan(const uint64_t *bsum, uint32_t nwords, uint64_t *bbase, uint64_t pool_cap, GraphCtl *gc,
                      hipStream_t s);
-void launch_ins_alloc(const uint64_t *tbm, uint32_t nwords, const uint64_t *wbase, uint64_t *dbl, uint32_t *tcnt,
-                      uint64_t *pool, const GraphCtl *gc, hipStream_t s);
+void launch_ins_alloc(const uint64_t *tbm, uint32_t nwords, const uint32_t *wsum, const uint64_t *bbase,
+                      uint64_t *dbl, uint32_t *tcnt, uint64_t *pool, const GraphCtl *gc, hipStream_t s);
 void launch_ins_scatter(const int4 *rec, int64_t k, const uint64_t *slot, const uint64_t *dbl, const uint32_t *tcnt,
                         uint64_t *pool, uint64_t *pos, const GraphCtl *gc, hipStream_t s);
 void launch_ins_reset(uint64_t *tbm, uint32_t nwords, uint32_t *tcnt, hipStream_t s);

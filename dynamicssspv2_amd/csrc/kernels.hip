@@ -108,8 +108,8 @@ __device__ __forceinline__ void apply(const FArgs &A, uint32_t os, uint32_t aux0
     const uint32_t xs = (x << A.kshift) | (os & ((1u << A.kshift) - 1u));
     if constexpr (MODE == M_RELAX) {
         const uint64_t nd = (uint64_t)aux0 + w;
-        if (nd >= DIST_INF) {
-            atomicOr(&A.c->err, ERR_DIST_OVERFLOW);
+        if (nd >= DIST_INF) {  // cannot beat a finite distance
+            if (p.sx == ST_INF) atomicOr(&A.c->err, ERR_DIST_SUSPECT);
             return;
         }
         const uint64_t cand = pack_state((uint32_t)nd, u);
@@ -135,7 +135,7 @@ __device__ __forceinline__ void apply(const FArgs &A, uint32_t os, uint32_t aux0
         if (p.sx == ST_INF) return;
         const uint64_t nd = (uint64_t)st_dist(p.sx) + w;
         if (nd >= DIST_INF) {
-            atomicOr(&A.c->err, ERR_DIST_OVERFLOW);
+            if (A.st[os] == ST_INF) atomicOr(&A.c->err, ERR_DIST_SUSPECT);
             return;
         }
         const uint64_t cand = pack_state((uint32_t)nd, x);
@@ -151,6 +151,8 @@ __device__ __forceinline__ void apply(const FArgs &A, uint32_t os, uint32_t aux0
         atomicMin(&A.lvl[xs], nl);
         mark(A.out_bm, xs);
         S.any = true;
+    } else if constexpr (MODE == M_OFCHECK) {
+        if (p.sx == ST_INF) atomicOr(&A.c->err, ERR_DIST_OVERFLOW);
     } else {  // M_PICK: os is the slot whose parent is being chosen
         if (p.sx == ST_INF) return;
         if ((uint64_t)st_dist(p.sx) + w != (uint64_t)aux0) return;
@@ -182,6 +184,8 @@ __device__ __forceinline__ bool owner_prep(const FArgs &A, uint32_t os, uint32_t
         aux0 = st_dist(su);
         aux1 = A.lvl[os];
         return su != ST_INF && aux1 != LVL_INF;
+    } else if constexpr (MODE == M_OFCHECK) {
+        return A.st[os] != ST_INF;
     } else if constexpr (MODE == M_PICK) {
         const uint64_t su = A.st[os];
         aux0 = st_dist(su);
@@ -410,33 +414,61 @@ __global__ __launch_bounds__(RB) void k_ins_count(const int4 *rec, int64_t k, ui
     slot[i] = ((uint64_t)ra << 32) | rb;
 }
 
-// wsum[w] = entries the touched vertices of bitmap word w need (old + new).
-__global__ __launch_bounds__(RB) void k_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const uint64_t *dbl,
-                                                    const uint32_t *tcnt, uint32_t *wsum)
+// Exclusive prefix sum over the 256 threads of a workgroup; *total gets the sum.
+__device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t *total)
 {
-    const uint32_t w = blockIdx.x * RB + threadIdx.x;
-    if (w >= nwords) return;
-    uint64_t word = tbm[w];
-    uint32_t sum = 0;
-    while (word) {
-        const uint32_t v = w * 64 + (uint32_t)__builtin_ctzll(word);
-        word &= word - 1;
-        sum += (uint32_t)dbl[v] + tcnt[v];
+    __shared__ uint64_t wtot[RB / 64];
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+    uint64_t incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t y = __shfl_up(incl, d);
+        if ((int)lane >= d) incl += y;
     }
-    wsum[w] = sum;
+    if (lane == 63) wtot[wv] = incl;
+    __syncthreads();
+    uint64_t before = 0, all = 0;
+#pragma unroll
+    for (int i = 0; i < RB / 64; ++i) {
+        before += i < (int)wv ? wtot[i] : 0ull;
+        all += wtot[i];
+    }
+    *total = all;
+    return before + incl - v;
 }
 
-// Single-workgroup exclusive scan of wsum into pool offsets wbase (from the
-// current pool top); advances the pool top.
-__global__ __launch_bounds__(1024) void k_ins_scan(const uint32_t *wsum, uint32_t nwords, uint64_t *wbase,
+// wsum[w] = pool entries the touched vertices of bitmap word w need (old + new
+// entries); bsum[b] = the sum over the 256 words of workgroup b.
+__global__ __launch_bounds__(RB) void k_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const uint64_t *dbl,
+                                                    const uint32_t *tcnt, uint32_t *wsum, uint64_t *bsum)
+{
+    const uint32_t w = blockIdx.x * RB + threadIdx.x;
+    uint32_t sum = 0;
+    if (w < nwords) {
+        uint64_t word = tbm[w];
+        while (word) {
+            const uint32_t v = w * 64 + (uint32_t)__builtin_ctzll(word);
+            word &= word - 1;
+            sum += (uint32_t)dbl[v] + tcnt[v];
+        }
+        wsum[w] = sum;
+    }
+    uint64_t total;
+    (void)block_exclusive_scan(sum, &total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// Single-workgroup exclusive scan of the nb workgroup sums into pool offsets
+// bbase (from the current pool top); advances the pool top.
+__global__ __launch_bounds__(1024) void k_ins_scan(const uint64_t *bsum, uint32_t nb, uint64_t *bbase,
                                                    uint64_t pool_cap, GraphCtl *gc)
 {
     __shared__ unsigned long long part[1024];
     const uint32_t t = threadIdx.x;
-    const uint32_t per = (nwords + 1023) / 1024;
-    const uint32_t lo = min(t * per, nwords), hi = min(lo + per, nwords);
+    const uint32_t per = (nb + 1023) / 1024;
+    const uint32_t lo = min(t * per, nb), hi = min(lo + per, nb);
     unsigned long long s = 0;
-    for (uint32_t i = lo; i < hi; ++i) s += wsum[i];
+    for (uint32_t i = lo; i < hi; ++i) s += bsum[i];
     part[t] = s;
     __syncthreads();
     for (uint32_t d = 1; d < 1024; d <<= 1) {
@@ -448,10 +480,9 @@ __global__ __launch_bounds__(1024) void k_ins_scan(const uint32_t *wsum, uint32_
     const unsigned long long top = gc->pool_top;
     unsigned long long run = top + part[t] - s;
     for (uint32_t i = lo; i < hi; ++i) {
-        wbase[i] = run;
-        run += wsum[i];
+        bbase[i] = run;
+        run += bsum[i];
     }
-    __syncthreads();
     if (t == 1023) {
         const unsigned long long total = part[1023];
         gc->batch_new = total;
@@ -463,14 +494,18 @@ __global__ __launch_bounds__(1024) void k_ins_scan(const uint32_t *wsum, uint32_
 }
 
 // Moves each touched vertex's delta row to its new block (old entries first);
-// leaves the old length in tcnt as the write base of the new entries.
-__global__ __launch_bounds__(RB) void k_ins_alloc(const uint64_t *tbm, uint32_t nwords, const uint64_t *wbase,
-                                                  uint64_t *dbl, uint32_t *tcnt, uint64_t *pool, const GraphCtl *gc)
+// leaves the old length in tcnt as the write base of the new entries.  Word w's
+// block starts at its workgroup's base plus the workgroup-local prefix of wsum.
+__global__ __launch_bounds__(RB) void k_ins_alloc(const uint64_t *tbm, uint32_t nwords, const uint32_t *wsum,
+                                                  const uint64_t *bbase, uint64_t *dbl, uint32_t *tcnt,
+                                                  uint64_t *pool, const GraphCtl *gc)
 {
     const uint32_t w = blockIdx.x * RB + threadIdx.x;
-    if (w >= nwords || (gc->err & ERR_POOL)) return;
+    const uint32_t mine = w < nwords ? wsum[w] : 0u;
+    uint64_t total;
+    uint64_t nb = bbase[blockIdx.x] + block_exclusive_scan(mine, &total);
+    if (w >= nwords || mine == 0 || (gc->err & ERR_POOL)) return;
     uint64_t word = tbm[w];
-    uint64_t nb = wbase[w];
     while (word) {
         const uint32_t v = w * 64 + (uint32_t)__builtin_ctzll(word);
         word &= word - 1;
@@ -621,7 +656,7 @@ __global__ __launch_bounds__(RB) void k_ins_offer(const int4 *rec, int64_t k, ui
             if (sa != ST_INF && !b_root) {
                 const uint64_t nd = (uint64_t)st_dist(sa) + w;
                 if (nd >= DIST_INF) {
-                    atomicOr(&c->err, ERR_DIST_OVERFLOW);
+                    if (sb == ST_INF) atomicOr(&c->err, ERR_DIST_SUSPECT);
                 } else {
                     const uint64_t cand = pack_state((uint32_t)nd, a);
                     if (cand < sb) {
@@ -634,7 +669,7 @@ __global__ __launch_bounds__(RB) void k_ins_offer(const int4 *rec, int64_t k, ui
             if (sb != ST_INF && !a_root) {
                 const uint64_t nd = (uint64_t)st_dist(sb) + w;
                 if (nd >= DIST_INF) {
-                    atomicOr(&c->err, ERR_DIST_OVERFLOW);
+                    if (sa == ST_INF) atomicOr(&c->err, ERR_DIST_SUSPECT);
                 } else {
                     const uint64_t cand = pack_state((uint32_t)nd, b);
                     if (cand < sa) {
@@ -717,7 +752,8 @@ void launch_frontier_ept(int mode, const FArgs &a, int grid, hipStream_t s)
     case M_INVAL: hipLaunchKernelGGL((k_frontier<M_INVAL, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
     case M_SEED: hipLaunchKernelGGL((k_frontier<M_SEED, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
     case M_TBFS: hipLaunchKernelGGL((k_frontier<M_TBFS, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_frontier<M_PICK, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
+    case M_PICK: hipLaunchKernelGGL((k_frontier<M_PICK, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_frontier<M_OFCHECK, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
     }
 }
 
@@ -761,24 +797,25 @@ void launch_ins_count(const int4 *rec, int64_t k, uint32_t *tcnt, uint64_t *slot
 }
 
 void launch_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const uint64_t *dbl, const uint32_t *tcnt,
-                        uint32_t *wsum, hipStream_t s)
+                        uint32_t *wsum, uint64_t *bsum, hipStream_t s)
 {
     if (nwords > 0)
-        hipLaunchKernelGGL(k_ins_wordsum, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, dbl, tcnt, wsum);
+        hipLaunchKernelGGL(k_ins_wordsum, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, dbl, tcnt, wsum,
+                           bsum);
 }
 
-void launch_ins_scan(const uint32_t *wsum, uint32_t nwords, uint64_t *wbase, uint64_t pool_cap, GraphCtl *gc,
+void launch_ins_scan(const uint64_t *bsum, uint32_t nwords, uint64_t *bbase, uint64_t pool_cap, GraphCtl *gc,
                      hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ins_scan, dim3(1), dim3(1024), 0, s, wsum, nwords, wbase, pool_cap, gc);
+    hipLaunchKernelGGL(k_ins_scan, dim3(1), dim3(1024), 0, s, bsum, blocks_for(nwords), bbase, pool_cap, gc);
 }
 
-void launch_ins_alloc(const uint64_t *tbm, uint32_t nwords, const uint64_t *wbase, uint64_t *dbl, uint32_t *tcnt,
-                      uint64_t *pool, const GraphCtl *gc, hipStream_t s)
+void launch_ins_alloc(const uint64_t *tbm, uint32_t nwords, const uint32_t *wsum, const uint64_t *bbase,
+                      uint64_t *dbl, uint32_t *tcnt, uint64_t *pool, const GraphCtl *gc, hipStream_t s)
 {
     if (nwords > 0)
-        hipLaunchKernelGGL(k_ins_alloc, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, wbase, dbl, tcnt, pool,
-                           gc);
+        hipLaunchKernelGGL(k_ins_alloc, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, wsum, bbase, dbl, tcnt,
+                           pool, gc);
 }
 
 void launch_ins_scatter(const int4 *rec, int64_t k, const uint64_t *slot, const uint64_t *dbl, const uint32_t *tcnt,

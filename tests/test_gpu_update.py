@@ -223,11 +223,27 @@ def test_errors_leave_state_untouched():
 
 
 def test_overflow_is_reported():
-    n = 3
-    g = ds.Graph(n, [0, 1], [1, 2], [2_000_000_000, 2_000_000_000])
+    n = 4
+    g = ds.Graph(n, [0, 1, 2], [1, 2, 3], [2_000_000_000] * 3)
     with pytest.raises(ds.DssspError) as e:
         ds.Tree(g, 0)
     assert e.value.code == ds.ERR_OVERFLOW
+
+
+def test_large_distances_below_the_limit():
+    """Distances up to 2^32-2 are exact; candidates past the limit that improve
+    nothing (here 2 -> 1 at 6e9) are no overflow."""
+    n = 4
+    g = ds.Graph(n, [0, 1, 0], [1, 2, 3], [2_000_000_000, 2_000_000_000, 2**31 - 1])
+    t = ds.Tree(g, 0)
+    p, d = t.read()
+    assert list(p) == [0, 0, 1, 0] and list(d) == [0, 2_000_000_000, 4_000_000_000, 2**31 - 1]
+    ds.update(g, t, np.array([[3, 2, 2**31 - 1, 1]], dtype=np.int32))  # 2^32 - 2 via 3: not shorter
+    p, d = t.read()
+    assert list(d) == [0, 2_000_000_000, 4_000_000_000, 2**31 - 1] and list(p) == [0, 0, 1, 0]
+    ds.update(g, t, np.array([[1, 2, 2_000_000_000, 0]], dtype=np.int32))  # now 2 hangs off 3
+    p, d = t.read()
+    assert list(p) == [0, 0, 3, 0] and d[2] == 2**32 - 2
 
 
 def test_update_after_overflow_error():
