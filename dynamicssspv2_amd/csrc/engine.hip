@@ -115,8 +115,12 @@ struct dsssp_graph {
     uint64_t pool_cap = 0;
     uint64_t pool_top = 0;       // host copy, valid between updates
     uint64_t delta_entries = 0;  // sum of delta row lengths
+    uint32_t *dcap = nullptr;  // capacity of each delta block
     uint32_t *tcnt = nullptr;
     uint64_t *tbm = nullptr;  // vertices touched by the batch's insertions
+    uint64_t *dbm = nullptr;  // vertices whose delta block may hold a deleted key
+    uint64_t *dset = nullptr; // deletion set of the batch (2 words a slot)
+    uint32_t dset_slots = 0;
     uint32_t *wsum = nullptr;   // per bitmap word: pool entries needed by the batch
     uint64_t *bsum = nullptr;   // per 256 words: sum of wsum
     uint64_t *bbase = nullptr;  // per 256 words: pool offset
@@ -388,28 +392,38 @@ int ensure_hq(dsssp_tree *t, uint64_t total_entries)
     return DSSSP_OK;
 }
 
-// Makes room in the delta pool for `add` new entries: compacts the pool (live
-// blocks only) and grows it when needed.
-int ensure_pool(dsssp_graph *g, uint64_t add)
+// Makes room in the delta pool for a batch of n_ins insertions.  A vertex whose
+// block overflows gets a block of max(2 cap, need, 4) entries, so one batch
+// allocates at most 2 * (live capacity) + 10 * n_ins entries, and the live
+// capacity is at most the pool top.  When that bound does not fit, the pool is
+// compacted (live blocks only, capacities kept) and grown.
+int ensure_pool(dsssp_graph *g, uint64_t n_ins)
 {
-    if (g->pool_top + g->delta_entries + add <= g->pool_cap) return DSSSP_OK;
-    uint64_t cap = std::max<uint64_t>(g->pool_cap, 2 * (g->delta_entries + add) + (1u << 20));
-    if (cap > 0xFFFFFFFFull) cap = 0xFFFFFFFFull;
-    if (g->delta_entries + add > cap) return fail(DSSSP_ERR_NOMEM, "delta pool would exceed 2^32 entries");
-    // Compacted on the host: rare, and keeps the kernels free of pool bookkeeping.
+    const uint64_t add = 10 * n_ins;
+    if (3 * g->pool_top + add <= g->pool_cap) return DSSSP_OK;
+    // Compacted on the host: rare (geometric growth), and keeps the kernels free
+    // of pool bookkeeping.
     std::vector<uint64_t> dbl((size_t)g->n), pool((size_t)g->pool_top), out;
+    std::vector<uint32_t> dcap((size_t)g->n);
     HIP_TRY(hipMemcpyAsync(dbl.data(), g->dbl, sizeof(uint64_t) * (size_t)g->n, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipMemcpyAsync(dcap.data(), g->dcap, sizeof(uint32_t) * (size_t)g->n, hipMemcpyDeviceToHost, g->stream));
     if (g->pool_top)
         HIP_TRY(hipMemcpyAsync(pool.data(), g->pool, sizeof(uint64_t) * (size_t)g->pool_top, hipMemcpyDeviceToHost,
                                g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
-    out.reserve((size_t)g->delta_entries);
+    uint64_t live = 0;
+    for (int64_t v = 0; v < g->n; ++v) live += dcap[v];
+    uint64_t cap = std::max<uint64_t>(g->pool_cap, 2 * (3 * live + add) + (1u << 24));
+    if (cap > 0xFFFFFFFFull) cap = 0xFFFFFFFFull;
+    if (3 * live + add > cap) return fail(DSSSP_ERR_NOMEM, "delta pool would exceed 2^32 entries");
+    out.reserve((size_t)live);
     for (int64_t v = 0; v < g->n; ++v) {
+        if (!dcap[v]) continue;
         const uint64_t d = dbl[v];
         const uint64_t beg = d >> 32, len = (uint32_t)d;
-        if (!len) continue;
         const uint64_t nb = out.size();
         out.insert(out.end(), pool.begin() + (int64_t)beg, pool.begin() + (int64_t)(beg + len));
+        out.resize(nb + dcap[v], 0ull);
         dbl[v] = (nb << 32) | len;
     }
     uint64_t *np = nullptr;
@@ -476,9 +490,43 @@ void scrub(dsssp_tree *t)
     launch_fill_u64(t->fb[1], 0ull, t->nw, s);
     launch_fill_u64(t->ib, 0ull, t->nw, s);
     launch_fill_u64(t->g->tbm, 0ull, t->g->nw, s);
+    launch_fill_u64(t->g->dbm, 0ull, t->g->nw, s);
     launch_fill_u32(t->g->tcnt, 0u, (uint64_t)t->g->n, s);
     zero_ring(t);
     (void)hipStreamSynchronize(s);
+}
+
+// Deletion set of at least 4 slots per deletion record (load factor <= 1/4).
+int ensure_dset(dsssp_graph *g, int64_t n_del)
+{
+    uint64_t want = 1024;
+    while (want < 4 * (uint64_t)n_del) want <<= 1;
+    if (want > (1ull << 31)) return fail(DSSSP_ERR_ARG, "too many deletions in one batch");
+    if (g->dset_slots >= want) return DSSSP_OK;
+    dev_free(g->dset);
+    g->dset_slots = 0;
+    if (int s = dev_alloc((void **)&g->dset, 16 * want, "deletion set")) return s;
+    g->dset_slots = (uint32_t)want;
+    return DSSSP_OK;
+}
+
+// Tombstones every copy of every deleted (a, b, w): base rows by binary search
+// per record copy, delta blocks by one pass over the marked rows against the
+// batch's deletion set (long delta blocks in chunks over many waves).
+void tombstone(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b)
+{
+    hipStream_t s = g->stream;
+    launch_fill_u64(g->dset, ~0ull, 2 * (uint64_t)g->dset_slots, s);
+    launch_del_set(b->rec, b->k, g->dset, g->dset_slots - 1, s);
+    launch_tomb(b->rec, b->k, g->view(), g->dbm, t->bstats, s);
+    FArgs a = base_args(t);
+    a.kshift = 0;
+    a.nslots = (uint32_t)g->n;
+    a.nwords = g->nw;
+    a.in_bm = g->dbm;  // consumed
+    a.dset = g->dset;
+    a.dmask = g->dset_slots - 1;
+    single_pass(t, M_TOMB, a);
 }
 
 int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stats, double h2d_ms)
@@ -486,7 +534,10 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
     DevGuard dg(g->device);
     hipStream_t s = g->stream;
     if (b->n_ins > 0) {
-        if (int e = ensure_pool(g, 2 * (uint64_t)b->n_ins)) return e;
+        if (int e = ensure_pool(g, (uint64_t)b->n_ins)) return e;
+    }
+    if (b->n_del > 0) {
+        if (int e = ensure_dset(g, b->n_del)) return e;
     }
     if (int e = ensure_hq(t, g->E + g->delta_entries + 2 * (uint64_t)b->n_ins)) return e;
     int64_t inv_rounds = 0, relax_rounds = 0, repair_rounds = 0;
@@ -498,9 +549,9 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
     if (b->n_ins > 0) {
         launch_fill_u32(&g->gc->err, 0u, 1, s);
         launch_ins_count(b->rec, b->k, g->tcnt, b->slot, g->tbm, s);
-        launch_ins_wordsum(g->tbm, g->nw, g->dbl, g->tcnt, g->wsum, g->bsum, s);
+        launch_ins_wordsum(g->tbm, g->nw, g->dbl, g->dcap, g->tcnt, g->wsum, g->bsum, s);
         launch_ins_scan(g->bsum, g->nw, g->bbase, g->pool_cap, g->gc, s);
-        launch_ins_alloc(g->tbm, g->nw, g->wsum, g->bbase, g->dbl, g->tcnt, g->pool, g->gc, s);
+        launch_ins_alloc(g->tbm, g->nw, g->wsum, g->bbase, g->dbl, g->dcap, g->tcnt, g->pool, g->gc, s);
         launch_ins_scatter(b->rec, b->k, b->slot, g->dbl, g->tcnt, g->pool, b->pos, g->gc, s);
         launch_ins_reset(g->tbm, g->nw, g->tcnt, s);
         if (b->zero_w) g->zero_w = true;
@@ -510,7 +561,8 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
     int code = DSSSP_OK;
     if (b->n_del > 0) {
         zero_ring(t);
-        launch_classify(b->rec, b->k, t->kshift, g->view(), t->st, t->fb[0], t->ib, &t->c->act[0], t->bstats, s);
+        launch_classify(b->rec, b->k, t->kshift, t->st, t->fb[0], t->ib, &t->c->act[0], s);
+        tombstone(g, t, b);
     }
     HIP_TRY(hipEventRecord(g->ev[2], s));
     if (b->n_del > 0) code = run_rounds(t, M_INVAL, inv_rounds, nullptr);
@@ -687,7 +739,10 @@ static void graph_release(dsssp_graph *g)
     dev_free(g->dbl);
     dev_free(g->pool);
     dev_free(g->tcnt);
+    dev_free(g->dcap);
     dev_free(g->tbm);
+    dev_free(g->dbm);
+    dev_free(g->dset);
     dev_free(g->wsum);
     dev_free(g->bsum);
     dev_free(g->bbase);
@@ -711,7 +766,9 @@ static int graph_alloc(dsssp_graph *g, uint64_t pool_cap)
     if (int s = dev_alloc((void **)&g->adj, sizeof(uint64_t) * g->E, "adjacency")) return s;
     if (int s = dev_alloc((void **)&g->dbl, sizeof(uint64_t) * n, "delta rows")) return s;
     if (int s = dev_alloc((void **)&g->tcnt, sizeof(uint32_t) * n, "insert counters")) return s;
+    if (int s = dev_alloc((void **)&g->dcap, sizeof(uint32_t) * n, "delta capacities")) return s;
     if (int s = dev_alloc((void **)&g->tbm, sizeof(uint64_t) * g->nw, "touched bitmap")) return s;
+    if (int s = dev_alloc((void **)&g->dbm, sizeof(uint64_t) * g->nw, "deletion rows bitmap")) return s;
     if (int s = dev_alloc((void **)&g->wsum, sizeof(uint32_t) * g->nw, "word sizes")) return s;
     const size_t nb = (g->nw + 255) / 256;
     if (int s = dev_alloc((void **)&g->bsum, sizeof(uint64_t) * nb, "group sizes")) return s;
@@ -723,7 +780,9 @@ static int graph_alloc(dsssp_graph *g, uint64_t pool_cap)
     memset(g->gc_host, 0, sizeof(GraphCtl));
     HIP_TRY(hipMemset(g->dbl, 0, sizeof(uint64_t) * n));
     HIP_TRY(hipMemset(g->tcnt, 0, sizeof(uint32_t) * n));
+    HIP_TRY(hipMemset(g->dcap, 0, sizeof(uint32_t) * n));
     HIP_TRY(hipMemset(g->tbm, 0, sizeof(uint64_t) * g->nw));
+    HIP_TRY(hipMemset(g->dbm, 0, sizeof(uint64_t) * g->nw));
     HIP_TRY(hipMemset(g->gc, 0, sizeof(GraphCtl)));
     return DSSSP_OK;
 }
@@ -744,7 +803,7 @@ extern "C" int dsssp_graph_create(int64_t n, int64_t m, const int32_t *a, const 
     g->n = n;
     g->E = adj.size();
     for (int64_t i = 0; i < m && !g->zero_w; ++i) g->zero_w = w[i] == 0;
-    const uint64_t pool_cap = std::max<uint64_t>(1u << 20, std::min<uint64_t>(g->E / 4, 0xFFFFFFFFull / 2));
+    const uint64_t pool_cap = std::min<uint64_t>(g->E / 2 + (1u << 20), 0xFFFFFFFFull / 2);
     if (int s = graph_alloc(g, pool_cap)) {
         graph_release(g);
         return s;
@@ -786,8 +845,8 @@ extern "C" int dsssp_graph_info(const dsssp_graph *g, int64_t *n, int64_t *m_liv
         *m_live = m;
     }
     if (device_bytes)
-        *device_bytes = 8 * ((uint64_t)g->n + 1) + 8 * g->E + 12 * (uint64_t)g->n + 12 * (uint64_t)g->nw +
-                        16 * (uint64_t)((g->nw + 255) / 256) + 8 * g->pool_cap;
+        *device_bytes = 8 * ((uint64_t)g->n + 1) + 8 * g->E + 16 * (uint64_t)g->n + 20 * (uint64_t)g->nw +
+                        16 * (uint64_t)((g->nw + 255) / 256) + 8 * g->pool_cap + 16 * (uint64_t)g->dset_slots;
     return DSSSP_OK;
 }
 
@@ -866,11 +925,13 @@ extern "C" int dsssp_graph_copy(dsssp_graph *dst, dsssp_graph *src)
     HIP_TRY(hipMemcpyAsync(dst->rp, src->rp, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToDevice, s));
     if (src->E) HIP_TRY(hipMemcpyAsync(dst->adj, src->adj, sizeof(uint64_t) * src->E, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemcpyAsync(dst->dbl, src->dbl, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dst->dcap, src->dcap, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
     if (src->pool_top)
         HIP_TRY(hipMemcpyAsync(dst->pool, src->pool, sizeof(uint64_t) * src->pool_top, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemcpyAsync(dst->gc, src->gc, sizeof(GraphCtl), hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemsetAsync(dst->tcnt, 0, sizeof(uint32_t) * n, s));
     HIP_TRY(hipMemsetAsync(dst->tbm, 0, sizeof(uint64_t) * dst->nw, s));
+    HIP_TRY(hipMemsetAsync(dst->dbm, 0, sizeof(uint64_t) * dst->nw, s));
     HIP_TRY(hipStreamSynchronize(s));
     dst->pool_top = src->pool_top;
     dst->delta_entries = src->delta_entries;

@@ -22,6 +22,7 @@ enum Mode : int {
     M_TBFS = 3,   // BFS levels over tight edges (zero-weight parent repair)
     M_PICK = 4,   // smallest-id tight neighbour (optionally one level up) as parent
     M_OFCHECK = 5,  // a reachable slot with a live edge to an unreachable one: overflow
+    M_TOMB = 6,     // delta rows only: tombstone entries whose key is in the deletion set
 };
 
 // Arguments of one frontier launch.  The input frontier is a bitmap over
@@ -55,6 +56,8 @@ struct FArgs {
     uint32_t use_lvl;       // M_PICK: require lvl[parent] + 1 == lvl[v]
     uint32_t kshift;        // log2 of the trees per vertex: slot = vertex << kshift | tree
     uint32_t ept;           // edges in flight per lane: 1, 2 or 4
+    const uint64_t *dset;   // M_TOMB: deletion set (open addressing, 2 words a slot)
+    uint32_t dmask;         // M_TOMB: slots - 1
 };
 
 constexpr int FRONTIER_BLOCK = 256;
@@ -66,19 +69,29 @@ void launch_reduce_stats(Counters *c, unsigned long long *bstats, hipStream_t s)
 // insertion of a batch into the delta rows
 void launch_ins_count(const int4 *rec, int64_t k, uint32_t *tcnt, uint64_t *slot, uint64_t *tbm, hipStream_t s);
 // Pool space per bitmap word (wsum) and per 256-word group (bsum), the scan of
-// the group sums (bbase, pool top), then the per-vertex moves.
-void launch_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const uint64_t *dbl, const uint32_t *tcnt,
-                        uint32_t *wsum, uint64_t *bsum, hipStream_t s);
+// the group sums (bbase, pool top), then the per-vertex block moves.  Delta
+// blocks have a capacity (dcap) that doubles when outgrown.
+void launch_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const uint64_t *dbl, const uint32_t *dcap,
+                        const uint32_t *tcnt, uint32_t *wsum, uint64_t *bsum, hipStream_t s);
 void launch_ins_scan(const uint64_t *bsum, uint32_t nwords, uint64_t *bbase, uint64_t pool_cap, GraphCtl *gc,
                      hipStream_t s);
 void launch_ins_alloc(const uint64_t *tbm, uint32_t nwords, const uint32_t *wsum, const uint64_t *bbase,
-                      uint64_t *dbl, uint32_t *tcnt, uint64_t *pool, const GraphCtl *gc, hipStream_t s);
+                      uint64_t *dbl, uint32_t *dcap, uint32_t *tcnt, uint64_t *pool, const GraphCtl *gc,
+                      hipStream_t s);
 void launch_ins_scatter(const int4 *rec, int64_t k, const uint64_t *slot, const uint64_t *dbl, const uint32_t *tcnt,
                         uint64_t *pool, uint64_t *pos, const GraphCtl *gc, hipStream_t s);
 void launch_ins_reset(uint64_t *tbm, uint32_t nwords, uint32_t *tcnt, hipStream_t s);
 // deletions, offers, roots, parent repair
-void launch_classify(const int4 *rec, int64_t k, uint32_t kshift, GView g, uint64_t *st, uint64_t *out_bm,
-                     uint64_t *inv_bm, uint32_t *out_act, unsigned long long *bstats, hipStream_t s);
+// Deletions: the tree-edge test (marks the roots of invalidated subtrees) and
+// the tombstoning of every copy of each deleted entry.
+void launch_classify(const int4 *rec, int64_t k, uint32_t kshift, uint64_t *st, uint64_t *out_bm, uint64_t *inv_bm,
+                     uint32_t *out_act, hipStream_t s);
+// The batch's deleted keys (min(a,b), max(a,b), w) go into an open-addressing
+// set of dmask+1 slots (2 words each, cleared to ~0 beforehand); base rows are
+// searched per record copy, and rows whose delta block is non-empty are marked
+// in dbm for an M_TOMB pass.
+void launch_del_set(const int4 *rec, int64_t k, uint64_t *dset, uint32_t dmask, hipStream_t s);
+void launch_tomb(const int4 *rec, int64_t k, GView g, uint64_t *dbm, unsigned long long *bstats, hipStream_t s);
 void launch_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint64_t *pos, const uint64_t *pool,
                       uint64_t *st, uint64_t *out_bm, uint32_t *out_act, Counters *c, hipStream_t s);
 // Roots: for j < count, vertex src[j] becomes a root (distance 0, parent itself)

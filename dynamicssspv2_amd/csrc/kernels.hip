@@ -59,6 +59,25 @@ __device__ __forceinline__ void add_stat(unsigned long long *bstats, int which, 
     if (v) atomicAdd(&bstats[(blockIdx.x % STAT_SLOTS) * NSTAT + which], (unsigned long long)v);
 }
 
+// Deletion set: slot i holds key words dset[2i] = min(a,b) << 32 | max(a,b) (~0:
+// empty) and dset[2i+1] = weight.
+__device__ __forceinline__ uint32_t dset_hash(uint64_t k1, uint32_t w)
+{
+    uint64_t h = k1 * 0x9E3779B97F4A7C15ull ^ ((uint64_t)w * 0xC2B2AE3D27D4EB4Full);
+    h ^= h >> 29;
+    return (uint32_t)(h ^ (h >> 32));
+}
+
+__device__ __forceinline__ bool dset_has(const uint64_t *dset, uint32_t dmask, uint32_t a, uint32_t b, uint32_t w)
+{
+    const uint64_t k1 = a < b ? ((uint64_t)a << 32 | b) : ((uint64_t)b << 32 | a);
+    for (uint32_t i = dset_hash(k1, w) & dmask;; i = (i + 1) & dmask) {
+        const uint64_t k = dset[2 * i];
+        if (k == ~0ull) return false;
+        if (k == k1 && (uint32_t)dset[2 * i + 1] == w) return true;
+    }
+}
+
 struct WaveStats {
     uint32_t front;  // wave-uniform
     uint64_t edges;  // wave-uniform
@@ -97,9 +116,10 @@ __device__ __forceinline__ Pre gather(const FArgs &A, uint32_t os, uint64_t ent)
 
 // Applies one edge; marks the slot to visit next.  A mark made on a state read
 // that was stale is harmless: that slot was changed in this round anyway.
+// The pull modes (SEED, PICK) only lower `pull`, the owner's candidate.
 template <int MODE>
 __device__ __forceinline__ void apply(const FArgs &A, uint32_t os, uint32_t aux0, uint32_t aux1, uint64_t ent,
-                                      const Pre &p, WaveStats &S)
+                                      const Pre &p, WaveStats &S, uint64_t &pull)
 {
     if (p.skip) return;
     const uint32_t u = os >> A.kshift;
@@ -131,18 +151,15 @@ __device__ __forceinline__ void apply(const FArgs &A, uint32_t os, uint32_t aux0
         mark(A.inv_bm, xs);
         S.any = true;
     } else if constexpr (MODE == M_SEED) {
-        // u was invalidated: take the best still-valid neighbour.
+        // u was invalidated: candidate from a still-valid neighbour, reduced per
+        // owner by the caller (every lane of a row targets the same slot).
         if (p.sx == ST_INF) return;
         const uint64_t nd = (uint64_t)st_dist(p.sx) + w;
         if (nd >= DIST_INF) {
-            if (A.st[os] == ST_INF) atomicOr(&A.c->err, ERR_DIST_SUSPECT);
+            atomicOr(&A.c->err, ERR_DIST_SUSPECT);  // the owner is unreachable so far
             return;
         }
-        const uint64_t cand = pack_state((uint32_t)nd, x);
-        if (cand >= A.st[os]) return;
-        atomicMin((unsigned long long *)&A.st[os], (unsigned long long)cand);
-        mark(A.out_bm, os);
-        S.any = true;
+        pull = min(pull, pack_state((uint32_t)nd, x));
     } else if constexpr (MODE == M_TBFS) {
         if (p.sx == ST_INF || st_parent(p.sx) == x) return;
         if ((uint64_t)aux0 + w != (uint64_t)st_dist(p.sx)) return;
@@ -157,7 +174,7 @@ __device__ __forceinline__ void apply(const FArgs &A, uint32_t os, uint32_t aux0
         if (p.sx == ST_INF) return;
         if ((uint64_t)st_dist(p.sx) + w != (uint64_t)aux0) return;
         if (A.use_lvl && p.lx + 1 != aux1) return;
-        if (x < A.best[os]) atomicMin(&A.best[os], x);
+        pull = min(pull, (uint64_t)x);
     }
 }
 
@@ -169,7 +186,26 @@ struct TileSmem {
     uint32_t blen[FW][64];
     uint32_t dbeg[FW][64];
     uint64_t b0[FW][64];
+    unsigned long long pull[FW][64];  // SEED / PICK: per-owner minimum candidate
 };
+
+constexpr bool is_pull(int mode) { return mode == M_SEED || mode == M_PICK; }
+
+// Owner side of a pull mode: the reduced candidate `best` of slot os.  Returns
+// true when it lowered the slot (SEED), so that the slot is marked.
+template <int MODE>
+__device__ __forceinline__ bool pull_commit(const FArgs &A, uint32_t os, uint64_t best)
+{
+    if (best == ~0ull) return false;
+    if constexpr (MODE == M_SEED) {
+        if (best >= A.st[os]) return false;
+        atomicMin((unsigned long long *)&A.st[os], (unsigned long long)best);
+        return true;
+    } else {
+        atomicMin(&A.best[os], (uint32_t)best);
+        return false;
+    }
+}
 
 // Owner-side fields of slot os; returns false when the owner has nothing to scan.
 template <int MODE>
@@ -219,8 +255,10 @@ __device__ void do_tile(const FArgs &A, uint32_t base, uint64_t word, TileSmem &
     uint64_t b0 = 0;
     if (valid) {
         const uint32_t u = os >> A.kshift;
-        b0 = A.g.rp[u];
-        blen = (uint32_t)(A.g.rp[u + 1] - b0);
+        if constexpr (MODE != M_TOMB) {  // M_TOMB scans delta rows only
+            b0 = A.g.rp[u];
+            blen = (uint32_t)(A.g.rp[u + 1] - b0);
+        }
         const uint64_t dd = A.g.dbl[u];
         dbeg = (uint32_t)(dd >> 32);
         deg = blen + (uint32_t)dd;
@@ -246,19 +284,20 @@ __device__ void do_tile(const FArgs &A, uint32_t base, uint64_t word, TileSmem &
     sm.blen[wv][lane] = blen;
     sm.dbeg[wv][lane] = dbeg;
     sm.b0[wv][lane] = b0;
+    if constexpr (is_pull(MODE)) sm.pull[wv][lane] = ~0ull;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     for (uint32_t eb = 0; eb < total; eb += 64 * EPT) {
         uint64_t ent[EPT];
-        uint32_t ows[EPT], a0[EPT], a1[EPT];
+        uint32_t ows[EPT], a0[EPT], a1[EPT], oi[EPT];
         bool act[EPT];
 #pragma unroll
         for (int j = 0; j < EPT; ++j) {
             const uint32_t e = eb + j * 64 + lane;
             act[j] = e < total;
             ent[j] = 0;
-            ows[j] = a0[j] = a1[j] = 0;
+            ows[j] = a0[j] = a1[j] = oi[j] = 0;
             if (act[j]) {
                 uint32_t o = 0;
 #pragma unroll
@@ -266,22 +305,52 @@ __device__ void do_tile(const FArgs &A, uint32_t base, uint64_t word, TileSmem &
                     if (sm.off[wv][o + st] <= e) o += st;
                 const uint32_t jj = e - sm.off[wv][o];
                 const uint32_t bl = sm.blen[wv][o];
+                oi[j] = o;
                 ows[j] = sm.u[wv][o];
                 a0[j] = sm.aux0[wv][o];
                 a1[j] = sm.aux1[wv][o];
                 ent[j] = jj < bl ? A.g.adj[sm.b0[wv][o] + jj] : A.g.pool[(uint64_t)sm.dbeg[wv][o] + (jj - bl)];
             }
         }
+        if constexpr (MODE == M_TOMB) {
+#pragma unroll
+            for (int j = 0; j < EPT; ++j) {
+                if (!act[j] || (ent[j] & E_DEAD)) continue;
+                if (dset_has(A.dset, A.dmask, ows[j], e_col(ent[j]), e_w(ent[j]))) {
+                    const uint32_t e = eb + j * 64 + lane;
+                    const uint32_t o = oi[j];
+                    A.g.pool[(uint64_t)sm.dbeg[wv][o] + (e - sm.off[wv][o])] = ent[j] | E_DEAD;
+                }
+            }
+            continue;
+        }
         Pre pre[EPT];
 #pragma unroll
         for (int j = 0; j < EPT; ++j) pre[j] = act[j] ? gather<MODE>(A, ows[j], ent[j]) : Pre{ST_INF, LVL_INF, true};
 #pragma unroll
-        for (int j = 0; j < EPT; ++j) apply<MODE>(A, ows[j], a0[j], a1[j], ent[j], pre[j], S);
+        for (int j = 0; j < EPT; ++j) {
+            uint64_t pull = ~0ull;
+            apply<MODE>(A, ows[j], a0[j], a1[j], ent[j], pre[j], S, pull);
+            if constexpr (is_pull(MODE)) {
+                if (pull != ~0ull) atomicMin(&sm.pull[wv][oi[j]], (unsigned long long)pull);
+            }
+        }
     }
     S.edges += total;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if constexpr (is_pull(MODE)) {
+        // One commit per owner, and the tile's marks as one bitmap-word update
+        // (a tile is the 64 slots of one bitmap word).
+        const bool hit = deg != 0 && pull_commit<MODE>(A, os, sm.pull[wv][lane]);
+        const unsigned long long mask = __ballot(hit);
+        if (mask && lane == 0) atomicOr((unsigned long long *)&A.out_bm[base >> 6], mask);
+        S.any |= hit;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
 }
 
 template <int MODE, int EPT>
@@ -292,8 +361,8 @@ __device__ void do_chunk(const FArgs &A, uint32_t item, WaveStats &S)
     const uint32_t os = (uint32_t)(hv >> 32);
     const uint32_t u = os >> A.kshift;
     const uint32_t c = (uint32_t)hv;
-    const uint64_t b0 = A.g.rp[u];
-    const uint32_t blen = (uint32_t)(A.g.rp[u + 1] - b0);
+    const uint64_t b0 = MODE == M_TOMB ? 0ull : A.g.rp[u];
+    const uint32_t blen = MODE == M_TOMB ? 0u : (uint32_t)(A.g.rp[u + 1] - b0);
     const uint64_t dd = A.g.dbl[u];
     const uint32_t dbeg = (uint32_t)(dd >> 32);
     const uint32_t deg = blen + (uint32_t)dd;
@@ -301,6 +370,7 @@ __device__ void do_chunk(const FArgs &A, uint32_t item, WaveStats &S)
     if (!owner_prep<MODE>(A, os, aux0, aux1)) return;
     const uint32_t start = c * A.chunk;
     const uint32_t end = min(start + A.chunk, deg);
+    uint64_t pull = ~0ull;  // pull modes: this lane's best candidate for os
     for (uint32_t jb = start; jb < end; jb += 64 * EPT) {
         uint64_t ent[EPT];
         bool act[EPT];
@@ -310,13 +380,30 @@ __device__ void do_chunk(const FArgs &A, uint32_t item, WaveStats &S)
             act[j] = jj < end;
             ent[j] = act[j] ? (jj < blen ? A.g.adj[b0 + jj] : A.g.pool[(uint64_t)dbeg + (jj - blen)]) : 0ull;
         }
+        if constexpr (MODE == M_TOMB) {
+#pragma unroll
+            for (int j = 0; j < EPT; ++j) {
+                if (!act[j] || (ent[j] & E_DEAD)) continue;
+                if (dset_has(A.dset, A.dmask, u, e_col(ent[j]), e_w(ent[j])))
+                    A.g.pool[(uint64_t)dbeg + (jb + j * 64 + lane)] = ent[j] | E_DEAD;
+            }
+            continue;
+        }
         Pre pre[EPT];
 #pragma unroll
         for (int j = 0; j < EPT; ++j) pre[j] = act[j] ? gather<MODE>(A, os, ent[j]) : Pre{ST_INF, LVL_INF, true};
 #pragma unroll
-        for (int j = 0; j < EPT; ++j) apply<MODE>(A, os, aux0, aux1, ent[j], pre[j], S);
+        for (int j = 0; j < EPT; ++j) apply<MODE>(A, os, aux0, aux1, ent[j], pre[j], S, pull);
     }
     if (end > start) S.edges += end - start;
+    if constexpr (is_pull(MODE)) {
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) pull = min(pull, (uint64_t)__shfl_xor((unsigned long long)pull, d));
+        if (lane == 0 && pull_commit<MODE>(A, os, pull)) {
+            mark(A.out_bm, os);
+            S.any = true;
+        }
+    }
 }
 
 template <int MODE, int EPT>
@@ -365,6 +452,8 @@ __global__ __launch_bounds__(FB) void k_frontier(FArgs A)
             add_stat(A.bstats, ST_RELAX_FRONT, S.front);
             add_stat(A.bstats, ST_RELAX_EDGES, S.edges);
             add_stat(A.bstats, ST_RELAX_MARKS, marks);
+        } else if constexpr (MODE == M_TOMB) {
+            add_stat(A.bstats, ST_PROBES, S.edges);
         } else {
             add_stat(A.bstats, ST_OTHER_EDGES, S.edges);
             if constexpr (MODE == M_SEED) add_stat(A.bstats, ST_INVALIDATED, S.front);
@@ -437,10 +526,19 @@ __device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t *t
     return before + incl - v;
 }
 
-// wsum[w] = pool entries the touched vertices of bitmap word w need (old + new
-// entries); bsum[b] = the sum over the 256 words of workgroup b.
+// Capacity of a delta block that must hold `need` entries and holds `cap`:
+// doubling, so that a row is moved O(log length) times over its lifetime.
+__device__ __forceinline__ uint32_t grown_cap(uint32_t cap, uint32_t need)
+{
+    return need <= cap ? cap : max(max(2 * cap, need), 4u);
+}
+
+// wsum[w] = new pool entries the touched vertices of bitmap word w need (a new
+// block for each vertex whose block overflows); bsum[b] = the sum over the 256
+// words of workgroup b.
 __global__ __launch_bounds__(RB) void k_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const uint64_t *dbl,
-                                                    const uint32_t *tcnt, uint32_t *wsum, uint64_t *bsum)
+                                                    const uint32_t *dcap, const uint32_t *tcnt, uint32_t *wsum,
+                                                    uint64_t *bsum)
 {
     const uint32_t w = blockIdx.x * RB + threadIdx.x;
     uint32_t sum = 0;
@@ -449,7 +547,8 @@ __global__ __launch_bounds__(RB) void k_ins_wordsum(const uint64_t *tbm, uint32_
         while (word) {
             const uint32_t v = w * 64 + (uint32_t)__builtin_ctzll(word);
             word &= word - 1;
-            sum += (uint32_t)dbl[v] + tcnt[v];
+            const uint32_t cap = dcap[v], need = (uint32_t)dbl[v] + tcnt[v];
+            if (need > cap) sum += grown_cap(cap, need);
         }
         wsum[w] = sum;
     }
@@ -493,29 +592,36 @@ __global__ __launch_bounds__(1024) void k_ins_scan(const uint64_t *bsum, uint32_
     }
 }
 
-// Moves each touched vertex's delta row to its new block (old entries first);
-// leaves the old length in tcnt as the write base of the new entries.  Word w's
-// block starts at its workgroup's base plus the workgroup-local prefix of wsum.
+// Gives each touched vertex room for its new entries: in place when its block
+// has the capacity, else a new block (old entries moved first).  Leaves the old
+// length in tcnt as the write base of the new entries.  Word w's new blocks
+// start at its workgroup's base plus the workgroup-local prefix of wsum.
 __global__ __launch_bounds__(RB) void k_ins_alloc(const uint64_t *tbm, uint32_t nwords, const uint32_t *wsum,
-                                                  const uint64_t *bbase, uint64_t *dbl, uint32_t *tcnt,
-                                                  uint64_t *pool, const GraphCtl *gc)
+                                                  const uint64_t *bbase, uint64_t *dbl, uint32_t *dcap,
+                                                  uint32_t *tcnt, uint64_t *pool, const GraphCtl *gc)
 {
     const uint32_t w = blockIdx.x * RB + threadIdx.x;
     const uint32_t mine = w < nwords ? wsum[w] : 0u;
     uint64_t total;
     uint64_t nb = bbase[blockIdx.x] + block_exclusive_scan(mine, &total);
-    if (w >= nwords || mine == 0 || (gc->err & ERR_POOL)) return;
+    if (w >= nwords || (gc->err & ERR_POOL)) return;
     uint64_t word = tbm[w];
     while (word) {
         const uint32_t v = w * 64 + (uint32_t)__builtin_ctzll(word);
         word &= word - 1;
         const uint64_t d = dbl[v];
         const uint64_t beg = d >> 32;
-        const uint32_t len = (uint32_t)d, add = tcnt[v];
-        for (uint32_t j = 0; j < len; ++j) pool[nb + j] = pool[beg + j];
-        dbl[v] = (nb << 32) | (uint64_t)(len + add);
+        const uint32_t len = (uint32_t)d, add = tcnt[v], cap = dcap[v];
+        if (len + add <= cap) {
+            dbl[v] = (beg << 32) | (uint64_t)(len + add);
+        } else {
+            const uint32_t nc = grown_cap(cap, len + add);
+            for (uint32_t j = 0; j < len; ++j) pool[nb + j] = pool[beg + j];
+            dbl[v] = (nb << 32) | (uint64_t)(len + add);
+            dcap[v] = nc;
+            nb += nc;
+        }
         tcnt[v] = len;
-        nb += len + add;
     }
 }
 
@@ -557,81 +663,95 @@ __global__ __launch_bounds__(RB) void k_ins_reset(uint64_t *tbm, uint32_t nwords
 // scan of the short delta row).
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ uint32_t tomb_row(GView g, uint32_t r, uint32_t c, uint32_t w)
+// Deletion set of the batch: one slot per deletion record (duplicates too).
+__global__ __launch_bounds__(RB) void k_del_set(const int4 *rec, int64_t k, uint64_t *dset, uint32_t dmask)
 {
-    const uint64_t key = pack_entry(c, w);
-    uint64_t lo = g.rp[r], hi = g.rp[r + 1];
-    const uint64_t end = hi;
-    uint32_t probes = 0;
-    while (lo < hi) {
-        const uint64_t mid = (lo + hi) >> 1;
-        ++probes;
-        if (e_key(g.adj[mid]) < key)
-            lo = mid + 1;
-        else
-            hi = mid;
+    const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
+    if (i >= k) return;
+    const int4 r = rec[i];
+    if (r.w != 0) return;
+    const uint32_t a = (uint32_t)r.x, b = (uint32_t)r.y, w = (uint32_t)r.z;
+    const uint64_t k1 = a < b ? ((uint64_t)a << 32 | b) : ((uint64_t)b << 32 | a);
+    for (uint32_t s = dset_hash(k1, w) & dmask;; s = (s + 1) & dmask) {
+        if (atomicCAS((unsigned long long *)&dset[2 * s], ~0ull, (unsigned long long)k1) == ~0ull) {
+            dset[2 * s + 1] = w;  // read by later kernels only
+            return;
+        }
     }
-    for (uint64_t j = lo; j < end; ++j) {
-        const uint64_t e = g.adj[j];
-        ++probes;
-        if (e_key(e) != key) break;
-        if (!(e & E_DEAD)) g.adj[j] = e | E_DEAD;
-    }
-    const uint64_t d = g.dbl[r];
-    const uint64_t beg = d >> 32;
-    const uint32_t len = (uint32_t)d;
-    for (uint32_t j = 0; j < len; ++j) {
-        const uint64_t e = g.pool[beg + j];
-        if (e_key(e) == key && !(e & E_DEAD)) g.pool[beg + j] = e | E_DEAD;
-    }
-    return probes + len;
 }
 
-// Thread i handles record i >> kshift for tree i & mask; tree 0 also tombstones.
-__global__ __launch_bounds__(RB) void k_classify(const int4 *rec, int64_t k, uint32_t kshift, GView g, uint64_t *st,
-                                                 uint64_t *out_bm, uint64_t *inv_bm, uint32_t *out_act,
-                                                 unsigned long long *bstats)
+// Deletion records: thread 2i+s tombstones copy s of record i in the base rows
+// (row a, then row b; a self-loop's two copies both sit in row a): a binary
+// search of the sorted row, then the run of equal keys.  Rows with a non-empty
+// delta block are marked in dbm; their delta entries are tombstoned by an
+// M_TOMB pass against the deletion set.
+__global__ __launch_bounds__(RB) void k_tomb(const int4 *rec, int64_t k, GView g, uint64_t *dbm,
+                                             unsigned long long *bstats)
+{
+    const int64_t t = (int64_t)blockIdx.x * RB + threadIdx.x;
+    const int64_t ri = t >> 1;
+    uint32_t probes = 0;
+    if (ri < k) {
+        const int4 r = rec[ri];
+        const bool side = t & 1;
+        if (r.w == 0 && !(side && r.x == r.y)) {
+            const uint32_t row = (uint32_t)(side ? r.y : r.x), col = (uint32_t)(side ? r.x : r.y);
+            const uint64_t key = pack_entry(col, (uint32_t)r.z);
+            uint64_t lo = g.rp[row], hi = g.rp[row + 1];
+            const uint64_t end = hi;
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) >> 1;
+                ++probes;
+                if (e_key(g.adj[mid]) < key)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            for (uint64_t j = lo; j < end; ++j) {
+                const uint64_t e = g.adj[j];
+                ++probes;
+                if (e_key(e) != key) break;
+                if (!(e & E_DEAD)) g.adj[j] = e | E_DEAD;
+            }
+            if ((uint32_t)g.dbl[row]) mark(dbm, row);
+        }
+    }
+    const uint32_t p = wave_sum(probes);
+    if (lane_id() == 0) add_stat(bstats, ST_PROBES, p);
+}
+
+// deleteEdge's tree-edge test (gpuFunctions_undir.cuh:189-198), weight ignored:
+// the child endpoint of a deleted tree edge roots an invalidated subtree.
+// Thread i handles record i >> kshift for tree i & mask.
+__global__ __launch_bounds__(RB) void k_classify(const int4 *rec, int64_t k, uint32_t kshift, uint64_t *st,
+                                                 uint64_t *out_bm, uint64_t *inv_bm, uint32_t *out_act)
 {
     const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
     const int64_t ri = i >> kshift;
     const uint32_t tree = (uint32_t)i & ((1u << kshift) - 1u);
     bool hit = false;
-    uint32_t probes = 0;
     if (ri < k) {
         const int4 r = rec[ri];
-        if (r.w == 0) {
-            const uint32_t a = (uint32_t)r.x, b = (uint32_t)r.y;
-            if (a != b) {
-                // deleteEdge's tree-edge test (gpuFunctions_undir.cuh:189-198): the
-                // child endpoint of a tree edge roots an invalidated subtree.
-                const uint32_t sa = (a << kshift) | tree, sb = (b << kshift) | tree;
-                const uint64_t vb = st[sb];
-                uint32_t victim = 0xFFFFFFFFu;
-                if (vb != ST_INF && st_parent(vb) == a) {
-                    victim = sb;
-                } else {
-                    const uint64_t va = st[sa];
-                    if (va != ST_INF && st_parent(va) == b) victim = sa;
-                }
-                if (victim != 0xFFFFFFFFu) {
-                    st[victim] = ST_INF;  // only ST_INF is written by this kernel
-                    mark(out_bm, victim);
-                    mark(inv_bm, victim);
-                    hit = true;
-                }
+        const uint32_t a = (uint32_t)r.x, b = (uint32_t)r.y;
+        if (r.w == 0 && a != b) {
+            const uint32_t sa = (a << kshift) | tree, sb = (b << kshift) | tree;
+            const uint64_t vb = st[sb];
+            uint32_t victim = 0xFFFFFFFFu;
+            if (vb != ST_INF && st_parent(vb) == a) {
+                victim = sb;
+            } else {
+                const uint64_t va = st[sa];
+                if (va != ST_INF && st_parent(va) == b) victim = sa;
             }
-            if (tree == 0) {
-                probes += tomb_row(g, a, b, (uint32_t)r.z);
-                if (a != b) probes += tomb_row(g, b, a, (uint32_t)r.z);
+            if (victim != 0xFFFFFFFFu) {
+                st[victim] = ST_INF;  // only ST_INF is written by this kernel
+                mark(out_bm, victim);
+                mark(inv_bm, victim);
+                hit = true;
             }
         }
     }
-    const bool any = __ballot(hit) != 0;
-    const uint32_t p = wave_sum(probes);
-    if (lane_id() == 0) {
-        if (any) *out_act = 1u;
-        add_stat(bstats, ST_PROBES, p);
-    }
+    if (__ballot(hit) != 0 && lane_id() == 0) *out_act = 1u;
 }
 
 // Offers every inserted edge that survived the batch's deletions in both
@@ -753,7 +873,8 @@ void launch_frontier_ept(int mode, const FArgs &a, int grid, hipStream_t s)
     case M_SEED: hipLaunchKernelGGL((k_frontier<M_SEED, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
     case M_TBFS: hipLaunchKernelGGL((k_frontier<M_TBFS, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
     case M_PICK: hipLaunchKernelGGL((k_frontier<M_PICK, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_frontier<M_OFCHECK, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
+    case M_OFCHECK: hipLaunchKernelGGL((k_frontier<M_OFCHECK, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_frontier<M_TOMB, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
     }
 }
 
@@ -796,11 +917,11 @@ void launch_ins_count(const int4 *rec, int64_t k, uint32_t *tcnt, uint64_t *slot
     if (k > 0) hipLaunchKernelGGL(k_ins_count, dim3(blocks_for(k)), dim3(RB), 0, s, rec, k, tcnt, slot, tbm);
 }
 
-void launch_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const uint64_t *dbl, const uint32_t *tcnt,
-                        uint32_t *wsum, uint64_t *bsum, hipStream_t s)
+void launch_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const uint64_t *dbl, const uint32_t *dcap,
+                        const uint32_t *tcnt, uint32_t *wsum, uint64_t *bsum, hipStream_t s)
 {
     if (nwords > 0)
-        hipLaunchKernelGGL(k_ins_wordsum, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, dbl, tcnt, wsum,
+        hipLaunchKernelGGL(k_ins_wordsum, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, dbl, dcap, tcnt, wsum,
                            bsum);
 }
 
@@ -811,11 +932,12 @@ void launch_ins_scan(const uint64_t *bsum, uint32_t nwords, uint64_t *bbase, uin
 }
 
 void launch_ins_alloc(const uint64_t *tbm, uint32_t nwords, const uint32_t *wsum, const uint64_t *bbase,
-                      uint64_t *dbl, uint32_t *tcnt, uint64_t *pool, const GraphCtl *gc, hipStream_t s)
+                      uint64_t *dbl, uint32_t *dcap, uint32_t *tcnt, uint64_t *pool, const GraphCtl *gc,
+                      hipStream_t s)
 {
     if (nwords > 0)
-        hipLaunchKernelGGL(k_ins_alloc, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, wsum, bbase, dbl, tcnt,
-                           pool, gc);
+        hipLaunchKernelGGL(k_ins_alloc, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, wsum, bbase, dbl, dcap,
+                           tcnt, pool, gc);
 }
 
 void launch_ins_scatter(const int4 *rec, int64_t k, const uint64_t *slot, const uint64_t *dbl, const uint32_t *tcnt,
@@ -830,12 +952,22 @@ void launch_ins_reset(uint64_t *tbm, uint32_t nwords, uint32_t *tcnt, hipStream_
     if (nwords > 0) hipLaunchKernelGGL(k_ins_reset, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, tcnt);
 }
 
-void launch_classify(const int4 *rec, int64_t k, uint32_t kshift, GView g, uint64_t *st, uint64_t *out_bm,
-                     uint64_t *inv_bm, uint32_t *out_act, unsigned long long *bstats, hipStream_t s)
+void launch_classify(const int4 *rec, int64_t k, uint32_t kshift, uint64_t *st, uint64_t *out_bm, uint64_t *inv_bm,
+                     uint32_t *out_act, hipStream_t s)
 {
     if (k > 0)
-        hipLaunchKernelGGL(k_classify, dim3(blocks_for(k << kshift)), dim3(RB), 0, s, rec, k, kshift, g, st, out_bm,
-                           inv_bm, out_act, bstats);
+        hipLaunchKernelGGL(k_classify, dim3(blocks_for(k << kshift)), dim3(RB), 0, s, rec, k, kshift, st, out_bm,
+                           inv_bm, out_act);
+}
+
+void launch_del_set(const int4 *rec, int64_t k, uint64_t *dset, uint32_t dmask, hipStream_t s)
+{
+    if (k > 0) hipLaunchKernelGGL(k_del_set, dim3(blocks_for(k)), dim3(RB), 0, s, rec, k, dset, dmask);
+}
+
+void launch_tomb(const int4 *rec, int64_t k, GView g, uint64_t *dbm, unsigned long long *bstats, hipStream_t s)
+{
+    if (k > 0) hipLaunchKernelGGL(k_tomb, dim3(blocks_for(2 * k)), dim3(RB), 0, s, rec, k, g, dbm, bstats);
 }
 
 void launch_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint64_t *pos, const uint64_t *pool,

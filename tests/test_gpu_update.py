@@ -262,6 +262,22 @@ def test_update_after_overflow_error():
     assert list(d[:3]) == [0, 2_000_000_000, 4_000_000_000] and d[3] == ds.INF
 
 
+def test_long_rows_tombstones():
+    """Deletions in rows longer than 64 entries (base and delta), including a key
+    with 150 parallel copies that spans several 64-entry scans."""
+    rng = np.random.default_rng(11)
+    n = 1000
+    a = [0] * (n - 1) + [0] * 150 + list(rng.integers(1, n, 3000))
+    b = list(range(1, n)) + [5] * 150 + list(rng.integers(1, n, 3000))
+    w = list(rng.integers(1, 50, n - 1)) + [7] * 150 + list(rng.integers(1, 50, 3000))
+    ins = [[0, int(x), int(rng.integers(1, 50)), 1] for x in rng.integers(1, n, 300)]
+    ins += [[0, 9, 3, 1]] * 70 + [[9, 0, 3, 1]] * 10  # 80 parallel copies in the delta rows of 0 and 9
+    dels = [[0, 5, 7, 0], [5, 0, 7, 0], [0, 9, 3, 0]]
+    dels += [[0, int(x), int(y), 0] for x, y in zip(range(1, n, 7), np.asarray(w[: n - 1])[::7])]
+    dels += [[int(p), int(q), int(r), 0] for p, q, r, _ in ins[:120:3]]
+    run_sequence(n, a, b, w, [np.array(ins, np.int32), np.array(dels, np.int32)], src=0, what="long-rows")
+
+
 @pytest.mark.parametrize("ept", [1, 2, 4])
 def test_heavy_rows_and_chunks(ept):
     """Tiny heavy threshold so that most rows go through the chunk path; every
