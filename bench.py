@@ -60,6 +60,8 @@ def parse():
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU Dijkstra baseline / exactness check")
     p.add_argument("--no-timing", action="store_true", help="no per-launch events (roofline unmeasured)")
+    p.add_argument("--copy-probe", action="store_true",
+                   help="only copy 1 GiB device-to-device five times (calibrates HBM byte counters), then exit")
     for knob in ("grid", "heavy_degree", "chunk", "round_batch", "use_graph", "edges_per_thread"):
         p.add_argument("--" + knob.replace("_", "-"), type=int, default=None, help="tuning knob (dsssp_tuning)")
     return p.parse_args()
@@ -416,6 +418,16 @@ def main():
         sys.exit(2)
     import numpy as np
 
+    if args.copy_probe:
+        import torch
+        x = torch.empty(1 << 27, dtype=torch.int64, device="cuda")
+        y = torch.empty_like(x)
+        x.fill_(1)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log("copied 1 GiB five times")
+        return
     import dynamicssspv2_amd as ds
     ds.set_device(0)
     knobs = {k: getattr(args, k) for k in ("grid", "heavy_degree", "chunk", "round_batch", "use_graph",

@@ -41,7 +41,7 @@ using namespace dsssp;
 
 namespace {
 
-dsssp_tuning g_tuning = {FRONTIER_BLOCK, 2048, 256, 256, 4, 0, 0, 4};
+dsssp_tuning g_tuning = {FRONTIER_BLOCK, 2048, 256, 256, 4, 0, 0, 4, 1024};
 std::mutex g_tuning_mu;
 thread_local int t_device = 0;
 
@@ -118,9 +118,9 @@ struct dsssp_graph {
     uint32_t *dcap = nullptr;  // capacity of each delta block
     uint32_t *tcnt = nullptr;
     uint64_t *tbm = nullptr;  // vertices touched by the batch's insertions
-    uint64_t *dbm = nullptr;  // vertices whose delta block may hold a deleted key
-    uint64_t *dset = nullptr; // deletion set of the batch (2 words a slot)
-    uint32_t dset_slots = 0;
+    uint64_t *emap = nullptr;  // edge map (fingerprint << 32 | position in row)
+    uint64_t emap_slots = 0;
+    uint64_t emap_used = 0;    // entries entered since the map was built (dead ones too)
     uint32_t *wsum = nullptr;   // per bitmap word: pool entries needed by the batch
     uint64_t *bsum = nullptr;   // per 256 words: sum of wsum
     uint64_t *bbase = nullptr;  // per 256 words: pool offset
@@ -136,7 +136,7 @@ struct dsssp_graph {
     bool destroy_pending = false;
     std::mutex mu;
 
-    GView view() const { return GView{rp, adj, dbl, pool}; }
+    GView view() const { return GView{rp, adj, dbl, pool, emap, emap_slots - 1}; }
 };
 
 struct dsssp_tree {
@@ -200,6 +200,7 @@ FArgs base_args(dsssp_tree *t)
     a.chunk = (uint32_t)tu.chunk;
     a.kshift = t->kshift;
     a.ept = (uint32_t)tu.edges_per_thread;
+    a.budget = (uint32_t)tu.tile_entries;
     return a;
 }
 
@@ -210,6 +211,7 @@ FArgs round_args(dsssp_tree *t, const FArgs &base, int64_t i)
     FArgs a = base;
     const int fi = (int)(i & 1), ci = (int)(i % 3);
     a.in_bm = t->fb[fi];
+    a.in_act = &t->c->act[ci];
     a.in_hq = t->hq[fi];
     a.in_hcnt = &t->c->hcnt[ci];
     a.out_bm = t->fb[fi ^ 1];
@@ -371,23 +373,27 @@ int repick_parents(dsssp_tree *t, bool use_lvl, int64_t &rounds)
     return DSSSP_OK;
 }
 
+// Item lists of the frontier kernel.  A launch leaves ceil(deg / chunk) items
+// per deferred slot (every slot is in a launch's input at most once): the slots
+// with deg > heavy entries (at most entries / heavy of them per tree) and, per
+// tile that passes its budget of entries, at most 63 others (so at most
+// 64 * entries / budget per tree, and at most n).
 int ensure_hq(dsssp_tree *t, uint64_t total_entries)
 {
     const dsssp_tuning tu = tuning();
-    // A row longer than `heavy` gives ceil(deg/chunk) <= deg/chunk + 1 chunks and
-    // at most entries/heavy rows are that long; a slot is processed at most
-    // once per round.
+    const uint64_t n = (uint64_t)t->g->n;
     const uint64_t per_tree = total_entries / (uint64_t)std::max(64, tu.chunk) +
-                              total_entries / (uint64_t)std::max(64, tu.heavy_degree) + 1;
+                              total_entries / (uint64_t)std::max(64, tu.heavy_degree) +
+                              std::min<uint64_t>(n, 64 * total_entries / (uint64_t)std::max(64, tu.tile_entries)) + 1;
     const uint64_t need = (per_tree << t->kshift) + 1024;
-    if (need > 0xFFFFFFFFull) return fail(DSSSP_ERR_ARG, "heavy chunk list would exceed 2^32 entries; raise chunk");
+    if (need > 0xFFFFFFFFull) return fail(DSSSP_ERR_ARG, "item list would exceed 2^32 entries; raise chunk");
     if (t->hcap >= need) return DSSSP_OK;
     dev_free(t->hq[0]);
     dev_free(t->hq[1]);
     t->hcap = 0;
     const uint64_t cap = std::min<uint64_t>(need + need / 4, 0xFFFFFFFFull);
-    if (int s = dev_alloc((void **)&t->hq[0], sizeof(uint64_t) * cap, "heavy chunk list")) return s;
-    if (int s = dev_alloc((void **)&t->hq[1], sizeof(uint64_t) * cap, "heavy chunk list")) return s;
+    if (int s = dev_alloc((void **)&t->hq[0], 16 * cap, "item list")) return s;
+    if (int s = dev_alloc((void **)&t->hq[1], 16 * cap, "item list")) return s;
     t->hcap = (uint32_t)cap;
     return DSSSP_OK;
 }
@@ -413,7 +419,7 @@ int ensure_pool(dsssp_graph *g, uint64_t n_ins)
     HIP_TRY(hipStreamSynchronize(g->stream));
     uint64_t live = 0;
     for (int64_t v = 0; v < g->n; ++v) live += dcap[v];
-    uint64_t cap = std::max<uint64_t>(g->pool_cap, 2 * (3 * live + add) + (1u << 24));
+    uint64_t cap = std::max<uint64_t>(g->pool_cap, 2 * (3 * live + add) + (1u << 16));
     if (cap > 0xFFFFFFFFull) cap = 0xFFFFFFFFull;
     if (3 * live + add > cap) return fail(DSSSP_ERR_NOMEM, "delta pool would exceed 2^32 entries");
     out.reserve((size_t)live);
@@ -456,7 +462,7 @@ int finish_errors(dsssp_tree *t, uint32_t gerr)
     const uint32_t err = t->c_host->err | gerr;
     if (err & ERR_DIST_OVERFLOW) return fail(DSSSP_ERR_OVERFLOW, "a distance reached 2^32-1; the tree is not exact");
     if (err & ERR_POOL) return fail(DSSSP_ERR_STATE, "delta pool overflow (internal)");
-    if (err & ERR_QUEUE) return fail(DSSSP_ERR_STATE, "heavy chunk list overflow (internal)");
+    if (err & ERR_QUEUE) return fail(DSSSP_ERR_STATE, "item list overflow (internal)");
     if (err & ERR_NO_TIGHT) return fail(DSSSP_ERR_TREE, "a reachable vertex has no tight in-edge: distances not exact");
     return DSSSP_OK;
 }
@@ -490,43 +496,53 @@ void scrub(dsssp_tree *t)
     launch_fill_u64(t->fb[1], 0ull, t->nw, s);
     launch_fill_u64(t->ib, 0ull, t->nw, s);
     launch_fill_u64(t->g->tbm, 0ull, t->g->nw, s);
-    launch_fill_u64(t->g->dbm, 0ull, t->g->nw, s);
     launch_fill_u32(t->g->tcnt, 0u, (uint64_t)t->g->n, s);
     zero_ring(t);
     (void)hipStreamSynchronize(s);
 }
 
-// Deletion set of at least 4 slots per deletion record (load factor <= 1/4).
-int ensure_dset(dsssp_graph *g, int64_t n_del)
+// (Re)builds the edge map for `entries` directed entries plus headroom for
+// `extra` more: every live entry of every row is entered by one identity pass
+// of the frontier kernel (rows scanned whole by their wave, no chunk lists).
+int build_emap(dsssp_graph *g, uint64_t entries, uint64_t extra)
 {
-    uint64_t want = 1024;
-    while (want < 4 * (uint64_t)n_del) want <<= 1;
-    if (want > (1ull << 31)) return fail(DSSSP_ERR_ARG, "too many deletions in one batch");
-    if (g->dset_slots >= want) return DSSSP_OK;
-    dev_free(g->dset);
-    g->dset_slots = 0;
-    if (int s = dev_alloc((void **)&g->dset, 16 * want, "deletion set")) return s;
-    g->dset_slots = (uint32_t)want;
+    const uint64_t slots = emap_slots_for(entries + extra);
+    if (slots != g->emap_slots) {
+        dev_free(g->emap);
+        g->emap_slots = 0;
+        if (int s = dev_alloc((void **)&g->emap, 8 * slots, "edge map")) return s;
+        g->emap_slots = slots;
+    }
+    launch_fill_u64(g->emap, ~0ull, slots, g->stream);
+    FArgs a;
+    memset(&a, 0, sizeof a);
+    a.g = g->view();
+    a.nslots = (uint32_t)g->n;
+    a.nwords = g->nw;
+    a.identity = 1;
+    a.tiles = 1;
+    a.heavy = 0xFFFFFFFFu;
+    a.budget = 0xFFFFFFFFu;
+    a.chunk = 64;
+    a.ept = 4;
+    launch_frontier(M_EMAP, a, std::max(1, tuning().grid), g->stream);
+    HIP_TRY(hipGetLastError());
+    g->emap_used = entries;
     return DSSSP_OK;
 }
 
-// Tombstones every copy of every deleted (a, b, w): base rows by binary search
-// per record copy, delta blocks by one pass over the marked rows against the
-// batch's deletion set (long delta blocks in chunks over many waves).
+// Keeps the edge map's load factor at most 0.7 with `add` more entries.
+int ensure_emap(dsssp_graph *g, uint64_t add)
+{
+    if ((g->emap_used + add) * 10 <= g->emap_slots * 7) return DSSSP_OK;
+    const uint64_t entries = g->E + g->delta_entries;  // live entries are at most these
+    return build_emap(g, entries, entries / 2 + add);
+}
+
+// Tombstones every copy of every deleted (a, b, w) in both rows.
 void tombstone(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b)
 {
-    hipStream_t s = g->stream;
-    launch_fill_u64(g->dset, ~0ull, 2 * (uint64_t)g->dset_slots, s);
-    launch_del_set(b->rec, b->k, g->dset, g->dset_slots - 1, s);
-    launch_tomb(b->rec, b->k, g->view(), g->dbm, t->bstats, s);
-    FArgs a = base_args(t);
-    a.kshift = 0;
-    a.nslots = (uint32_t)g->n;
-    a.nwords = g->nw;
-    a.in_bm = g->dbm;  // consumed
-    a.dset = g->dset;
-    a.dmask = g->dset_slots - 1;
-    single_pass(t, M_TOMB, a);
+    launch_tomb(b->rec, b->k, g->view(), t->bstats, g->stream);
 }
 
 int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stats, double h2d_ms)
@@ -536,8 +552,8 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
     if (b->n_ins > 0) {
         if (int e = ensure_pool(g, (uint64_t)b->n_ins)) return e;
     }
-    if (b->n_del > 0) {
-        if (int e = ensure_dset(g, b->n_del)) return e;
+    if (b->n_ins > 0) {
+        if (int e = ensure_emap(g, 2 * (uint64_t)b->n_ins)) return e;
     }
     if (int e = ensure_hq(t, g->E + g->delta_entries + 2 * (uint64_t)b->n_ins)) return e;
     int64_t inv_rounds = 0, relax_rounds = 0, repair_rounds = 0;
@@ -552,7 +568,8 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
         launch_ins_wordsum(g->tbm, g->nw, g->dbl, g->dcap, g->tcnt, g->wsum, g->bsum, s);
         launch_ins_scan(g->bsum, g->nw, g->bbase, g->pool_cap, g->gc, s);
         launch_ins_alloc(g->tbm, g->nw, g->wsum, g->bbase, g->dbl, g->dcap, g->tcnt, g->pool, g->gc, s);
-        launch_ins_scatter(b->rec, b->k, b->slot, g->dbl, g->tcnt, g->pool, b->pos, g->gc, s);
+        launch_ins_scatter(b->rec, b->k, b->slot, g->view(), g->tcnt, b->pos, g->gc, s);
+        g->emap_used += 2 * (uint64_t)b->n_ins;
         launch_ins_reset(g->tbm, g->nw, g->tcnt, s);
         if (b->zero_w) g->zero_w = true;
     }
@@ -701,10 +718,11 @@ extern "C" int dsssp_set_tuning(const dsssp_tuning *t)
     if (t->chunk) nt.chunk = t->chunk;
     if (t->round_batch) nt.round_batch = t->round_batch;
     if (t->edges_per_thread) nt.edges_per_thread = t->edges_per_thread;
+    if (t->tile_entries) nt.tile_entries = t->tile_entries;
     nt.kernel_timing = t->kernel_timing;
     nt.use_graph = t->use_graph;
     if (nt.grid < 1 || nt.grid > 65536 || nt.heavy_degree < 64 || nt.chunk < 64 || nt.chunk % 64 ||
-        nt.round_batch < 1 || nt.round_batch > 64 ||
+        nt.round_batch < 1 || nt.round_batch > 64 || nt.tile_entries < 64 ||
         (nt.edges_per_thread != 1 && nt.edges_per_thread != 2 && nt.edges_per_thread != 4))
         return fail(DSSSP_ERR_ARG, "tuning out of range");
     g_tuning = nt;
@@ -741,8 +759,7 @@ static void graph_release(dsssp_graph *g)
     dev_free(g->tcnt);
     dev_free(g->dcap);
     dev_free(g->tbm);
-    dev_free(g->dbm);
-    dev_free(g->dset);
+    dev_free(g->emap);
     dev_free(g->wsum);
     dev_free(g->bsum);
     dev_free(g->bbase);
@@ -768,7 +785,6 @@ static int graph_alloc(dsssp_graph *g, uint64_t pool_cap)
     if (int s = dev_alloc((void **)&g->tcnt, sizeof(uint32_t) * n, "insert counters")) return s;
     if (int s = dev_alloc((void **)&g->dcap, sizeof(uint32_t) * n, "delta capacities")) return s;
     if (int s = dev_alloc((void **)&g->tbm, sizeof(uint64_t) * g->nw, "touched bitmap")) return s;
-    if (int s = dev_alloc((void **)&g->dbm, sizeof(uint64_t) * g->nw, "deletion rows bitmap")) return s;
     if (int s = dev_alloc((void **)&g->wsum, sizeof(uint32_t) * g->nw, "word sizes")) return s;
     const size_t nb = (g->nw + 255) / 256;
     if (int s = dev_alloc((void **)&g->bsum, sizeof(uint64_t) * nb, "group sizes")) return s;
@@ -782,7 +798,6 @@ static int graph_alloc(dsssp_graph *g, uint64_t pool_cap)
     HIP_TRY(hipMemset(g->tcnt, 0, sizeof(uint32_t) * n));
     HIP_TRY(hipMemset(g->dcap, 0, sizeof(uint32_t) * n));
     HIP_TRY(hipMemset(g->tbm, 0, sizeof(uint64_t) * g->nw));
-    HIP_TRY(hipMemset(g->dbm, 0, sizeof(uint64_t) * g->nw));
     HIP_TRY(hipMemset(g->gc, 0, sizeof(GraphCtl)));
     return DSSSP_OK;
 }
@@ -803,7 +818,7 @@ extern "C" int dsssp_graph_create(int64_t n, int64_t m, const int32_t *a, const 
     g->n = n;
     g->E = adj.size();
     for (int64_t i = 0; i < m && !g->zero_w; ++i) g->zero_w = w[i] == 0;
-    const uint64_t pool_cap = std::min<uint64_t>(g->E / 2 + (1u << 20), 0xFFFFFFFFull / 2);
+    const uint64_t pool_cap = std::min<uint64_t>(g->E / 2 + (1u << 16), 0xFFFFFFFFull / 2);
     if (int s = graph_alloc(g, pool_cap)) {
         graph_release(g);
         return s;
@@ -814,6 +829,14 @@ extern "C" int dsssp_graph_create(int64_t n, int64_t m, const int32_t *a, const 
         const int s = fail(DSSSP_ERR_HIP, "graph upload failed: %s", hipGetErrorString(hipGetLastError()));
         graph_release(g);
         return s;
+    }
+    int s = build_emap(g, g->E, g->E / 8);
+    if (!s && hipStreamSynchronize(g->stream) != hipSuccess)
+        s = fail(DSSSP_ERR_HIP, "edge map build failed: %s", hipGetErrorString(hipGetLastError()));
+    if (s) {
+        const std::string keep = last_error();
+        graph_release(g);
+        return fail(s, "%s", keep.c_str());
     }
     *out = g;
     return DSSSP_OK;
@@ -846,7 +869,7 @@ extern "C" int dsssp_graph_info(const dsssp_graph *g, int64_t *n, int64_t *m_liv
     }
     if (device_bytes)
         *device_bytes = 8 * ((uint64_t)g->n + 1) + 8 * g->E + 16 * (uint64_t)g->n + 20 * (uint64_t)g->nw +
-                        16 * (uint64_t)((g->nw + 255) / 256) + 8 * g->pool_cap + 16 * (uint64_t)g->dset_slots;
+                        16 * (uint64_t)((g->nw + 255) / 256) + 8 * g->pool_cap + 8 * g->emap_slots;
     return DSSSP_OK;
 }
 
@@ -929,12 +952,19 @@ extern "C" int dsssp_graph_copy(dsssp_graph *dst, dsssp_graph *src)
     if (src->pool_top)
         HIP_TRY(hipMemcpyAsync(dst->pool, src->pool, sizeof(uint64_t) * src->pool_top, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemcpyAsync(dst->gc, src->gc, sizeof(GraphCtl), hipMemcpyDeviceToDevice, s));
+    if (dst->emap_slots != src->emap_slots) {
+        dev_free(dst->emap);
+        dst->emap_slots = 0;
+        if (int e = dev_alloc((void **)&dst->emap, 8 * src->emap_slots, "edge map")) return e;
+        dst->emap_slots = src->emap_slots;
+    }
+    HIP_TRY(hipMemcpyAsync(dst->emap, src->emap, 8 * src->emap_slots, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemsetAsync(dst->tcnt, 0, sizeof(uint32_t) * n, s));
     HIP_TRY(hipMemsetAsync(dst->tbm, 0, sizeof(uint64_t) * dst->nw, s));
-    HIP_TRY(hipMemsetAsync(dst->dbm, 0, sizeof(uint64_t) * dst->nw, s));
     HIP_TRY(hipStreamSynchronize(s));
     dst->pool_top = src->pool_top;
     dst->delta_entries = src->delta_entries;
+    dst->emap_used = src->emap_used;
     dst->zero_w = src->zero_w;
     return DSSSP_OK;
 }

@@ -59,6 +59,14 @@ void parallel_chunks(int64_t n, int64_t chunk, F f, int threads = 0)
 }
 
 // splitmix64 finaliser: a counter-based generator, identical on every machine.
+// Slots of the edge map for `entries` directed entries: load factor <= 0.7.
+inline uint64_t emap_slots_for(uint64_t entries)
+{
+    uint64_t s = 1024;
+    while (s * 7 < entries * 10) s <<= 1;
+    return s;
+}
+
 inline uint64_t mix64(uint64_t x)
 {
     x += 0x9E3779B97F4A7C15ull;

@@ -12,6 +12,8 @@ struct GView {
     uint64_t *adj;        // base entries
     const uint64_t *dbl;  // delta descriptors
     uint64_t *pool;       // delta entries
+    uint64_t *emap;       // edge map: slots of fingerprint << 32 | position in row
+    uint64_t emask;       // edge map slots - 1
 };
 
 // Frontier kernel modes.
@@ -22,7 +24,7 @@ enum Mode : int {
     M_TBFS = 3,   // BFS levels over tight edges (zero-weight parent repair)
     M_PICK = 4,   // smallest-id tight neighbour (optionally one level up) as parent
     M_OFCHECK = 5,  // a reachable slot with a live edge to an unreachable one: overflow
-    M_TOMB = 6,     // delta rows only: tombstone entries whose key is in the deletion set
+    M_EMAP = 6,     // enter every live entry into the edge map (identity pass)
 };
 
 // Arguments of one frontier launch.  The input frontier is a bitmap over
@@ -34,6 +36,7 @@ struct FArgs {
     uint32_t *lvl;
     uint32_t *best;
     uint64_t *in_bm;        // input frontier bitmap
+    const uint32_t *in_act; // set when the previous launch marked in_bm (null: always scan)
     const uint64_t *in_hq;  // heavy chunks to process
     const uint32_t *in_hcnt;
     uint64_t *out_bm;       // marks of this launch
@@ -56,8 +59,7 @@ struct FArgs {
     uint32_t use_lvl;       // M_PICK: require lvl[parent] + 1 == lvl[v]
     uint32_t kshift;        // log2 of the trees per vertex: slot = vertex << kshift | tree
     uint32_t ept;           // edges in flight per lane: 1, 2 or 4
-    const uint64_t *dset;   // M_TOMB: deletion set (open addressing, 2 words a slot)
-    uint32_t dmask;         // M_TOMB: slots - 1
+    uint32_t budget;        // entries a fresh tile takes; later slots become items
 };
 
 constexpr int FRONTIER_BLOCK = 256;
@@ -78,20 +80,16 @@ void launch_ins_scan(const uint64_t *bsum, uint32_t nwords, uint64_t *bbase, uin
 void launch_ins_alloc(const uint64_t *tbm, uint32_t nwords, const uint32_t *wsum, const uint64_t *bbase,
                       uint64_t *dbl, uint32_t *dcap, uint32_t *tcnt, uint64_t *pool, const GraphCtl *gc,
                       hipStream_t s);
-void launch_ins_scatter(const int4 *rec, int64_t k, const uint64_t *slot, const uint64_t *dbl, const uint32_t *tcnt,
-                        uint64_t *pool, uint64_t *pos, const GraphCtl *gc, hipStream_t s);
+void launch_ins_scatter(const int4 *rec, int64_t k, const uint64_t *slot, GView g, const uint32_t *tcnt,
+                        uint64_t *pos, const GraphCtl *gc, hipStream_t s);
 void launch_ins_reset(uint64_t *tbm, uint32_t nwords, uint32_t *tcnt, hipStream_t s);
 // deletions, offers, roots, parent repair
 // Deletions: the tree-edge test (marks the roots of invalidated subtrees) and
 // the tombstoning of every copy of each deleted entry.
 void launch_classify(const int4 *rec, int64_t k, uint32_t kshift, uint64_t *st, uint64_t *out_bm, uint64_t *inv_bm,
                      uint32_t *out_act, hipStream_t s);
-// The batch's deleted keys (min(a,b), max(a,b), w) go into an open-addressing
-// set of dmask+1 slots (2 words each, cleared to ~0 beforehand); base rows are
-// searched per record copy, and rows whose delta block is non-empty are marked
-// in dbm for an M_TOMB pass.
-void launch_del_set(const int4 *rec, int64_t k, uint64_t *dset, uint32_t dmask, hipStream_t s);
-void launch_tomb(const int4 *rec, int64_t k, GView g, uint64_t *dbm, unsigned long long *bstats, hipStream_t s);
+// Tombstones every copy of each deleted entry, found through the edge map.
+void launch_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats, hipStream_t s);
 void launch_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint64_t *pos, const uint64_t *pool,
                       uint64_t *st, uint64_t *out_bm, uint32_t *out_act, Counters *c, hipStream_t s);
 // Roots: for j < count, vertex src[j] becomes a root (distance 0, parent itself)

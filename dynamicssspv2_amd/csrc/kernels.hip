@@ -59,23 +59,55 @@ __device__ __forceinline__ void add_stat(unsigned long long *bstats, int which, 
     if (v) atomicAdd(&bstats[(blockIdx.x % STAT_SLOTS) * NSTAT + which], (unsigned long long)v);
 }
 
-// Deletion set: slot i holds key words dset[2i] = min(a,b) << 32 | max(a,b) (~0:
-// empty) and dset[2i+1] = weight.
-__device__ __forceinline__ uint32_t dset_hash(uint64_t k1, uint32_t w)
+// Edge map.  Every live directed entry (row, col, w) has a slot holding a
+// 32-bit fingerprint of the key and the entry's position in its row (base
+// entries first, then the delta block; positions never change, as blocks move
+// in order).  A lookup verifies the entry itself, so it is exact; ~0 = empty.
+__device__ __forceinline__ uint64_t emap_hash(uint32_t row, uint32_t col, uint32_t w)
 {
-    uint64_t h = k1 * 0x9E3779B97F4A7C15ull ^ ((uint64_t)w * 0xC2B2AE3D27D4EB4Full);
-    h ^= h >> 29;
-    return (uint32_t)(h ^ (h >> 32));
+    uint64_t h = ((uint64_t)row << 32 | col) * 0x9E3779B97F4A7C15ull;
+    h ^= (uint64_t)w * 0xC2B2AE3D27D4EB4Full;
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ull;
+    return h ^ (h >> 29);
 }
 
-__device__ __forceinline__ bool dset_has(const uint64_t *dset, uint32_t dmask, uint32_t a, uint32_t b, uint32_t w)
+__device__ __forceinline__ uint32_t emap_fp(uint64_t h)
 {
-    const uint64_t k1 = a < b ? ((uint64_t)a << 32 | b) : ((uint64_t)b << 32 | a);
-    for (uint32_t i = dset_hash(k1, w) & dmask;; i = (i + 1) & dmask) {
-        const uint64_t k = dset[2 * i];
-        if (k == ~0ull) return false;
-        if (k == k1 && (uint32_t)dset[2 * i + 1] == w) return true;
+    const uint32_t f = (uint32_t)(h >> 32);
+    return f == 0xFFFFFFFFu ? 0xFFFFFFFEu : f;
+}
+
+__device__ __forceinline__ void emap_insert(const GView &g, uint32_t row, uint32_t col, uint32_t w, uint32_t pos)
+{
+    const uint64_t h = emap_hash(row, col, w);
+    const uint64_t v = (uint64_t)emap_fp(h) << 32 | pos;
+    for (uint64_t s = h & g.emask;; s = (s + 1) & g.emask)
+        if (atomicCAS((unsigned long long *)&g.emap[s], ~0ull, (unsigned long long)v) == ~0ull) return;
+}
+
+// Tombstones every live copy of (row, col, w); returns slots read.
+__device__ __forceinline__ uint32_t emap_kill(const GView &g, uint32_t row, uint32_t col, uint32_t w)
+{
+    const uint64_t h = emap_hash(row, col, w);
+    const uint32_t fp = emap_fp(h);
+    const uint64_t key = pack_entry(col, w);
+    const uint64_t b0 = g.rp[row];
+    const uint32_t blen = (uint32_t)(g.rp[row + 1] - b0);
+    const uint64_t d = g.dbl[row];
+    const uint32_t dl = (uint32_t)d;
+    uint32_t probes = 0;
+    for (uint64_t s = h & g.emask;; s = (s + 1) & g.emask) {
+        const uint64_t v = g.emap[s];
+        ++probes;
+        if (v == ~0ull) break;
+        const uint32_t pos = (uint32_t)v;
+        if ((uint32_t)(v >> 32) != fp || pos >= blen + dl) continue;
+        uint64_t *ep = pos < blen ? &g.adj[b0 + pos] : &g.pool[(d >> 32) + (pos - blen)];
+        const uint64_t e = *ep;
+        if (e_key(e) == key && !(e & E_DEAD)) *ep = e | E_DEAD;
     }
+    return probes;
 }
 
 struct WaveStats {
@@ -232,43 +264,87 @@ __device__ __forceinline__ bool owner_prep(const FArgs &A, uint32_t os, uint32_t
     }
 }
 
-// Queues the chunks of a row longer than A.heavy for the next launch.
-__device__ __forceinline__ void emit_chunks(const FArgs &A, uint32_t os, uint32_t deg)
+// Work of a launch: the set slots of its input bitmap and the items left by the
+// previous launch.  Set slots are compacted across bitmap words into tiles of up
+// to 64 slots, one per lane; a tile's rows are concatenated into one range of
+// entries that the lanes share.  A slot whose row has more than A.heavy entries,
+// or whose row would end past the tile's first A.budget entries, stays out of
+// the tile: its row becomes items of A.chunk entries for the next launch, where
+// any wave takes one.  So every wave's share of a launch is bounded.  An item is
+// two words: a slot mask within one bitmap word, and word index << 32 | first
+// entry.
+
+// Position of the k-th set bit of w (k < popcount(w)).
+__device__ __forceinline__ uint32_t select_bit(uint64_t w, uint32_t k)
 {
-    const uint32_t nch = (deg + A.chunk - 1) / A.chunk;
-    const uint32_t base = atomicAdd(A.out_hcnt, nch);
-    if (base + nch <= A.hcap) {
-        for (uint32_t c = 0; c < nch; ++c) A.out_hq[base + c] = ((uint64_t)os << 32) | c;
-    } else {
-        atomicOr(&A.c->err, ERR_QUEUE);
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t half = 32; half > 0; half >>= 1) {
+        const uint32_t c = (uint32_t)__popcll(w & ((1ull << half) - 1ull));
+        if (k >= c) {
+            k -= c;
+            w >>= half;
+            pos += half;
+        }
     }
+    return pos;
 }
 
-// One tile: the set bits of bitmap word `word`, slots base .. base+63.
+// Entries [start, ...) of the rows of slots os (one per lane, where valid).  A
+// fresh tile (from the input bitmap) hands its heavy rows to items and covers all
+// its other entries; an item covers A.chunk entries of its range.
 template <int MODE, int EPT>
-__device__ void do_tile(const FArgs &A, uint32_t base, uint64_t word, TileSmem &sm, uint32_t wv, WaveStats &S)
+__device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start, bool fresh, TileSmem &sm,
+                         uint32_t wv, WaveStats &S)
 {
     const uint32_t lane = lane_id();
-    const bool valid = (word >> lane) & 1ull;
-    const uint32_t os = base + lane;
     uint32_t deg = 0, blen = 0, dbeg = 0, aux0 = 0, aux1 = 0;
     uint64_t b0 = 0;
     if (valid) {
         const uint32_t u = os >> A.kshift;
-        if constexpr (MODE != M_TOMB) {  // M_TOMB scans delta rows only
-            b0 = A.g.rp[u];
-            blen = (uint32_t)(A.g.rp[u + 1] - b0);
-        }
+        b0 = A.g.rp[u];
+        blen = (uint32_t)(A.g.rp[u + 1] - b0);
         const uint64_t dd = A.g.dbl[u];
         dbeg = (uint32_t)(dd >> 32);
         deg = blen + (uint32_t)dd;
         if (!owner_prep<MODE>(A, os, aux0, aux1)) deg = 0;
-        if (deg > A.heavy) {
-            emit_chunks(A, os, deg);
-            deg = 0;
-        }
     }
-    S.front += (uint32_t)__popcll(word);
+    if (fresh) {
+        S.front += (uint32_t)__popcll(__ballot(valid));
+        bool defer = deg > A.heavy;
+        {  // rows ending past the budget, the tile's first row excepted
+            uint32_t ib = defer ? 0u : deg;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t y = __shfl_up(ib, d);
+                if ((int)lane >= d) ib += y;
+            }
+            if (!defer && deg && ib > A.budget && ib > deg) defer = true;
+        }
+        const uint32_t nch = defer ? (deg + A.chunk - 1) / A.chunk : 0u;
+        uint32_t ic = nch;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(ic, d);
+            if ((int)lane >= d) ic += y;
+        }
+        const uint32_t tot = __shfl(ic, 63);
+        if (tot) {  // one list reservation per tile
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(A.out_hcnt, tot);
+            base = __shfl(base, 0);
+            if (base + tot > A.hcap) {
+                if (lane == 0) atomicOr(&A.c->err, ERR_QUEUE);
+            } else {
+                const uint64_t it0 = base + ic - nch;
+                for (uint32_t c = 0; c < nch; ++c) {
+                    A.out_hq[2 * (it0 + c)] = 1ull << (os & 63u);
+                    A.out_hq[2 * (it0 + c) + 1] = ((uint64_t)(os >> 6) << 32) | (c * A.chunk);
+                }
+            }
+        }
+        if (defer) deg = 0;
+    }
     uint32_t incl = deg;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -276,7 +352,8 @@ __device__ void do_tile(const FArgs &A, uint32_t base, uint64_t word, TileSmem &
         if ((int)lane >= d) incl += y;
     }
     const uint32_t total = __shfl(incl, 63);
-    if (total == 0) return;
+    const uint32_t end = fresh ? total : min(start + A.chunk, total);
+    if (end <= start) return;
     sm.off[wv][lane] = incl - deg;
     sm.u[wv][lane] = os;
     sm.aux0[wv][lane] = aux0;
@@ -288,16 +365,16 @@ __device__ void do_tile(const FArgs &A, uint32_t base, uint64_t word, TileSmem &
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (uint32_t eb = 0; eb < total; eb += 64 * EPT) {
+    for (uint32_t eb = start; eb < end; eb += 64 * EPT) {
         uint64_t ent[EPT];
-        uint32_t ows[EPT], a0[EPT], a1[EPT], oi[EPT];
+        uint32_t ows[EPT], a0[EPT], a1[EPT], oi[EPT], pos[EPT];
         bool act[EPT];
 #pragma unroll
         for (int j = 0; j < EPT; ++j) {
             const uint32_t e = eb + j * 64 + lane;
-            act[j] = e < total;
+            act[j] = e < end;
             ent[j] = 0;
-            ows[j] = a0[j] = a1[j] = oi[j] = 0;
+            ows[j] = a0[j] = a1[j] = oi[j] = pos[j] = 0;
             if (act[j]) {
                 uint32_t o = 0;
 #pragma unroll
@@ -306,22 +383,17 @@ __device__ void do_tile(const FArgs &A, uint32_t base, uint64_t word, TileSmem &
                 const uint32_t jj = e - sm.off[wv][o];
                 const uint32_t bl = sm.blen[wv][o];
                 oi[j] = o;
+                pos[j] = jj;
                 ows[j] = sm.u[wv][o];
                 a0[j] = sm.aux0[wv][o];
                 a1[j] = sm.aux1[wv][o];
                 ent[j] = jj < bl ? A.g.adj[sm.b0[wv][o] + jj] : A.g.pool[(uint64_t)sm.dbeg[wv][o] + (jj - bl)];
             }
         }
-        if constexpr (MODE == M_TOMB) {
+        if constexpr (MODE == M_EMAP) {
 #pragma unroll
-            for (int j = 0; j < EPT; ++j) {
-                if (!act[j] || (ent[j] & E_DEAD)) continue;
-                if (dset_has(A.dset, A.dmask, ows[j], e_col(ent[j]), e_w(ent[j]))) {
-                    const uint32_t e = eb + j * 64 + lane;
-                    const uint32_t o = oi[j];
-                    A.g.pool[(uint64_t)sm.dbeg[wv][o] + (e - sm.off[wv][o])] = ent[j] | E_DEAD;
-                }
-            }
+            for (int j = 0; j < EPT; ++j)
+                if (act[j] && !(ent[j] & E_DEAD)) emap_insert(A.g, ows[j], e_col(ent[j]), e_w(ent[j]), pos[j]);
             continue;
         }
         Pre pre[EPT];
@@ -336,73 +408,21 @@ __device__ void do_tile(const FArgs &A, uint32_t base, uint64_t word, TileSmem &
             }
         }
     }
-    S.edges += total;
+    S.edges += end - start;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if constexpr (is_pull(MODE)) {
-        // One commit per owner, and the tile's marks as one bitmap-word update
-        // (a tile is the 64 slots of one bitmap word).
-        const bool hit = deg != 0 && pull_commit<MODE>(A, os, sm.pull[wv][lane]);
-        const unsigned long long mask = __ballot(hit);
-        if (mask && lane == 0) atomicOr((unsigned long long *)&A.out_bm[base >> 6], mask);
-        S.any |= hit;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-}
-
-template <int MODE, int EPT>
-__device__ void do_chunk(const FArgs &A, uint32_t item, WaveStats &S)
-{
-    const uint32_t lane = lane_id();
-    const uint64_t hv = A.in_hq[item];
-    const uint32_t os = (uint32_t)(hv >> 32);
-    const uint32_t u = os >> A.kshift;
-    const uint32_t c = (uint32_t)hv;
-    const uint64_t b0 = MODE == M_TOMB ? 0ull : A.g.rp[u];
-    const uint32_t blen = MODE == M_TOMB ? 0u : (uint32_t)(A.g.rp[u + 1] - b0);
-    const uint64_t dd = A.g.dbl[u];
-    const uint32_t dbeg = (uint32_t)(dd >> 32);
-    const uint32_t deg = blen + (uint32_t)dd;
-    uint32_t aux0 = 0, aux1 = 0;
-    if (!owner_prep<MODE>(A, os, aux0, aux1)) return;
-    const uint32_t start = c * A.chunk;
-    const uint32_t end = min(start + A.chunk, deg);
-    uint64_t pull = ~0ull;  // pull modes: this lane's best candidate for os
-    for (uint32_t jb = start; jb < end; jb += 64 * EPT) {
-        uint64_t ent[EPT];
-        bool act[EPT];
-#pragma unroll
-        for (int j = 0; j < EPT; ++j) {
-            const uint32_t jj = jb + j * 64 + lane;
-            act[j] = jj < end;
-            ent[j] = act[j] ? (jj < blen ? A.g.adj[b0 + jj] : A.g.pool[(uint64_t)dbeg + (jj - blen)]) : 0ull;
-        }
-        if constexpr (MODE == M_TOMB) {
-#pragma unroll
-            for (int j = 0; j < EPT; ++j) {
-                if (!act[j] || (ent[j] & E_DEAD)) continue;
-                if (dset_has(A.dset, A.dmask, u, e_col(ent[j]), e_w(ent[j])))
-                    A.g.pool[(uint64_t)dbeg + (jb + j * 64 + lane)] = ent[j] | E_DEAD;
-            }
-            continue;
-        }
-        Pre pre[EPT];
-#pragma unroll
-        for (int j = 0; j < EPT; ++j) pre[j] = act[j] ? gather<MODE>(A, os, ent[j]) : Pre{ST_INF, LVL_INF, true};
-#pragma unroll
-        for (int j = 0; j < EPT; ++j) apply<MODE>(A, os, aux0, aux1, ent[j], pre[j], S, pull);
-    }
-    if (end > start) S.edges += end - start;
-    if constexpr (is_pull(MODE)) {
-#pragma unroll
-        for (int d = 32; d > 0; d >>= 1) pull = min(pull, (uint64_t)__shfl_xor((unsigned long long)pull, d));
-        if (lane == 0 && pull_commit<MODE>(A, os, pull)) {
+        // One commit per owner with entries in this range.
+        const uint32_t lo = sm.off[wv][lane];
+        const bool mine = deg != 0 && lo < end && lo + deg > start;
+        if (mine && pull_commit<MODE>(A, os, sm.pull[wv][lane])) {
             mark(A.out_bm, os);
             S.any = true;
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -418,15 +438,29 @@ __global__ __launch_bounds__(FB) void k_frontier(FArgs A)
     const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t gw = blockIdx.x * FW + wv;
     const uint32_t nw = gridDim.x * FW;
-    // Chunk count written by the previous launch; clamped to the list capacity.
+    // Items left by the previous launch; count clamped to the list capacity.
     const uint32_t H = A.in_hcnt ? min(*A.in_hcnt, A.hcap) : 0u;
+    // A round after convergence has an empty bitmap and no items: it returns
+    // without scanning.
+    if (H == 0 && A.in_act && *A.in_act == 0) return;
     WaveStats S{0, 0, 0, false};
-    for (uint32_t it = gw; it < H; it += nw) do_chunk<MODE, EPT>(A, it, S);
+    for (uint32_t it = gw; it < H; it += nw) {
+        const uint64_t mask = A.in_hq[2 * (uint64_t)it];
+        const uint64_t ws = A.in_hq[2 * (uint64_t)it + 1];
+        const uint32_t widx = (uint32_t)(ws >> 32);
+        do_slots<MODE, EPT>(A, widx * 64 + lane, (mask >> lane) & 1ull, (uint32_t)ws, false, sm, wv, S);
+    }
     if (A.tiles) {
-        for (uint32_t wb = gw * 64; wb < A.nwords; wb += nw * 64) {
-            const uint32_t wi = wb + lane;
+        // The bitmap is read in blocks of 8 words (64 bytes); in one pass a wave
+        // reads 8 blocks nw apart, so that a cluster of non-empty words (frontiers
+        // cluster by vertex id) spreads over many waves.  The set slots of the 64
+        // words are then taken 64 at a time, in word order.
+        const uint32_t nblk = (A.nwords + 7) / 8;
+        for (uint32_t pb = 0; pb < nblk; pb += nw * 8) {
+            const uint32_t blk = pb + (lane >> 3) * nw + gw;
+            const uint32_t wi = blk * 8 + (lane & 7);
             uint64_t word = 0;
-            if (wi < A.nwords) {
+            if (blk < nblk && wi < A.nwords) {
                 if (A.identity) {
                     const uint32_t lo = wi * 64;
                     word = A.nslots - lo >= 64 ? ~0ull : ((1ull << (A.nslots - lo)) - 1ull);
@@ -435,12 +469,29 @@ __global__ __launch_bounds__(FB) void k_frontier(FArgs A)
                     if (word && A.consume) A.in_bm[wi] = 0;
                 }
             }
-            unsigned long long nz = __ballot(word != 0);
-            while (nz) {
-                const uint32_t l = (uint32_t)__builtin_ctzll(nz);
-                nz &= nz - 1;
-                const uint64_t tw = __shfl(word, (int)l);
-                do_tile<MODE, EPT>(A, (wb + l) * 64, tw, sm, wv, S);
+            const uint32_t cnt = (uint32_t)__popcll(word);
+            uint32_t incl = cnt;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t y = __shfl_up(incl, d);
+                if ((int)lane >= d) incl += y;
+            }
+            const uint32_t T = __shfl(incl, 63);
+            for (uint32_t r0 = 0; r0 < T; r0 += 64) {
+                const uint32_t q = r0 + lane;  // rank of this lane's slot
+                uint32_t o = 0;                // owner lane: the number of lanes with incl <= q
+#pragma unroll
+                for (uint32_t st = 32; st > 0; st >>= 1) {
+                    const uint32_t v = (uint32_t)__shfl((int)incl, (int)(o + st - 1));
+                    if (v <= q) o += st;
+                }
+                o = min(o, 63u);
+                const uint64_t ow = __shfl(word, (int)o);
+                const uint32_t oex = (uint32_t)__shfl((int)(incl - cnt), (int)o);
+                const uint32_t owi = (uint32_t)__shfl((int)wi, (int)o);
+                const bool valid = q < T;
+                const uint32_t os = valid ? owi * 64 + select_bit(ow, q - oex) : 0u;
+                do_slots<MODE, EPT>(A, os, valid, 0u, true, sm, wv, S);
             }
         }
     }
@@ -452,8 +503,8 @@ __global__ __launch_bounds__(FB) void k_frontier(FArgs A)
             add_stat(A.bstats, ST_RELAX_FRONT, S.front);
             add_stat(A.bstats, ST_RELAX_EDGES, S.edges);
             add_stat(A.bstats, ST_RELAX_MARKS, marks);
-        } else if constexpr (MODE == M_TOMB) {
-            add_stat(A.bstats, ST_PROBES, S.edges);
+        } else if constexpr (MODE == M_EMAP) {
+            // no statistics: the map is built outside updates
         } else {
             add_stat(A.bstats, ST_OTHER_EDGES, S.edges);
             if constexpr (MODE == M_SEED) add_stat(A.bstats, ST_INVALIDATED, S.front);
@@ -625,9 +676,10 @@ __global__ __launch_bounds__(RB) void k_ins_alloc(const uint64_t *tbm, uint32_t 
     }
 }
 
-__global__ __launch_bounds__(RB) void k_ins_scatter(const int4 *rec, int64_t k, const uint64_t *slot,
-                                                    const uint64_t *dbl, const uint32_t *tcnt, uint64_t *pool,
-                                                    uint64_t *pos, const GraphCtl *gc)
+// Writes both directed entries of each insertion and enters them into the edge
+// map (position in row = base length + index in the delta block).
+__global__ __launch_bounds__(RB) void k_ins_scatter(const int4 *rec, int64_t k, const uint64_t *slot, GView g,
+                                                    const uint32_t *tcnt, uint64_t *pos, const GraphCtl *gc)
 {
     const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
     if (i >= k) return;
@@ -637,12 +689,16 @@ __global__ __launch_bounds__(RB) void k_ins_scatter(const int4 *rec, int64_t k, 
         pos[i] = ~0ull;
         return;
     }
+    const uint32_t a = (uint32_t)r.x, b = (uint32_t)r.y, w = (uint32_t)r.z;
     const uint64_t s = slot[i];
-    const uint64_t pa = (dbl[r.x] >> 32) + tcnt[r.x] + (s >> 32);
-    const uint64_t pb = (dbl[r.y] >> 32) + tcnt[r.y] + (uint32_t)s;
-    pool[pa] = pack_entry((uint32_t)r.y, (uint32_t)r.z);
-    pool[pb] = pack_entry((uint32_t)r.x, (uint32_t)r.z);
+    const uint32_t ia = tcnt[a] + (uint32_t)(s >> 32), ib = tcnt[b] + (uint32_t)s;
+    const uint64_t pa = (g.dbl[a] >> 32) + ia;
+    const uint64_t pb = (g.dbl[b] >> 32) + ib;
+    g.pool[pa] = pack_entry(b, w);
+    g.pool[pb] = pack_entry(a, w);
     pos[i] = (pa << 32) | pb;
+    emap_insert(g, a, b, w, (uint32_t)(g.rp[a + 1] - g.rp[a]) + ia);
+    emap_insert(g, b, a, w, (uint32_t)(g.rp[b + 1] - g.rp[b]) + ib);
 }
 
 __global__ __launch_bounds__(RB) void k_ins_reset(uint64_t *tbm, uint32_t nwords, uint32_t *tcnt)
@@ -663,30 +719,9 @@ __global__ __launch_bounds__(RB) void k_ins_reset(uint64_t *tbm, uint32_t nwords
 // scan of the short delta row).
 // ---------------------------------------------------------------------------
 
-// Deletion set of the batch: one slot per deletion record (duplicates too).
-__global__ __launch_bounds__(RB) void k_del_set(const int4 *rec, int64_t k, uint64_t *dset, uint32_t dmask)
-{
-    const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
-    if (i >= k) return;
-    const int4 r = rec[i];
-    if (r.w != 0) return;
-    const uint32_t a = (uint32_t)r.x, b = (uint32_t)r.y, w = (uint32_t)r.z;
-    const uint64_t k1 = a < b ? ((uint64_t)a << 32 | b) : ((uint64_t)b << 32 | a);
-    for (uint32_t s = dset_hash(k1, w) & dmask;; s = (s + 1) & dmask) {
-        if (atomicCAS((unsigned long long *)&dset[2 * s], ~0ull, (unsigned long long)k1) == ~0ull) {
-            dset[2 * s + 1] = w;  // read by later kernels only
-            return;
-        }
-    }
-}
-
-// Deletion records: thread 2i+s tombstones copy s of record i in the base rows
-// (row a, then row b; a self-loop's two copies both sit in row a): a binary
-// search of the sorted row, then the run of equal keys.  Rows with a non-empty
-// delta block are marked in dbm; their delta entries are tombstoned by an
-// M_TOMB pass against the deletion set.
-__global__ __launch_bounds__(RB) void k_tomb(const int4 *rec, int64_t k, GView g, uint64_t *dbm,
-                                             unsigned long long *bstats)
+// Deletion records: thread 2i+s tombstones the copies of record i in row a
+// (s = 0) or row b (s = 1); a self-loop's two copies both sit in row a.
+__global__ __launch_bounds__(RB) void k_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats)
 {
     const int64_t t = (int64_t)blockIdx.x * RB + threadIdx.x;
     const int64_t ri = t >> 1;
@@ -694,27 +729,8 @@ __global__ __launch_bounds__(RB) void k_tomb(const int4 *rec, int64_t k, GView g
     if (ri < k) {
         const int4 r = rec[ri];
         const bool side = t & 1;
-        if (r.w == 0 && !(side && r.x == r.y)) {
-            const uint32_t row = (uint32_t)(side ? r.y : r.x), col = (uint32_t)(side ? r.x : r.y);
-            const uint64_t key = pack_entry(col, (uint32_t)r.z);
-            uint64_t lo = g.rp[row], hi = g.rp[row + 1];
-            const uint64_t end = hi;
-            while (lo < hi) {
-                const uint64_t mid = (lo + hi) >> 1;
-                ++probes;
-                if (e_key(g.adj[mid]) < key)
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            for (uint64_t j = lo; j < end; ++j) {
-                const uint64_t e = g.adj[j];
-                ++probes;
-                if (e_key(e) != key) break;
-                if (!(e & E_DEAD)) g.adj[j] = e | E_DEAD;
-            }
-            if ((uint32_t)g.dbl[row]) mark(dbm, row);
-        }
+        if (r.w == 0 && !(side && r.x == r.y))
+            probes = emap_kill(g, (uint32_t)(side ? r.y : r.x), (uint32_t)(side ? r.x : r.y), (uint32_t)r.z);
     }
     const uint32_t p = wave_sum(probes);
     if (lane_id() == 0) add_stat(bstats, ST_PROBES, p);
@@ -874,7 +890,7 @@ void launch_frontier_ept(int mode, const FArgs &a, int grid, hipStream_t s)
     case M_TBFS: hipLaunchKernelGGL((k_frontier<M_TBFS, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
     case M_PICK: hipLaunchKernelGGL((k_frontier<M_PICK, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
     case M_OFCHECK: hipLaunchKernelGGL((k_frontier<M_OFCHECK, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_frontier<M_TOMB, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_frontier<M_EMAP, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
     }
 }
 
@@ -940,11 +956,10 @@ void launch_ins_alloc(const uint64_t *tbm, uint32_t nwords, const uint32_t *wsum
                            tcnt, pool, gc);
 }
 
-void launch_ins_scatter(const int4 *rec, int64_t k, const uint64_t *slot, const uint64_t *dbl, const uint32_t *tcnt,
-                        uint64_t *pool, uint64_t *pos, const GraphCtl *gc, hipStream_t s)
+void launch_ins_scatter(const int4 *rec, int64_t k, const uint64_t *slot, GView g, const uint32_t *tcnt,
+                        uint64_t *pos, const GraphCtl *gc, hipStream_t s)
 {
-    if (k > 0)
-        hipLaunchKernelGGL(k_ins_scatter, dim3(blocks_for(k)), dim3(RB), 0, s, rec, k, slot, dbl, tcnt, pool, pos, gc);
+    if (k > 0) hipLaunchKernelGGL(k_ins_scatter, dim3(blocks_for(k)), dim3(RB), 0, s, rec, k, slot, g, tcnt, pos, gc);
 }
 
 void launch_ins_reset(uint64_t *tbm, uint32_t nwords, uint32_t *tcnt, hipStream_t s)
@@ -960,14 +975,9 @@ void launch_classify(const int4 *rec, int64_t k, uint32_t kshift, uint64_t *st, 
                            inv_bm, out_act);
 }
 
-void launch_del_set(const int4 *rec, int64_t k, uint64_t *dset, uint32_t dmask, hipStream_t s)
+void launch_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats, hipStream_t s)
 {
-    if (k > 0) hipLaunchKernelGGL(k_del_set, dim3(blocks_for(k)), dim3(RB), 0, s, rec, k, dset, dmask);
-}
-
-void launch_tomb(const int4 *rec, int64_t k, GView g, uint64_t *dbm, unsigned long long *bstats, hipStream_t s)
-{
-    if (k > 0) hipLaunchKernelGGL(k_tomb, dim3(blocks_for(2 * k)), dim3(RB), 0, s, rec, k, g, dbm, bstats);
+    if (k > 0) hipLaunchKernelGGL(k_tomb, dim3(blocks_for(2 * k)), dim3(RB), 0, s, rec, k, g, bstats);
 }
 
 void launch_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint64_t *pos, const uint64_t *pool,

@@ -50,13 +50,14 @@ struct Request {
     bool done = false;
 };
 
-// Device bytes of an instance: CSR + delta descriptors/counters + an equal
-// share of the delta pool + tree state, stamps, queues, invalidation list.
+// Device bytes of an instance: CSR, delta descriptors/capacities/counters and
+// bitmaps, the initial delta pool (E/2 + 2^16 entries), the edge map, and the
+// tree (state, bitmaps, chunk lists).
 uint64_t instance_bytes(int64_t n, int64_t m)
 {
-    const uint64_t E = 2 * (uint64_t)m;
-    return 8 * ((uint64_t)n + 1) + 8 * E + 12 * (uint64_t)n + 8 * std::max<uint64_t>(E / 4, 1024) +
-           24 * (uint64_t)n + 16 * (E / 512 + 1);
+    const uint64_t E = 2 * (uint64_t)m, N = (uint64_t)n;
+    return 8 * (N + 1) + 8 * E + 16 * N + 3 * N / 8 + 8 * (E / 2 + (1u << 16)) + 8 * emap_slots_for(E + E / 8) +
+           8 * N + 3 * N / 8 + 16 * (E / 128 + 1024);
 }
 
 }  // namespace

@@ -99,13 +99,16 @@ typedef struct {
 typedef struct {
     int32_t block;            /* threads per workgroup of frontier kernels (fixed 256) */
     int32_t grid;             /* workgroups per frontier launch                        */
-    int32_t heavy_degree;     /* rows longer than this are split into chunks           */
-    int32_t chunk;            /* entries per heavy chunk (multiple of 64)              */
+    int32_t heavy_degree;     /* a tile (the set slots of one 64-bit frontier word)
+                                 with more row entries than this is split           */
+    int32_t chunk;            /* entries per split piece (multiple of 64)              */
     int32_t round_batch;      /* rounds launched before the first convergence check;
                                  the group doubles (up to 64) while rounds continue   */
     int32_t kernel_timing;    /* 1: time every relaxation launch with events           */
     int32_t use_graph;        /* 1: launch round groups as captured hipGraphs          */
     int32_t edges_per_thread; /* adjacency entries in flight per lane: 1, 2 or 4       */
+    int32_t tile_entries;     /* row entries a wave takes per tile of up to 64 frontier
+                                 slots; slots past it are deferred as pieces         */
 } dsssp_tuning;
 
 /* ---- library ---------------------------------------------------------- */

@@ -262,6 +262,24 @@ def test_update_after_overflow_error():
     assert list(d[:3]) == [0, 2_000_000_000, 4_000_000_000] and d[3] == ds.INF
 
 
+def test_pool_compaction_and_map_rebuild():
+    """Forty insertion-heavy batches on a 500-vertex ring: the delta pool is
+    compacted and grown and the edge map rebuilt several times; every batch is
+    checked."""
+    rng = np.random.default_rng(5)
+    n = 500
+    a = np.arange(n, dtype=np.int32)
+    b = ((a + 1) % n).astype(np.int32)
+    w = rng.integers(1, 20, n).astype(np.int32)
+    batches, ca, cb, cw = [], a, b, w
+    for _ in range(40):
+        chg = random_batch(rng, n, ca, cb, cw, 400, 0.8, 1, 20)
+        batches.append(chg)
+        ca, cb, cw = oracle.apply_changes(n, ca, cb, cw, chg)
+    g, _ = run_sequence(n, a, b, w, batches, src=0, what="compaction")
+    assert g.info()["device_bytes"] > 0
+
+
 def test_long_rows_tombstones():
     """Deletions in rows longer than 64 entries (base and delta), including a key
     with 150 parallel copies that spans several 64-entry scans."""
