@@ -1,20 +1,23 @@
 // common.h -- layouts and constants shared by the host code and the gfx950 kernels.
 //
 // HBM layout (all arrays 64-bit aligned, one allocation each):
-//   graph  rp   uint64[n+1]   base CSR row offsets
+//   graph  vr   VRec[n]       per-vertex record (32 B): base row offset and
+//                             length, delta block capacity, delta block offset
+//                             and length -- one cache line holds two
 //          adj  uint64[E]     base entries (neighbour << 32 | weight), rows sorted
 //                             ascending, i.e. by (neighbour, weight)
-//          dbl  uint64[n]     delta row descriptor (pool offset << 32 | length)
-//          pool uint64[cap]   delta entries appended by insertions (rows unsorted)
-//          tcnt uint32[n]     per-vertex insertion counters, zero between updates
-//          tbm  uint64[n/64]  bitmap of vertices touched by the batch's insertions
+//          pool uint64[cap]   delta blocks: entries appended by insertions (rows
+//                             unsorted); a block's capacity doubles when outgrown
+//          emap uint64[2^k]   edge map: per live directed entry, a 32-bit key
+//                             fingerprint and its position in its row
+//          tbm  uint64[n/64]  rows whose delta block must grow in this batch
 //          wsum/bsum/bbase    pool space per bitmap word / per 256 words, and the
 //                             pool offset of each 256-word group, for one batch
 //   tree   st   uint64[S]     packed state (distance << 32 | parent), S = n * K'
 //                             slots (K' = trees rounded up to a power of two)
 //          fb   2 x uint64[S/64]  frontier bitmaps (ping-pong between rounds)
 //          ib   uint64[S/64]  bitmap of slots invalidated by the current update
-//          hq   2 x uint64[h] heavy-row chunks (slot << 32 | chunk)
+//          hq   2 x 2 x uint64[h] deferred pieces of rows (items)
 // The weight's bit 31 is the tombstone bit (weights are >= 0), playing the role
 // of the reference's wt = -1 (gpuFunctions_undir.cuh:203,210,321,384).
 #pragma once
@@ -41,6 +44,18 @@ DSSSP_HD uint64_t pack_entry(uint32_t col, uint32_t w) { return ((uint64_t)col <
 DSSSP_HD uint32_t e_col(uint64_t e) { return (uint32_t)(e >> 32); }
 DSSSP_HD uint32_t e_w(uint64_t e) { return (uint32_t)e; }
 DSSSP_HD uint64_t e_key(uint64_t e) { return e & ~E_DEAD; }
+
+// Per-vertex record.  A row's entries are its base entries adj[boff, boff+blen)
+// followed by its delta entries pool[dbl >> 32, (dbl >> 32) + (uint32)dbl); a
+// position in a row counts base entries first.
+struct VRec {
+    uint64_t boff;  // first base entry
+    uint32_t blen;  // base entries
+    uint32_t dcap;  // delta block capacity
+    uint64_t dbl;   // delta block: pool offset << 32 | length
+    uint64_t pad;
+};
+static_assert(sizeof(VRec) == 32, "VRec is 32 bytes");
 
 // Error bits raised by kernels (Counters::err / GraphCtl::err).
 constexpr uint32_t ERR_DIST_OVERFLOW = 1u;

@@ -99,6 +99,8 @@ int check_device(int dev)
 
 inline uint32_t words_of(uint64_t bits) { return (uint32_t)((bits + 63) / 64); }
 
+constexpr uint32_t INIT_DCAP = 4;  // initial delta block capacity of every row
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -111,13 +113,12 @@ struct dsssp_graph {
     int64_t n = 0;
     uint32_t nw = 0;  // words of a vertex bitmap
     uint64_t E = 0;   // base entries
-    uint64_t *rp = nullptr, *adj = nullptr, *dbl = nullptr, *pool = nullptr;
+    VRec *vr = nullptr;  // per-vertex records
+    uint64_t *adj = nullptr, *pool = nullptr;
     uint64_t pool_cap = 0;
     uint64_t pool_top = 0;       // host copy, valid between updates
     uint64_t delta_entries = 0;  // sum of delta row lengths
-    uint32_t *dcap = nullptr;  // capacity of each delta block
-    uint32_t *tcnt = nullptr;
-    uint64_t *tbm = nullptr;  // vertices touched by the batch's insertions
+    uint64_t *tbm = nullptr;  // rows whose delta block grows in this batch
     uint64_t *emap = nullptr;  // edge map (fingerprint << 32 | position in row)
     uint64_t emap_slots = 0;
     uint64_t emap_used = 0;    // entries entered since the map was built (dead ones too)
@@ -136,7 +137,7 @@ struct dsssp_graph {
     bool destroy_pending = false;
     std::mutex mu;
 
-    GView view() const { return GView{rp, adj, dbl, pool, emap, emap_slots - 1}; }
+    GView view() const { return GView{vr, adj, pool, emap, emap_slots - 1}; }
 };
 
 struct dsssp_tree {
@@ -170,7 +171,7 @@ struct dsssp_batch {
     int64_t k = 0, n_ins = 0, n_del = 0;
     bool zero_w = false;
     int4 *rec = nullptr;
-    uint64_t *slot = nullptr, *pos = nullptr;
+    uint32_t *tslot = nullptr;  // per record copy: position in its row's delta block
 };
 
 namespace {
@@ -409,34 +410,33 @@ int ensure_pool(dsssp_graph *g, uint64_t n_ins)
     if (3 * g->pool_top + add <= g->pool_cap) return DSSSP_OK;
     // Compacted on the host: rare (geometric growth), and keeps the kernels free
     // of pool bookkeeping.
-    std::vector<uint64_t> dbl((size_t)g->n), pool((size_t)g->pool_top), out;
-    std::vector<uint32_t> dcap((size_t)g->n);
-    HIP_TRY(hipMemcpyAsync(dbl.data(), g->dbl, sizeof(uint64_t) * (size_t)g->n, hipMemcpyDeviceToHost, g->stream));
-    HIP_TRY(hipMemcpyAsync(dcap.data(), g->dcap, sizeof(uint32_t) * (size_t)g->n, hipMemcpyDeviceToHost, g->stream));
+    std::vector<VRec> vr((size_t)g->n);
+    std::vector<uint64_t> pool((size_t)g->pool_top), out;
+    HIP_TRY(hipMemcpyAsync(vr.data(), g->vr, sizeof(VRec) * (size_t)g->n, hipMemcpyDeviceToHost, g->stream));
     if (g->pool_top)
         HIP_TRY(hipMemcpyAsync(pool.data(), g->pool, sizeof(uint64_t) * (size_t)g->pool_top, hipMemcpyDeviceToHost,
                                g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
     uint64_t live = 0;
-    for (int64_t v = 0; v < g->n; ++v) live += dcap[v];
+    for (int64_t v = 0; v < g->n; ++v) live += vr[v].dcap;
     uint64_t cap = std::max<uint64_t>(g->pool_cap, 2 * (3 * live + add) + (1u << 16));
     if (cap > 0xFFFFFFFFull) cap = 0xFFFFFFFFull;
     if (3 * live + add > cap) return fail(DSSSP_ERR_NOMEM, "delta pool would exceed 2^32 entries");
     out.reserve((size_t)live);
     for (int64_t v = 0; v < g->n; ++v) {
-        if (!dcap[v]) continue;
-        const uint64_t d = dbl[v];
+        if (!vr[v].dcap) continue;
+        const uint64_t d = vr[v].dbl;
         const uint64_t beg = d >> 32, len = (uint32_t)d;
         const uint64_t nb = out.size();
         out.insert(out.end(), pool.begin() + (int64_t)beg, pool.begin() + (int64_t)(beg + len));
-        out.resize(nb + dcap[v], 0ull);
-        dbl[v] = (nb << 32) | len;
+        out.resize(nb + vr[v].dcap, 0ull);
+        vr[v].dbl = (nb << 32) | len;
     }
     uint64_t *np = nullptr;
     if (int s = dev_alloc((void **)&np, sizeof(uint64_t) * cap, "delta pool")) return s;
     if (!out.empty())
         HIP_TRY(hipMemcpyAsync(np, out.data(), sizeof(uint64_t) * out.size(), hipMemcpyHostToDevice, g->stream));
-    HIP_TRY(hipMemcpyAsync(g->dbl, dbl.data(), sizeof(uint64_t) * (size_t)g->n, hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipMemcpyAsync(g->vr, vr.data(), sizeof(VRec) * (size_t)g->n, hipMemcpyHostToDevice, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
     dev_free(g->pool);
     g->pool = np;
@@ -496,7 +496,6 @@ void scrub(dsssp_tree *t)
     launch_fill_u64(t->fb[1], 0ull, t->nw, s);
     launch_fill_u64(t->ib, 0ull, t->nw, s);
     launch_fill_u64(t->g->tbm, 0ull, t->g->nw, s);
-    launch_fill_u32(t->g->tcnt, 0u, (uint64_t)t->g->n, s);
     zero_ring(t);
     (void)hipStreamSynchronize(s);
 }
@@ -564,13 +563,12 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
     // 1. insertions into the resident CSR
     if (b->n_ins > 0) {
         launch_fill_u32(&g->gc->err, 0u, 1, s);
-        launch_ins_count(b->rec, b->k, g->tcnt, b->slot, g->tbm, s);
-        launch_ins_wordsum(g->tbm, g->nw, g->dbl, g->dcap, g->tcnt, g->wsum, g->bsum, s);
+        launch_ins_place(b->rec, b->k, g->view(), b->tslot, g->tbm, s);
+        launch_ins_wordsum(g->tbm, g->nw, g->vr, g->wsum, g->bsum, s);
         launch_ins_scan(g->bsum, g->nw, g->bbase, g->pool_cap, g->gc, s);
-        launch_ins_alloc(g->tbm, g->nw, g->wsum, g->bbase, g->dbl, g->dcap, g->tcnt, g->pool, g->gc, s);
-        launch_ins_scatter(b->rec, b->k, b->slot, g->view(), g->tcnt, b->pos, g->gc, s);
+        launch_ins_alloc(g->tbm, g->nw, g->wsum, g->bbase, g->vr, g->pool, g->gc, s);
+        launch_ins_pending(b->rec, b->k, g->view(), b->tslot, g->gc, s);
         g->emap_used += 2 * (uint64_t)b->n_ins;
-        launch_ins_reset(g->tbm, g->nw, g->tcnt, s);
         if (b->zero_w) g->zero_w = true;
     }
     HIP_TRY(hipEventRecord(g->ev[1], s));
@@ -593,7 +591,7 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
             single_pass(t, M_SEED, a);
         }
         if (b->n_ins > 0)
-            launch_ins_offer(b->rec, b->k, t->kshift, b->pos, g->pool, t->st, t->fb[0], &t->c->act[0], t->c, s);
+            launch_ins_offer(b->rec, b->k, t->kshift, b->tslot, g->view(), t->st, t->fb[0], &t->c->act[0], t->c, s);
     }
     HIP_TRY(hipEventRecord(g->ev[4], s));
     // 5. relaxation
@@ -752,12 +750,9 @@ static void graph_release(dsssp_graph *g)
     if (!g) return;
     DevGuard dg(g->device);
     if (g->stream) (void)hipStreamSynchronize(g->stream);
-    dev_free(g->rp);
+    dev_free(g->vr);
     dev_free(g->adj);
-    dev_free(g->dbl);
     dev_free(g->pool);
-    dev_free(g->tcnt);
-    dev_free(g->dcap);
     dev_free(g->tbm);
     dev_free(g->emap);
     dev_free(g->wsum);
@@ -779,11 +774,8 @@ static int graph_alloc(dsssp_graph *g, uint64_t pool_cap)
     for (auto &e : g->ev) HIP_TRY(hipEventCreate(&e));
     const size_t n = (size_t)g->n;
     g->nw = words_of(n);
-    if (int s = dev_alloc((void **)&g->rp, sizeof(uint64_t) * (n + 1), "row offsets")) return s;
+    if (int s = dev_alloc((void **)&g->vr, sizeof(VRec) * n, "vertex records")) return s;
     if (int s = dev_alloc((void **)&g->adj, sizeof(uint64_t) * g->E, "adjacency")) return s;
-    if (int s = dev_alloc((void **)&g->dbl, sizeof(uint64_t) * n, "delta rows")) return s;
-    if (int s = dev_alloc((void **)&g->tcnt, sizeof(uint32_t) * n, "insert counters")) return s;
-    if (int s = dev_alloc((void **)&g->dcap, sizeof(uint32_t) * n, "delta capacities")) return s;
     if (int s = dev_alloc((void **)&g->tbm, sizeof(uint64_t) * g->nw, "touched bitmap")) return s;
     if (int s = dev_alloc((void **)&g->wsum, sizeof(uint32_t) * g->nw, "word sizes")) return s;
     const size_t nb = (g->nw + 255) / 256;
@@ -794,9 +786,6 @@ static int graph_alloc(dsssp_graph *g, uint64_t pool_cap)
     if (int s = dev_alloc((void **)&g->pool, sizeof(uint64_t) * pool_cap, "delta pool")) return s;
     HIP_TRY(hipHostMalloc((void **)&g->gc_host, sizeof(GraphCtl), hipHostMallocDefault));
     memset(g->gc_host, 0, sizeof(GraphCtl));
-    HIP_TRY(hipMemset(g->dbl, 0, sizeof(uint64_t) * n));
-    HIP_TRY(hipMemset(g->tcnt, 0, sizeof(uint32_t) * n));
-    HIP_TRY(hipMemset(g->dcap, 0, sizeof(uint32_t) * n));
     HIP_TRY(hipMemset(g->tbm, 0, sizeof(uint64_t) * g->nw));
     HIP_TRY(hipMemset(g->gc, 0, sizeof(GraphCtl)));
     return DSSSP_OK;
@@ -813,24 +802,41 @@ extern "C" int dsssp_graph_create(int64_t n, int64_t m, const int32_t *a, const 
     if (int s = check_device(t_device)) return s;
     std::vector<uint64_t> rp, adj;
     if (int s = build_csr(n, m, a, b, w, rp, adj)) return s;
+    if ((uint64_t)n * INIT_DCAP * 3 + adj.size() / 2 > 0xFFFFFFFFull / 2)
+        return fail(DSSSP_ERR_ARG, "graph too large for the delta pool (n=%lld, entries=%zu)", (long long)n,
+                    adj.size());
     dsssp_graph *g = new dsssp_graph();
     g->device = t_device;
     g->n = n;
     g->E = adj.size();
     for (int64_t i = 0; i < m && !g->zero_w; ++i) g->zero_w = w[i] == 0;
-    const uint64_t pool_cap = std::min<uint64_t>(g->E / 2 + (1u << 16), 0xFFFFFFFFull / 2);
+    // Every row starts with a delta block of INIT_DCAP entries, so most
+    // insertions land in place; the pool leaves room for the growth bound of
+    // ensure_pool.
+    const uint64_t pool_cap = std::min<uint64_t>(g->E / 2 + 3 * INIT_DCAP * (uint64_t)n + (1u << 16),
+                                                 0xFFFFFFFFull / 2);
     if (int s = graph_alloc(g, pool_cap)) {
         graph_release(g);
         return s;
     }
     DevGuard dg(g->device);
-    if (hipMemcpy(g->rp, rp.data(), sizeof(uint64_t) * rp.size(), hipMemcpyHostToDevice) != hipSuccess ||
+    std::vector<VRec> vr((size_t)n);
+    parallel_ranges(n, [&](int64_t lo, int64_t hi, int) {
+        for (int64_t v = lo; v < hi; ++v)
+            vr[v] = VRec{rp[v], (uint32_t)(rp[v + 1] - rp[v]), INIT_DCAP, ((uint64_t)v * INIT_DCAP) << 32, 0ull};
+    });
+    if (hipMemcpy(g->vr, vr.data(), sizeof(VRec) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess ||
         (g->E && hipMemcpy(g->adj, adj.data(), sizeof(uint64_t) * g->E, hipMemcpyHostToDevice) != hipSuccess)) {
         const int s = fail(DSSSP_ERR_HIP, "graph upload failed: %s", hipGetErrorString(hipGetLastError()));
         graph_release(g);
         return s;
     }
-    int s = build_emap(g, g->E, g->E / 8);
+    g->pool_top = INIT_DCAP * (uint64_t)n;
+    g->gc_host->pool_top = g->pool_top;
+    int s = hipMemcpy(&g->gc->pool_top, &g->pool_top, sizeof(uint64_t), hipMemcpyHostToDevice) == hipSuccess
+                ? DSSSP_OK
+                : fail(DSSSP_ERR_HIP, "graph upload failed");
+    if (!s) s = build_emap(g, g->E, g->E / 8);
     if (!s && hipStreamSynchronize(g->stream) != hipSuccess)
         s = fail(DSSSP_ERR_HIP, "edge map build failed: %s", hipGetErrorString(hipGetLastError()));
     if (s) {
@@ -868,7 +874,7 @@ extern "C" int dsssp_graph_info(const dsssp_graph *g, int64_t *n, int64_t *m_liv
         *m_live = m;
     }
     if (device_bytes)
-        *device_bytes = 8 * ((uint64_t)g->n + 1) + 8 * g->E + 16 * (uint64_t)g->n + 20 * (uint64_t)g->nw +
+        *device_bytes = 32 * (uint64_t)g->n + 8 * g->E + 12 * (uint64_t)g->nw +
                         16 * (uint64_t)((g->nw + 255) / 256) + 8 * g->pool_cap + 8 * g->emap_slots;
     return DSSSP_OK;
 }
@@ -879,11 +885,11 @@ extern "C" int dsssp_graph_export(dsssp_graph *g, int64_t *m, int32_t *a, int32_
     if (!g || !m) return fail(DSSSP_ERR_ARG, "null argument");
     DevGuard dg(g->device);
     const size_t n = (size_t)g->n;
-    std::vector<uint64_t> rp(n + 1), adj(g->E), dbl(n), pool(g->pool_top);
+    std::vector<VRec> vr(n);
+    std::vector<uint64_t> adj(g->E), pool(g->pool_top);
     HIP_TRY(hipStreamSynchronize(g->stream));
-    HIP_TRY(hipMemcpy(rp.data(), g->rp, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(vr.data(), g->vr, sizeof(VRec) * n, hipMemcpyDeviceToHost));
     if (g->E) HIP_TRY(hipMemcpy(adj.data(), g->adj, sizeof(uint64_t) * g->E, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(dbl.data(), g->dbl, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
     if (g->pool_top) HIP_TRY(hipMemcpy(pool.data(), g->pool, sizeof(uint64_t) * g->pool_top, hipMemcpyDeviceToHost));
     // Each undirected edge is stored once per endpoint (a self-loop twice in its
     // row): keep the copy in the row of the smaller endpoint.
@@ -902,8 +908,8 @@ extern "C" int dsssp_graph_export(dsssp_graph *g, int64_t *m, int32_t *a, int32_
     };
     for (size_t v = 0; v < n; ++v) {
         uint64_t selfs = 0;
-        for (uint64_t j = rp[v]; j < rp[v + 1]; ++j) take((uint32_t)v, adj[j], selfs);
-        const uint64_t beg = dbl[v] >> 32, len = (uint32_t)dbl[v];
+        for (uint64_t j = vr[v].boff; j < vr[v].boff + vr[v].blen; ++j) take((uint32_t)v, adj[j], selfs);
+        const uint64_t beg = vr[v].dbl >> 32, len = (uint32_t)vr[v].dbl;
         for (uint64_t j = 0; j < len; ++j) take((uint32_t)v, pool[beg + j], selfs);
     }
     *m = cnt;
@@ -945,10 +951,8 @@ extern "C" int dsssp_graph_copy(dsssp_graph *dst, dsssp_graph *src)
     }
     const size_t n = (size_t)src->n;
     hipStream_t s = dst->stream;
-    HIP_TRY(hipMemcpyAsync(dst->rp, src->rp, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dst->vr, src->vr, sizeof(VRec) * n, hipMemcpyDeviceToDevice, s));
     if (src->E) HIP_TRY(hipMemcpyAsync(dst->adj, src->adj, sizeof(uint64_t) * src->E, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(hipMemcpyAsync(dst->dbl, src->dbl, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(hipMemcpyAsync(dst->dcap, src->dcap, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
     if (src->pool_top)
         HIP_TRY(hipMemcpyAsync(dst->pool, src->pool, sizeof(uint64_t) * src->pool_top, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemcpyAsync(dst->gc, src->gc, sizeof(GraphCtl), hipMemcpyDeviceToDevice, s));
@@ -959,7 +963,6 @@ extern "C" int dsssp_graph_copy(dsssp_graph *dst, dsssp_graph *src)
         dst->emap_slots = src->emap_slots;
     }
     HIP_TRY(hipMemcpyAsync(dst->emap, src->emap, 8 * src->emap_slots, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(hipMemsetAsync(dst->tcnt, 0, sizeof(uint32_t) * n, s));
     HIP_TRY(hipMemsetAsync(dst->tbm, 0, sizeof(uint64_t) * dst->nw, s));
     HIP_TRY(hipStreamSynchronize(s));
     dst->pool_top = src->pool_top;
@@ -1224,8 +1227,7 @@ extern "C" int dsssp_batch_upload(dsssp_graph *g, const dsssp_change *chg, int64
     b->n_del = nd;
     b->zero_w = zw;
     int s = dev_alloc((void **)&b->rec, sizeof(int4) * (size_t)std::max<int64_t>(k, 1), "batch");
-    if (!s) s = dev_alloc((void **)&b->slot, sizeof(uint64_t) * (size_t)std::max<int64_t>(k, 1), "batch slots");
-    if (!s) s = dev_alloc((void **)&b->pos, sizeof(uint64_t) * (size_t)std::max<int64_t>(k, 1), "batch positions");
+    if (!s) s = dev_alloc((void **)&b->tslot, sizeof(uint32_t) * 2 * (size_t)std::max<int64_t>(k, 1), "batch slots");
     if (!s && k > 0 &&
         hipMemcpyAsync(b->rec, chg, sizeof(int4) * (size_t)k, hipMemcpyHostToDevice, g->stream) != hipSuccess)
         s = fail(DSSSP_ERR_HIP, "batch upload failed");
@@ -1247,8 +1249,7 @@ extern "C" int dsssp_batch_destroy(dsssp_batch *b)
         DevGuard dg(g->device);
         (void)hipStreamSynchronize(g->stream);
         dev_free(b->rec);
-        dev_free(b->slot);
-        dev_free(b->pos);
+        dev_free(b->tslot);
         delete b;
     }
     graph_detach(g);

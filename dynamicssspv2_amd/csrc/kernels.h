@@ -8,9 +8,8 @@ namespace dsssp {
 
 // Read-mostly view of the device CSR handed to kernels by value.
 struct GView {
-    const uint64_t *rp;   // n+1 base offsets
+    VRec *vr;             // per-vertex records
     uint64_t *adj;        // base entries
-    const uint64_t *dbl;  // delta descriptors
     uint64_t *pool;       // delta entries
     uint64_t *emap;       // edge map: slots of fingerprint << 32 | position in row
     uint64_t emask;       // edge map slots - 1
@@ -68,30 +67,28 @@ void launch_frontier(int mode, const FArgs &a, int grid, hipStream_t s);
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 void launch_reduce_stats(Counters *c, unsigned long long *bstats, hipStream_t s);
-// insertion of a batch into the delta rows
-void launch_ins_count(const int4 *rec, int64_t k, uint32_t *tcnt, uint64_t *slot, uint64_t *tbm, hipStream_t s);
-// Pool space per bitmap word (wsum) and per 256-word group (bsum), the scan of
-// the group sums (bbase, pool top), then the per-vertex block moves.  Delta
-// blocks have a capacity (dcap) that doubles when outgrown.
-void launch_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const uint64_t *dbl, const uint32_t *dcap,
-                        const uint32_t *tcnt, uint32_t *wsum, uint64_t *bsum, hipStream_t s);
+// Insertion of a batch into the delta blocks.  Thread 2i+s places copy s of
+// record i (tslot[2i+s] = its position in the delta block, bit 31 set while
+// pending); rows that overflow are marked in tbm and grown: pool space per
+// bitmap word (wsum) and per 256-word group (bsum), the scan of the group sums
+// (bbase, pool top), the block moves, then the pending copies.
+constexpr uint32_t SLOT_PENDING = 0x80000000u;
+void launch_ins_place(const int4 *rec, int64_t k, GView g, uint32_t *tslot, uint64_t *tbm, hipStream_t s);
+void launch_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const VRec *vr, uint32_t *wsum, uint64_t *bsum,
+                        hipStream_t s);
 void launch_ins_scan(const uint64_t *bsum, uint32_t nwords, uint64_t *bbase, uint64_t pool_cap, GraphCtl *gc,
                      hipStream_t s);
-void launch_ins_alloc(const uint64_t *tbm, uint32_t nwords, const uint32_t *wsum, const uint64_t *bbase,
-                      uint64_t *dbl, uint32_t *dcap, uint32_t *tcnt, uint64_t *pool, const GraphCtl *gc,
-                      hipStream_t s);
-void launch_ins_scatter(const int4 *rec, int64_t k, const uint64_t *slot, GView g, const uint32_t *tcnt,
-                        uint64_t *pos, const GraphCtl *gc, hipStream_t s);
-void launch_ins_reset(uint64_t *tbm, uint32_t nwords, uint32_t *tcnt, hipStream_t s);
-// deletions, offers, roots, parent repair
+void launch_ins_alloc(uint64_t *tbm, uint32_t nwords, const uint32_t *wsum, const uint64_t *bbase, VRec *vr,
+                      uint64_t *pool, const GraphCtl *gc, hipStream_t s);
+void launch_ins_pending(const int4 *rec, int64_t k, GView g, uint32_t *tslot, const GraphCtl *gc, hipStream_t s);
 // Deletions: the tree-edge test (marks the roots of invalidated subtrees) and
 // the tombstoning of every copy of each deleted entry.
 void launch_classify(const int4 *rec, int64_t k, uint32_t kshift, uint64_t *st, uint64_t *out_bm, uint64_t *inv_bm,
                      uint32_t *out_act, hipStream_t s);
 // Tombstones every copy of each deleted entry, found through the edge map.
 void launch_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats, hipStream_t s);
-void launch_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint64_t *pos, const uint64_t *pool,
-                      uint64_t *st, uint64_t *out_bm, uint32_t *out_act, Counters *c, hipStream_t s);
+void launch_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint32_t *tslot, GView g, uint64_t *st,
+                      uint64_t *out_bm, uint32_t *out_act, Counters *c, hipStream_t s);
 // Roots: for j < count, vertex src[j] becomes a root (distance 0, parent itself)
 // of tree (per_tree ? j : 0) and is marked.
 void launch_set_sources(uint64_t *st, const uint32_t *src, uint32_t count, uint32_t kshift, bool per_tree,

@@ -92,9 +92,9 @@ __device__ __forceinline__ uint32_t emap_kill(const GView &g, uint32_t row, uint
     const uint64_t h = emap_hash(row, col, w);
     const uint32_t fp = emap_fp(h);
     const uint64_t key = pack_entry(col, w);
-    const uint64_t b0 = g.rp[row];
-    const uint32_t blen = (uint32_t)(g.rp[row + 1] - b0);
-    const uint64_t d = g.dbl[row];
+    const uint64_t b0 = g.vr[row].boff;
+    const uint32_t blen = g.vr[row].blen;
+    const uint64_t d = g.vr[row].dbl;
     const uint32_t dl = (uint32_t)d;
     uint32_t probes = 0;
     for (uint64_t s = h & g.emask;; s = (s + 1) & g.emask) {
@@ -301,10 +301,10 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
     uint32_t deg = 0, blen = 0, dbeg = 0, aux0 = 0, aux1 = 0;
     uint64_t b0 = 0;
     if (valid) {
-        const uint32_t u = os >> A.kshift;
-        b0 = A.g.rp[u];
-        blen = (uint32_t)(A.g.rp[u + 1] - b0);
-        const uint64_t dd = A.g.dbl[u];
+        const VRec *r = &A.g.vr[os >> A.kshift];
+        b0 = r->boff;
+        blen = r->blen;
+        const uint64_t dd = r->dbl;
         dbeg = (uint32_t)(dd >> 32);
         deg = blen + (uint32_t)dd;
         if (!owner_prep<MODE>(A, os, aux0, aux1)) deg = 0;
@@ -438,62 +438,67 @@ __global__ __launch_bounds__(FB) void k_frontier(FArgs A)
     const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t gw = blockIdx.x * FW + wv;
     const uint32_t nw = gridDim.x * FW;
+    // The bitmap is read in blocks of 8 words (64 bytes); in one pass a wave
+    // reads 8 blocks nw apart, so that a cluster of non-empty words (frontiers
+    // cluster by vertex id) spreads over many waves.  The set slots of the 64
+    // words are then taken 64 at a time, in word order.
+    const uint32_t nblk = A.tiles ? (A.nwords + 7) / 8 : 0u;
+    auto load_word = [&](uint32_t pb, uint32_t &wi) -> uint64_t {
+        const uint32_t blk = pb + (lane >> 3) * nw + gw;
+        wi = blk * 8 + (lane & 7);
+        uint64_t word = 0;
+        if (blk < nblk && wi < A.nwords) {
+            if (A.identity) {
+                const uint32_t lo = wi * 64;
+                word = A.nslots - lo >= 64 ? ~0ull : ((1ull << (A.nslots - lo)) - 1ull);
+            } else {
+                word = A.in_bm[wi];
+                if (word && A.consume) A.in_bm[wi] = 0;
+            }
+        }
+        return word;
+    };
+    // The first pass's words are loaded alongside the item count.
+    uint32_t wi = 0;
+    uint64_t word = nblk ? load_word(0, wi) : 0ull;
     // Items left by the previous launch; count clamped to the list capacity.
     const uint32_t H = A.in_hcnt ? min(*A.in_hcnt, A.hcap) : 0u;
     // A round after convergence has an empty bitmap and no items: it returns
-    // without scanning.
+    // without scanning further.
     if (H == 0 && A.in_act && *A.in_act == 0) return;
     WaveStats S{0, 0, 0, false};
+    for (uint32_t pb = 0; pb < nblk; pb += nw * 8) {
+        if (pb) word = load_word(pb, wi);
+        const uint32_t cnt = (uint32_t)__popcll(word);
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(incl, d);
+            if ((int)lane >= d) incl += y;
+        }
+        const uint32_t T = __shfl(incl, 63);
+        for (uint32_t r0 = 0; r0 < T; r0 += 64) {
+            const uint32_t q = r0 + lane;  // rank of this lane's slot
+            uint32_t o = 0;                // owner lane: the number of lanes with incl <= q
+#pragma unroll
+            for (uint32_t st = 32; st > 0; st >>= 1) {
+                const uint32_t v = (uint32_t)__shfl((int)incl, (int)(o + st - 1));
+                if (v <= q) o += st;
+            }
+            o = min(o, 63u);
+            const uint64_t ow = __shfl(word, (int)o);
+            const uint32_t oex = (uint32_t)__shfl((int)(incl - cnt), (int)o);
+            const uint32_t owi = (uint32_t)__shfl((int)wi, (int)o);
+            const bool valid = q < T;
+            const uint32_t os = valid ? owi * 64 + select_bit(ow, q - oex) : 0u;
+            do_slots<MODE, EPT>(A, os, valid, 0u, true, sm, wv, S);
+        }
+    }
     for (uint32_t it = gw; it < H; it += nw) {
         const uint64_t mask = A.in_hq[2 * (uint64_t)it];
         const uint64_t ws = A.in_hq[2 * (uint64_t)it + 1];
         const uint32_t widx = (uint32_t)(ws >> 32);
         do_slots<MODE, EPT>(A, widx * 64 + lane, (mask >> lane) & 1ull, (uint32_t)ws, false, sm, wv, S);
-    }
-    if (A.tiles) {
-        // The bitmap is read in blocks of 8 words (64 bytes); in one pass a wave
-        // reads 8 blocks nw apart, so that a cluster of non-empty words (frontiers
-        // cluster by vertex id) spreads over many waves.  The set slots of the 64
-        // words are then taken 64 at a time, in word order.
-        const uint32_t nblk = (A.nwords + 7) / 8;
-        for (uint32_t pb = 0; pb < nblk; pb += nw * 8) {
-            const uint32_t blk = pb + (lane >> 3) * nw + gw;
-            const uint32_t wi = blk * 8 + (lane & 7);
-            uint64_t word = 0;
-            if (blk < nblk && wi < A.nwords) {
-                if (A.identity) {
-                    const uint32_t lo = wi * 64;
-                    word = A.nslots - lo >= 64 ? ~0ull : ((1ull << (A.nslots - lo)) - 1ull);
-                } else {
-                    word = A.in_bm[wi];
-                    if (word && A.consume) A.in_bm[wi] = 0;
-                }
-            }
-            const uint32_t cnt = (uint32_t)__popcll(word);
-            uint32_t incl = cnt;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const uint32_t y = __shfl_up(incl, d);
-                if ((int)lane >= d) incl += y;
-            }
-            const uint32_t T = __shfl(incl, 63);
-            for (uint32_t r0 = 0; r0 < T; r0 += 64) {
-                const uint32_t q = r0 + lane;  // rank of this lane's slot
-                uint32_t o = 0;                // owner lane: the number of lanes with incl <= q
-#pragma unroll
-                for (uint32_t st = 32; st > 0; st >>= 1) {
-                    const uint32_t v = (uint32_t)__shfl((int)incl, (int)(o + st - 1));
-                    if (v <= q) o += st;
-                }
-                o = min(o, 63u);
-                const uint64_t ow = __shfl(word, (int)o);
-                const uint32_t oex = (uint32_t)__shfl((int)(incl - cnt), (int)o);
-                const uint32_t owi = (uint32_t)__shfl((int)wi, (int)o);
-                const bool valid = q < T;
-                const uint32_t os = valid ? owi * 64 + select_bit(ow, q - oex) : 0u;
-                do_slots<MODE, EPT>(A, os, valid, 0u, true, sm, wv, S);
-            }
-        }
     }
     const uint32_t marks = wave_sum(S.marks);
     const bool any = __ballot(S.any) != 0;
@@ -535,23 +540,47 @@ __global__ __launch_bounds__(256) void k_reduce_stats(Counters *c, unsigned long
 }
 
 // ---------------------------------------------------------------------------
-// Insertion of a batch into the resident CSR (delta rows in a pool).  Pool
-// space is carved deterministically: per-word sizes, one scan, per-word bases.
+// Insertion of a batch into the resident CSR (delta blocks in a pool).  Most
+// copies land in spare capacity of their row's block; only rows that overflow
+// get a new block, carved deterministically: per-word sizes, one scan,
+// per-word bases.
 // ---------------------------------------------------------------------------
 
-// Counts new entries per endpoint; the first touch of a vertex marks it.
-__global__ __launch_bounds__(RB) void k_ins_count(const int4 *rec, int64_t k, uint32_t *tcnt, uint64_t *slot,
+// Copy s of insertion record i: row a (s = 0) or row b (s = 1); a self-loop's
+// two copies both go to row a.
+__device__ __forceinline__ bool ins_copy(const int4 *rec, int64_t t, uint32_t &row, uint32_t &col, uint32_t &w)
+{
+    const int4 r = rec[t >> 1];
+    if (r.w != 1) return false;
+    const bool side = t & 1;
+    row = (uint32_t)(side && r.x != r.y ? r.y : r.x);
+    col = (uint32_t)(side ? r.x : r.y);
+    w = (uint32_t)r.z;
+    return true;
+}
+
+// Reserves a position with a returning atomicAdd on the row's delta length;
+// with room in the block the copy is written and entered in the edge map, else
+// it waits for the block to grow (the thread that reserved position == capacity
+// marks the row).
+__global__ __launch_bounds__(RB) void k_ins_place(const int4 *rec, int64_t k, GView g, uint32_t *tslot,
                                                   uint64_t *tbm)
 {
-    const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
-    if (i >= k) return;
-    const int4 r = rec[i];
-    if (r.w != 1) return;
-    const uint32_t ra = atomicAdd(&tcnt[r.x], 1u);
-    const uint32_t rb = atomicAdd(&tcnt[r.y], 1u);
-    if (ra == 0) mark(tbm, (uint32_t)r.x);
-    if (rb == 0) mark(tbm, (uint32_t)r.y);
-    slot[i] = ((uint64_t)ra << 32) | rb;
+    const int64_t t = (int64_t)blockIdx.x * RB + threadIdx.x;
+    uint32_t row, col, w;
+    if (t >= 2 * k || !ins_copy(rec, t, row, col, w)) return;
+    VRec *r = &g.vr[row];
+    const uint32_t cap = r->dcap, blen = r->blen;  // not written by this kernel
+    const uint64_t o = atomicAdd((unsigned long long *)&r->dbl, 1ull);
+    const uint32_t pos = (uint32_t)o;
+    if (pos < cap) {
+        g.pool[(o >> 32) + pos] = pack_entry(col, w);
+        emap_insert(g, row, col, w, blen + pos);
+        tslot[t] = pos;
+    } else {
+        tslot[t] = pos | SLOT_PENDING;
+        if (pos == cap) mark(tbm, row);
+    }
 }
 
 // Exclusive prefix sum over the 256 threads of a workgroup; *total gets the sum.
@@ -584,12 +613,10 @@ __device__ __forceinline__ uint32_t grown_cap(uint32_t cap, uint32_t need)
     return need <= cap ? cap : max(max(2 * cap, need), 4u);
 }
 
-// wsum[w] = new pool entries the touched vertices of bitmap word w need (a new
-// block for each vertex whose block overflows); bsum[b] = the sum over the 256
-// words of workgroup b.
-__global__ __launch_bounds__(RB) void k_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const uint64_t *dbl,
-                                                    const uint32_t *dcap, const uint32_t *tcnt, uint32_t *wsum,
-                                                    uint64_t *bsum)
+// wsum[w] = pool entries for the new blocks of the rows marked in bitmap word w;
+// bsum[b] = the sum over the 256 words of workgroup b.
+__global__ __launch_bounds__(RB) void k_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const VRec *vr,
+                                                    uint32_t *wsum, uint64_t *bsum)
 {
     const uint32_t w = blockIdx.x * RB + threadIdx.x;
     uint32_t sum = 0;
@@ -598,8 +625,7 @@ __global__ __launch_bounds__(RB) void k_ins_wordsum(const uint64_t *tbm, uint32_
         while (word) {
             const uint32_t v = w * 64 + (uint32_t)__builtin_ctzll(word);
             word &= word - 1;
-            const uint32_t cap = dcap[v], need = (uint32_t)dbl[v] + tcnt[v];
-            if (need > cap) sum += grown_cap(cap, need);
+            sum += grown_cap(vr[v].dcap, (uint32_t)vr[v].dbl);
         }
         wsum[w] = sum;
     }
@@ -643,75 +669,48 @@ __global__ __launch_bounds__(1024) void k_ins_scan(const uint64_t *bsum, uint32_
     }
 }
 
-// Gives each touched vertex room for its new entries: in place when its block
-// has the capacity, else a new block (old entries moved first).  Leaves the old
-// length in tcnt as the write base of the new entries.  Word w's new blocks
-// start at its workgroup's base plus the workgroup-local prefix of wsum.
-__global__ __launch_bounds__(RB) void k_ins_alloc(const uint64_t *tbm, uint32_t nwords, const uint32_t *wsum,
-                                                  const uint64_t *bbase, uint64_t *dbl, uint32_t *dcap,
-                                                  uint32_t *tcnt, uint64_t *pool, const GraphCtl *gc)
+// Moves each marked row's block (its entries written so far) to a new block;
+// the length already counts the pending copies.  Clears the marks.  Word w's
+// new blocks start at its workgroup's base plus the workgroup-local prefix of
+// wsum.
+__global__ __launch_bounds__(RB) void k_ins_alloc(uint64_t *tbm, uint32_t nwords, const uint32_t *wsum,
+                                                  const uint64_t *bbase, VRec *vr, uint64_t *pool, const GraphCtl *gc)
 {
     const uint32_t w = blockIdx.x * RB + threadIdx.x;
     const uint32_t mine = w < nwords ? wsum[w] : 0u;
     uint64_t total;
     uint64_t nb = bbase[blockIdx.x] + block_exclusive_scan(mine, &total);
-    if (w >= nwords || (gc->err & ERR_POOL)) return;
+    if (w >= nwords || !mine) return;
     uint64_t word = tbm[w];
+    tbm[w] = 0;
+    if (gc->err & ERR_POOL) return;
     while (word) {
         const uint32_t v = w * 64 + (uint32_t)__builtin_ctzll(word);
         word &= word - 1;
-        const uint64_t d = dbl[v];
+        VRec *r = &vr[v];
+        const uint64_t d = r->dbl;
+        const uint32_t len = (uint32_t)d, cap = r->dcap;
         const uint64_t beg = d >> 32;
-        const uint32_t len = (uint32_t)d, add = tcnt[v], cap = dcap[v];
-        if (len + add <= cap) {
-            dbl[v] = (beg << 32) | (uint64_t)(len + add);
-        } else {
-            const uint32_t nc = grown_cap(cap, len + add);
-            for (uint32_t j = 0; j < len; ++j) pool[nb + j] = pool[beg + j];
-            dbl[v] = (nb << 32) | (uint64_t)(len + add);
-            dcap[v] = nc;
-            nb += nc;
-        }
-        tcnt[v] = len;
+        for (uint32_t j = 0; j < cap; ++j) pool[nb + j] = pool[beg + j];
+        r->dbl = (nb << 32) | len;
+        r->dcap = grown_cap(cap, len);
+        nb += r->dcap;
     }
 }
 
-// Writes both directed entries of each insertion and enters them into the edge
-// map (position in row = base length + index in the delta block).
-__global__ __launch_bounds__(RB) void k_ins_scatter(const int4 *rec, int64_t k, const uint64_t *slot, GView g,
-                                                    const uint32_t *tcnt, uint64_t *pos, const GraphCtl *gc)
+// Writes the copies that waited for their row's new block.
+__global__ __launch_bounds__(RB) void k_ins_pending(const int4 *rec, int64_t k, GView g, uint32_t *tslot,
+                                                    const GraphCtl *gc)
 {
-    const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
-    if (i >= k) return;
-    const int4 r = rec[i];
-    if (r.w != 1) return;
-    if (gc->err & ERR_POOL) {  // reported to the host; nothing was placed
-        pos[i] = ~0ull;
-        return;
-    }
-    const uint32_t a = (uint32_t)r.x, b = (uint32_t)r.y, w = (uint32_t)r.z;
-    const uint64_t s = slot[i];
-    const uint32_t ia = tcnt[a] + (uint32_t)(s >> 32), ib = tcnt[b] + (uint32_t)s;
-    const uint64_t pa = (g.dbl[a] >> 32) + ia;
-    const uint64_t pb = (g.dbl[b] >> 32) + ib;
-    g.pool[pa] = pack_entry(b, w);
-    g.pool[pb] = pack_entry(a, w);
-    pos[i] = (pa << 32) | pb;
-    emap_insert(g, a, b, w, (uint32_t)(g.rp[a + 1] - g.rp[a]) + ia);
-    emap_insert(g, b, a, w, (uint32_t)(g.rp[b + 1] - g.rp[b]) + ib);
-}
-
-__global__ __launch_bounds__(RB) void k_ins_reset(uint64_t *tbm, uint32_t nwords, uint32_t *tcnt)
-{
-    const uint32_t w = blockIdx.x * RB + threadIdx.x;
-    if (w >= nwords) return;
-    uint64_t word = tbm[w];
-    if (!word) return;
-    tbm[w] = 0;
-    while (word) {
-        tcnt[w * 64 + (uint32_t)__builtin_ctzll(word)] = 0;
-        word &= word - 1;
-    }
+    const int64_t t = (int64_t)blockIdx.x * RB + threadIdx.x;
+    uint32_t row, col, w;
+    if (t >= 2 * k || !(tslot[t] & SLOT_PENDING) || !ins_copy(rec, t, row, col, w)) return;
+    if (gc->err & ERR_POOL) return;  // reported to the host; nothing was placed
+    const uint32_t pos = tslot[t] & ~SLOT_PENDING;
+    const VRec *r = &g.vr[row];
+    g.pool[(r->dbl >> 32) + pos] = pack_entry(col, w);
+    emap_insert(g, row, col, w, r->blen + pos);
+    tslot[t] = pos;
 }
 
 // ---------------------------------------------------------------------------
@@ -772,9 +771,9 @@ __global__ __launch_bounds__(RB) void k_classify(const int4 *rec, int64_t k, uin
 
 // Offers every inserted edge that survived the batch's deletions in both
 // directions (the role of insertEdge, gpuFunctions_undir.cuh:217-251).
-__global__ __launch_bounds__(RB) void k_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint64_t *pos,
-                                                  const uint64_t *pool, uint64_t *st, uint64_t *out_bm,
-                                                  uint32_t *out_act, Counters *c)
+__global__ __launch_bounds__(RB) void k_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint32_t *tslot,
+                                                  GView g, uint64_t *st, uint64_t *out_bm, uint32_t *out_act,
+                                                  Counters *c)
 {
     const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
     const int64_t ri = i >> kshift;
@@ -782,7 +781,10 @@ __global__ __launch_bounds__(RB) void k_ins_offer(const int4 *rec, int64_t k, ui
     bool hit = false;
     if (ri < k) {
         const int4 r = rec[ri];
-        if (r.w == 1 && r.x != r.y && pos[ri] != ~0ull && !(pool[pos[ri] >> 32] & E_DEAD)) {
+        const uint32_t ta = tslot[2 * ri];
+        // the copy in row a is tombstoned when a deletion of the batch removed it
+        if (r.w == 1 && r.x != r.y && !(ta & SLOT_PENDING) &&
+            !(g.pool[(g.vr[r.x].dbl >> 32) + ta] & E_DEAD)) {
             const uint32_t a = (uint32_t)r.x, b = (uint32_t)r.y;
             const uint32_t sa_slot = (a << kshift) | tree, sb_slot = (b << kshift) | tree;
             const uint32_t w = (uint32_t)r.z;
@@ -928,17 +930,16 @@ void launch_reduce_stats(Counters *c, unsigned long long *bstats, hipStream_t s)
     hipLaunchKernelGGL(k_reduce_stats, dim3(1), dim3(256), 0, s, c, bstats);
 }
 
-void launch_ins_count(const int4 *rec, int64_t k, uint32_t *tcnt, uint64_t *slot, uint64_t *tbm, hipStream_t s)
+void launch_ins_place(const int4 *rec, int64_t k, GView g, uint32_t *tslot, uint64_t *tbm, hipStream_t s)
 {
-    if (k > 0) hipLaunchKernelGGL(k_ins_count, dim3(blocks_for(k)), dim3(RB), 0, s, rec, k, tcnt, slot, tbm);
+    if (k > 0) hipLaunchKernelGGL(k_ins_place, dim3(blocks_for(2 * k)), dim3(RB), 0, s, rec, k, g, tslot, tbm);
 }
 
-void launch_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const uint64_t *dbl, const uint32_t *dcap,
-                        const uint32_t *tcnt, uint32_t *wsum, uint64_t *bsum, hipStream_t s)
+void launch_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const VRec *vr, uint32_t *wsum, uint64_t *bsum,
+                        hipStream_t s)
 {
     if (nwords > 0)
-        hipLaunchKernelGGL(k_ins_wordsum, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, dbl, dcap, tcnt, wsum,
-                           bsum);
+        hipLaunchKernelGGL(k_ins_wordsum, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, vr, wsum, bsum);
 }
 
 void launch_ins_scan(const uint64_t *bsum, uint32_t nwords, uint64_t *bbase, uint64_t pool_cap, GraphCtl *gc,
@@ -947,24 +948,17 @@ void launch_ins_scan(const uint64_t *bsum, uint32_t nwords, uint64_t *bbase, uin
     hipLaunchKernelGGL(k_ins_scan, dim3(1), dim3(1024), 0, s, bsum, blocks_for(nwords), bbase, pool_cap, gc);
 }
 
-void launch_ins_alloc(const uint64_t *tbm, uint32_t nwords, const uint32_t *wsum, const uint64_t *bbase,
-                      uint64_t *dbl, uint32_t *dcap, uint32_t *tcnt, uint64_t *pool, const GraphCtl *gc,
-                      hipStream_t s)
+void launch_ins_alloc(uint64_t *tbm, uint32_t nwords, const uint32_t *wsum, const uint64_t *bbase, VRec *vr,
+                      uint64_t *pool, const GraphCtl *gc, hipStream_t s)
 {
     if (nwords > 0)
-        hipLaunchKernelGGL(k_ins_alloc, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, wsum, bbase, dbl, dcap,
-                           tcnt, pool, gc);
+        hipLaunchKernelGGL(k_ins_alloc, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, wsum, bbase, vr, pool,
+                           gc);
 }
 
-void launch_ins_scatter(const int4 *rec, int64_t k, const uint64_t *slot, GView g, const uint32_t *tcnt,
-                        uint64_t *pos, const GraphCtl *gc, hipStream_t s)
+void launch_ins_pending(const int4 *rec, int64_t k, GView g, uint32_t *tslot, const GraphCtl *gc, hipStream_t s)
 {
-    if (k > 0) hipLaunchKernelGGL(k_ins_scatter, dim3(blocks_for(k)), dim3(RB), 0, s, rec, k, slot, g, tcnt, pos, gc);
-}
-
-void launch_ins_reset(uint64_t *tbm, uint32_t nwords, uint32_t *tcnt, hipStream_t s)
-{
-    if (nwords > 0) hipLaunchKernelGGL(k_ins_reset, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, tcnt);
+    if (k > 0) hipLaunchKernelGGL(k_ins_pending, dim3(blocks_for(2 * k)), dim3(RB), 0, s, rec, k, g, tslot, gc);
 }
 
 void launch_classify(const int4 *rec, int64_t k, uint32_t kshift, uint64_t *st, uint64_t *out_bm, uint64_t *inv_bm,
@@ -980,11 +974,11 @@ void launch_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats
     if (k > 0) hipLaunchKernelGGL(k_tomb, dim3(blocks_for(2 * k)), dim3(RB), 0, s, rec, k, g, bstats);
 }
 
-void launch_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint64_t *pos, const uint64_t *pool,
-                      uint64_t *st, uint64_t *out_bm, uint32_t *out_act, Counters *c, hipStream_t s)
+void launch_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint32_t *tslot, GView g, uint64_t *st,
+                      uint64_t *out_bm, uint32_t *out_act, Counters *c, hipStream_t s)
 {
     if (k > 0)
-        hipLaunchKernelGGL(k_ins_offer, dim3(blocks_for(k << kshift)), dim3(RB), 0, s, rec, k, kshift, pos, pool, st,
+        hipLaunchKernelGGL(k_ins_offer, dim3(blocks_for(k << kshift)), dim3(RB), 0, s, rec, k, kshift, tslot, g, st,
                            out_bm, out_act, c);
 }
 

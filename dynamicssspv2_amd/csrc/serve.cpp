@@ -51,12 +51,12 @@ struct Request {
 };
 
 // Device bytes of an instance: CSR, delta descriptors/capacities/counters and
-// bitmaps, the initial delta pool (E/2 + 2^16 entries), the edge map, and the
+// bitmaps, the initial delta pool (E/2 + 12 n + 2^16 entries), the edge map, and the
 // tree (state, bitmaps, chunk lists).
 uint64_t instance_bytes(int64_t n, int64_t m)
 {
     const uint64_t E = 2 * (uint64_t)m, N = (uint64_t)n;
-    return 8 * (N + 1) + 8 * E + 16 * N + 3 * N / 8 + 8 * (E / 2 + (1u << 16)) + 8 * emap_slots_for(E + E / 8) +
+    return 8 * (N + 1) + 8 * E + 16 * N + 3 * N / 8 + 8 * (E / 2 + 12 * N + (1u << 16)) + 8 * emap_slots_for(E + E / 8) +
            8 * N + 3 * N / 8 + 16 * (E / 128 + 1024);
 }
 
