@@ -62,7 +62,7 @@ def parse():
     p.add_argument("--no-timing", action="store_true", help="no per-launch events (roofline unmeasured)")
     p.add_argument("--copy-probe", action="store_true",
                    help="only copy 1 GiB device-to-device five times (calibrates HBM byte counters), then exit")
-    for knob in ("grid", "heavy_degree", "chunk", "round_batch", "use_graph", "edges_per_thread"):
+    for knob in ("grid", "heavy_degree", "chunk", "round_batch", "use_graph", "edges_per_thread", "tile_entries"):
         p.add_argument("--" + knob.replace("_", "-"), type=int, default=None, help="tuning knob (dsssp_tuning)")
     return p.parse_args()
 
@@ -112,19 +112,39 @@ def cpu_check(ds, np, g, n, srcs, par, dist, k, label, run_cpu=True):
     return check, cpu
 
 
-def roofline(stats):
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_frontier_relax_traffic_rmat_s24_k1e6.md")
+
+
+def measured_traffic():
+    """HBM bytes per relaxation dispatch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE runs of this workload, copy-calibrated)."""
+    try:
+        for ln in open(TRAFFIC_FILE):
+            if ln.startswith("machine-readable:"):
+                return json.loads(ln.split(":", 1)[1])["hbm_bytes_per_dispatch"]
+    except (OSError, ValueError, KeyError):
+        pass
+    return None
+
+
+def roofline(stats, traffic_applies=False):
     relax_bytes = sum(s["alg_bytes_relax"] for s in stats)
     relax_ms = sum(s["t_relax_kernel_ms"] for s in stats)
     launches = sum(s["relax_launches"] for s in stats)
     achieved = relax_bytes / (relax_ms / 1e3) / 1e9 if relax_ms > 0 else None
-    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": (achieved / HBM_PEAK_GBPS) if achieved else None, "traffic": None,
-            "kernel": "k_frontier<M_RELAX>", "launches": launches,
-            "avg_launch_us": relax_ms / max(1, launches) * 1e3,
-            "alg_bytes_per_launch": relax_bytes / max(1, launches)}
+    traffic = measured_traffic() if traffic_applies else None
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+           "frac": (achieved / HBM_PEAK_GBPS) if achieved else None, "traffic": traffic,
+           "kernel": "k_frontier<M_RELAX>", "launches": launches,
+           "avg_launch_us": relax_ms / max(1, launches) * 1e3,
+           "alg_bytes_per_launch": relax_bytes / max(1, launches)}
+    if traffic:
+        out["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
+        out["traffic_GBps"] = traffic * launches / (relax_ms / 1e3) / 1e9 if relax_ms > 0 else None
+    return out
 
 
-def line(args, value, wall, steps, cfg, stats, check, cpu, unit="changed edges/s", extra=None):
+def line(args, value, wall, steps, cfg, stats, check, cpu, unit="changed edges/s", extra=None, traffic=False):
     import numpy as np
     lat = [s["t_update_ms"] for s in stats]
     alg = sum(s["alg_bytes"] for s in stats)
@@ -136,7 +156,7 @@ def line(args, value, wall, steps, cfg, stats, check, cpu, unit="changed edges/s
         "update_alg_GBps": alg / (sum(lat) / 1e3) / 1e9 if sum(lat) > 0 else None,
         "dist_bitexact": bool(check.get("gpu_recompute_equal", True)
                               and check.get("dijkstra_mismatches", 0) == 0 and check.get("certificate", 0) == 0),
-        "check": check, "roofline": roofline(stats), "cpu_baseline": cpu,
+        "check": check, "roofline": roofline(stats, traffic), "cpu_baseline": cpu,
         "phases_ms": {ph: sum(s[f"t_{ph}_ms"] for s in stats) / max(1, len(stats))
                       for ph in ("insert", "classify", "invalidate", "seed", "relax")},
         "rounds": {"invalidate": sum(s["inv_rounds"] for s in stats) / max(1, len(stats)),
@@ -160,7 +180,7 @@ def timed_steps(ds, g, t, res, warmup, timing=True):
     return stats, wall
 
 
-def run_plain(args, ds, np, n, a, b, w, src, batches, label, cfg):
+def run_plain(args, ds, np, n, a, b, w, src, batches, label, cfg, traffic=False, extra_batches=None):
     g = ds.Graph(n, a, b, w)
     t = ds.Tree(g, src)
     res = [ds.Batch(g, c) for c in batches]
@@ -171,6 +191,17 @@ def run_plain(args, ds, np, n, a, b, w, src, batches, label, cfg):
     value = k * args.steps / wall
     log(f"{label}: {args.steps} steps in {wall:.3f}s -> {value:.3e} changed edges/s, "
         f"p50 {pct([s['t_update_ms'] for s in stats], 50):.3f} ms")
+    extra = {}
+    if extra_batches:
+        # The same workload continued without per-launch timing events, whose
+        # dispatch-side recording adds a few microseconds to every launch.
+        res = [ds.Batch(g, c) for c in extra_batches]
+        st2, wall2 = timed_steps(ds, g, t, res, 0, False)
+        for r in res:
+            r.close()
+        extra["value_without_launch_events"] = k * len(extra_batches) / wall2
+        extra["p50_ms_without_launch_events"] = pct([s["t_update_ms"] for s in st2], 50)
+        log(f"{label}: without launch events {extra['value_without_launch_events']:.3e} changed edges/s")
     par, dist = t.read()
     fpar, fdist = ds.Tree(g, src).read()
     check = {"gpu_recompute_equal": bool(np.array_equal(dist, fdist) and np.array_equal(par, fpar))}
@@ -178,7 +209,7 @@ def run_plain(args, ds, np, n, a, b, w, src, batches, label, cfg):
     if not args.no_cpu:
         c2, cpu = cpu_check(ds, np, g, n, [src], par, dist, k, label)
         check.update(c2)
-    line(args, value, wall, args.steps, cfg, stats, check, cpu)
+    line(args, value, wall, args.steps, cfg, stats, check, cpu, extra=extra, traffic=traffic)
 
 
 def config1(args, ds, np):
@@ -198,12 +229,16 @@ def config2(args, ds, np):
     src = int(np.argmax(deg))  # ties -> smallest id
     log(f"rmat scale={args.scale} n={n} m={len(a)} max_deg={deg.max()} src={src} ({time.time() - t0:.1f}s)")
     t0 = time.time()
-    batches = disjoint_batches(ds, np, n, a, b, w, args.k, args.ins, args.warmup + args.steps, args.seed + 1)
-    log(f"{args.warmup + args.steps} batches of k={args.k} generated ({time.time() - t0:.1f}s)")
+    nb = args.warmup + 2 * args.steps if not args.no_timing else args.warmup + args.steps
+    allb = disjoint_batches(ds, np, n, a, b, w, args.k, args.ins, nb, args.seed + 1)
+    batches, extra_batches = allb[:args.warmup + args.steps], allb[args.warmup + args.steps:]
+    log(f"{nb} batches of k={args.k} generated ({time.time() - t0:.1f}s)")
     cfg = {"workload": f"rmat-s{args.scale}-ef{args.edge_factor} batch k={args.k} ins={args.ins:g}",
            "scale": args.scale, "n": n, "m": len(a), "k": args.k, "ins_frac": args.ins, "src": src,
            "weights": "U[1,1000]"}
-    run_plain(args, ds, np, n, a, b, w, src, batches, f"rmat-s{args.scale}", cfg)
+    default = args.scale == 24 and args.edge_factor == 16 and args.k == 1_000_000 and args.ins == 0.5
+    run_plain(args, ds, np, n, a, b, w, src, batches, f"rmat-s{args.scale}", cfg, traffic=default,
+              extra_batches=extra_batches)
 
 
 def config3(args, ds, np):
@@ -217,7 +252,8 @@ def config3(args, ds, np):
     batches = [ds.gen_changes_grid(side, a, b, w, par0, k, 0.7, 0.5, 3, 1, 10000, seed=args.seed + i)
                for i in range(args.warmup + args.steps)]
     # every step starts from the same graph and tree (restored outside the timing)
-    g, t = g0.clone(), ds.Tree(g0, 0)
+    g = g0.clone()
+    t = ds.Tree(g, parent=par0, dist=t0.read()[1])  # a tree on the clone; its state is restored per step
     stats = []
     for i, c in enumerate(batches):
         g.copy_from(g0)
@@ -300,7 +336,8 @@ def config4b(args, ds, np):
     roots = np.array([p[5] + p[6] for p in parts], np.int32)
     g0 = ds.Graph(N, A, B, W)
     t0 = ds.Tree(g0, roots=roots)
-    g, t = g0.clone(), ds.Tree(g0, roots=roots)
+    g = g0.clone()
+    t = ds.Tree(g, roots=roots)  # a tree on the clone; its state is restored per step
     stats = []
     for i in range(args.warmup + args.steps):
         g.copy_from(g0)
@@ -431,7 +468,7 @@ def main():
     import dynamicssspv2_amd as ds
     ds.set_device(0)
     knobs = {k: getattr(args, k) for k in ("grid", "heavy_degree", "chunk", "round_batch", "use_graph",
-                                           "edges_per_thread") if getattr(args, k) is not None}
+                                           "edges_per_thread", "tile_entries") if getattr(args, k) is not None}
     if knobs:
         ds.set_tuning(**knobs)
     log(f"tuning {ds.get_tuning()}")

@@ -302,11 +302,8 @@ int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms)
                 g->kev.resize((size_t)(2 * group));
                 for (size_t j = old; j < g->kev.size(); ++j) HIP_TRY(hipEventCreate(&g->kev[j]));
             }
-            for (int j = 0; j < group; ++j) {
-                HIP_TRY(hipEventRecord(g->kev[2 * j], s));
-                launch_frontier(mode, round_args(t, base, i + j), grid, s);
-                HIP_TRY(hipEventRecord(g->kev[2 * j + 1], s));
-            }
+            for (int j = 0; j < group; ++j)
+                launch_frontier(mode, round_args(t, base, i + j), grid, s, g->kev[2 * j], g->kev[2 * j + 1]);
         } else if (tu.use_graph) {
             hipGraphExec_t exec = nullptr;
             if (int e = captured_rounds(t, mode, i, group, grid, base, &exec)) return e;

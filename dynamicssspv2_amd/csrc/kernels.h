@@ -63,7 +63,10 @@ struct FArgs {
 
 constexpr int FRONTIER_BLOCK = 256;
 
-void launch_frontier(int mode, const FArgs &a, int grid, hipStream_t s);
+// With start/stop events, they are recorded by the dispatch itself
+// (hipExtLaunchKernel), bracketing exactly this kernel.
+void launch_frontier(int mode, const FArgs &a, int grid, hipStream_t s, hipEvent_t start = nullptr,
+                     hipEvent_t stop = nullptr);
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 void launch_reduce_stats(Counters *c, unsigned long long *bstats, hipStream_t s);

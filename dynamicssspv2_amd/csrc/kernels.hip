@@ -27,6 +27,7 @@
 // The reference does one thread per frontier vertex with unsynchronised
 // read-modify-writes (gpuFunctions_undir.cuh:370-433) and one full O(n) filter
 // pass over 16-byte records per round (:125-174); none of that is reused here.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -570,12 +571,11 @@ __global__ __launch_bounds__(RB) void k_ins_place(const int4 *rec, int64_t k, GV
     uint32_t row, col, w;
     if (t >= 2 * k || !ins_copy(rec, t, row, col, w)) return;
     VRec *r = &g.vr[row];
-    const uint32_t cap = r->dcap, blen = r->blen;  // not written by this kernel
     const uint64_t o = atomicAdd((unsigned long long *)&r->dbl, 1ull);
-    const uint32_t pos = (uint32_t)o;
+    const uint32_t pos = (uint32_t)o, cap = r->dcap;
     if (pos < cap) {
         g.pool[(o >> 32) + pos] = pack_entry(col, w);
-        emap_insert(g, row, col, w, blen + pos);
+        emap_insert(g, row, col, w, r->blen + pos);
         tslot[t] = pos;
     } else {
         tslot[t] = pos | SLOT_PENDING;
@@ -882,17 +882,26 @@ __global__ __launch_bounds__(RB) void k_apply_best(uint32_t nslots, uint32_t ksh
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + RB - 1) / RB); }
 
+template <int MODE, int EPT>
+void launch_mode(const FArgs &a, int grid, hipStream_t s, hipEvent_t start, hipEvent_t stop)
+{
+    if (start || stop)
+        hipExtLaunchKernelGGL((k_frontier<MODE, EPT>), dim3(grid), dim3(FB), 0, s, start, stop, 0, a);
+    else
+        hipLaunchKernelGGL((k_frontier<MODE, EPT>), dim3(grid), dim3(FB), 0, s, a);
+}
+
 template <int EPT>
-void launch_frontier_ept(int mode, const FArgs &a, int grid, hipStream_t s)
+void launch_frontier_ept(int mode, const FArgs &a, int grid, hipStream_t s, hipEvent_t start, hipEvent_t stop)
 {
     switch (mode) {
-    case M_RELAX: hipLaunchKernelGGL((k_frontier<M_RELAX, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
-    case M_INVAL: hipLaunchKernelGGL((k_frontier<M_INVAL, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
-    case M_SEED: hipLaunchKernelGGL((k_frontier<M_SEED, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
-    case M_TBFS: hipLaunchKernelGGL((k_frontier<M_TBFS, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
-    case M_PICK: hipLaunchKernelGGL((k_frontier<M_PICK, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
-    case M_OFCHECK: hipLaunchKernelGGL((k_frontier<M_OFCHECK, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_frontier<M_EMAP, EPT>), dim3(grid), dim3(FB), 0, s, a); break;
+    case M_RELAX: launch_mode<M_RELAX, EPT>(a, grid, s, start, stop); break;
+    case M_INVAL: launch_mode<M_INVAL, EPT>(a, grid, s, start, stop); break;
+    case M_SEED: launch_mode<M_SEED, EPT>(a, grid, s, start, stop); break;
+    case M_TBFS: launch_mode<M_TBFS, EPT>(a, grid, s, start, stop); break;
+    case M_PICK: launch_mode<M_PICK, EPT>(a, grid, s, start, stop); break;
+    case M_OFCHECK: launch_mode<M_OFCHECK, EPT>(a, grid, s, start, stop); break;
+    default: launch_mode<M_EMAP, EPT>(a, grid, s, start, stop); break;
     }
 }
 
@@ -902,12 +911,12 @@ void launch_frontier_ept(int mode, const FArgs &a, int grid, hipStream_t s)
 // Host launchers.
 // ---------------------------------------------------------------------------
 
-void launch_frontier(int mode, const FArgs &a, int grid, hipStream_t s)
+void launch_frontier(int mode, const FArgs &a, int grid, hipStream_t s, hipEvent_t start, hipEvent_t stop)
 {
     switch (a.ept) {
-    case 1: launch_frontier_ept<1>(mode, a, grid, s); break;
-    case 2: launch_frontier_ept<2>(mode, a, grid, s); break;
-    default: launch_frontier_ept<4>(mode, a, grid, s); break;
+    case 1: launch_frontier_ept<1>(mode, a, grid, s, start, stop); break;
+    case 2: launch_frontier_ept<2>(mode, a, grid, s, start, stop); break;
+    default: launch_frontier_ept<4>(mode, a, grid, s, start, stop); break;
     }
 }
 
