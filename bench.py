@@ -62,7 +62,8 @@ def parse():
     p.add_argument("--no-timing", action="store_true", help="no per-launch events (roofline unmeasured)")
     p.add_argument("--copy-probe", action="store_true",
                    help="only copy 1 GiB device-to-device five times (calibrates HBM byte counters), then exit")
-    for knob in ("grid", "heavy_degree", "chunk", "round_batch", "use_graph", "edges_per_thread", "tile_entries"):
+    for knob in ("grid", "heavy_degree", "chunk", "round_batch", "use_graph", "edges_per_thread", "tile_entries",
+                 "local_tiles"):
         p.add_argument("--" + knob.replace("_", "-"), type=int, default=None, help="tuning knob (dsssp_tuning)")
     return p.parse_args()
 
@@ -468,7 +469,8 @@ def main():
     import dynamicssspv2_amd as ds
     ds.set_device(0)
     knobs = {k: getattr(args, k) for k in ("grid", "heavy_degree", "chunk", "round_batch", "use_graph",
-                                           "edges_per_thread", "tile_entries") if getattr(args, k) is not None}
+                                           "edges_per_thread", "tile_entries", "local_tiles")
+             if getattr(args, k) is not None}
     if knobs:
         ds.set_tuning(**knobs)
     log(f"tuning {ds.get_tuning()}")

@@ -79,7 +79,7 @@ class ServerTotals(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [(f, ctypes.c_int32) for f in ("block", "grid", "heavy_degree", "chunk", "round_batch",
                                               "kernel_timing", "use_graph", "edges_per_thread",
-                                              "tile_entries")]
+                                              "tile_entries", "local_tiles")]
 
 
 def build(force: bool = True) -> None:

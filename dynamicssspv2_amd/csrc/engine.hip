@@ -41,7 +41,7 @@ using namespace dsssp;
 
 namespace {
 
-dsssp_tuning g_tuning = {FRONTIER_BLOCK, 2048, 256, 256, 4, 0, 0, 4, 1024};
+dsssp_tuning g_tuning = {FRONTIER_BLOCK, 2048, 256, 256, 4, 0, 0, 4, 1024, 4};
 std::mutex g_tuning_mu;
 thread_local int t_device = 0;
 
@@ -202,6 +202,7 @@ FArgs base_args(dsssp_tree *t)
     a.kshift = t->kshift;
     a.ept = (uint32_t)tu.edges_per_thread;
     a.budget = (uint32_t)tu.tile_entries;
+    a.local = tu.local_tiles > 0 ? (uint32_t)tu.local_tiles : 0u;
     return a;
 }
 
@@ -714,10 +715,12 @@ extern "C" int dsssp_set_tuning(const dsssp_tuning *t)
     if (t->round_batch) nt.round_batch = t->round_batch;
     if (t->edges_per_thread) nt.edges_per_thread = t->edges_per_thread;
     if (t->tile_entries) nt.tile_entries = t->tile_entries;
+    if (t->local_tiles) nt.local_tiles = t->local_tiles;
     nt.kernel_timing = t->kernel_timing;
     nt.use_graph = t->use_graph;
     if (nt.grid < 1 || nt.grid > 65536 || nt.heavy_degree < 64 || nt.chunk < 64 || nt.chunk % 64 ||
-        nt.round_batch < 1 || nt.round_batch > 64 || nt.tile_entries < 64 ||
+        nt.round_batch < 1 || nt.round_batch > 64 || nt.tile_entries < 64 || nt.local_tiles < -1 ||
+        nt.local_tiles > 1024 ||
         (nt.edges_per_thread != 1 && nt.edges_per_thread != 2 && nt.edges_per_thread != 4))
         return fail(DSSSP_ERR_ARG, "tuning out of range");
     g_tuning = nt;

@@ -59,6 +59,7 @@ struct FArgs {
     uint32_t kshift;        // log2 of the trees per vertex: slot = vertex << kshift | tree
     uint32_t ept;           // edges in flight per lane: 1, 2 or 4
     uint32_t budget;        // entries a fresh tile takes; later slots become items
+    uint32_t local;         // push modes: tiles of newly marked slots a wave visits itself
 };
 
 constexpr int FRONTIER_BLOCK = 256;

@@ -147,14 +147,17 @@ __device__ __forceinline__ Pre gather(const FArgs &A, uint32_t os, uint64_t ent)
     return p;
 }
 
-// Applies one edge; marks the slot to visit next.  A mark made on a state read
-// that was stale is harmless: that slot was changed in this round anyway.
-// The pull modes (SEED, PICK) only lower `pull`, the owner's candidate.
+// Applies one edge.  The push modes (RELAX, INVAL, TBFS) return the slot to
+// visit next (~0u: none); a slot found through a stale state read is harmless
+// to visit, it changed anyway.  The pull modes (SEED, PICK) only lower `pull`,
+// the owner's candidate.
+constexpr uint32_t NO_SLOT = 0xFFFFFFFFu;
+
 template <int MODE>
-__device__ __forceinline__ void apply(const FArgs &A, uint32_t os, uint32_t aux0, uint32_t aux1, uint64_t ent,
-                                      const Pre &p, WaveStats &S, uint64_t &pull)
+__device__ __forceinline__ uint32_t apply(const FArgs &A, uint32_t os, uint32_t aux0, uint32_t aux1, uint64_t ent,
+                                          const Pre &p, WaveStats &S, uint64_t &pull)
 {
-    if (p.skip) return;
+    if (p.skip) return NO_SLOT;
     const uint32_t u = os >> A.kshift;
     const uint32_t x = e_col(ent);
     const uint32_t w = e_w(ent);
@@ -163,51 +166,68 @@ __device__ __forceinline__ void apply(const FArgs &A, uint32_t os, uint32_t aux0
         const uint64_t nd = (uint64_t)aux0 + w;
         if (nd >= DIST_INF) {  // cannot beat a finite distance
             if (p.sx == ST_INF) atomicOr(&A.c->err, ERR_DIST_SUSPECT);
-            return;
+            return NO_SLOT;
         }
         const uint64_t cand = pack_state((uint32_t)nd, u);
         // A root (parent itself, distance 0) never changes: with zero weights a
         // candidate (0, smaller id) would otherwise take its place.
-        if (cand >= p.sx || st_parent(p.sx) == x) return;
+        if (cand >= p.sx || st_parent(p.sx) == x) return NO_SLOT;
         atomicMin((unsigned long long *)&A.st[xs], (unsigned long long)cand);
-        mark(A.out_bm, xs);
         S.marks++;
-        S.any = true;
+        return xs;
     } else if constexpr (MODE == M_INVAL) {
         // x is a child of u in the pre-update tree: reset it and descend.
         // Tombstoned entries are scanned too, so a child hanging off a deleted
         // parallel copy is still found.  Only ST_INF is ever written in this
         // phase, so a plain store is race-free.
-        if (p.sx == ST_INF || st_parent(p.sx) != u) return;
+        if (p.sx == ST_INF || st_parent(p.sx) != u) return NO_SLOT;
         A.st[xs] = ST_INF;
-        mark(A.out_bm, xs);
         mark(A.inv_bm, xs);
-        S.any = true;
+        return xs;
     } else if constexpr (MODE == M_SEED) {
         // u was invalidated: candidate from a still-valid neighbour, reduced per
         // owner by the caller (every lane of a row targets the same slot).
-        if (p.sx == ST_INF) return;
+        if (p.sx == ST_INF) return NO_SLOT;
         const uint64_t nd = (uint64_t)st_dist(p.sx) + w;
         if (nd >= DIST_INF) {
             atomicOr(&A.c->err, ERR_DIST_SUSPECT);  // the owner is unreachable so far
-            return;
+            return NO_SLOT;
         }
         pull = min(pull, pack_state((uint32_t)nd, x));
     } else if constexpr (MODE == M_TBFS) {
-        if (p.sx == ST_INF || st_parent(p.sx) == x) return;
-        if ((uint64_t)aux0 + w != (uint64_t)st_dist(p.sx)) return;
+        if (p.sx == ST_INF || st_parent(p.sx) == x) return NO_SLOT;
+        if ((uint64_t)aux0 + w != (uint64_t)st_dist(p.sx)) return NO_SLOT;
         const uint32_t nl = aux1 + 1;
-        if (nl >= p.lx) return;
+        if (nl >= p.lx) return NO_SLOT;
         atomicMin(&A.lvl[xs], nl);
-        mark(A.out_bm, xs);
-        S.any = true;
+        return xs;
     } else if constexpr (MODE == M_OFCHECK) {
         if (p.sx == ST_INF) atomicOr(&A.c->err, ERR_DIST_OVERFLOW);
     } else {  // M_PICK: os is the slot whose parent is being chosen
-        if (p.sx == ST_INF) return;
-        if ((uint64_t)st_dist(p.sx) + w != (uint64_t)aux0) return;
-        if (A.use_lvl && p.lx + 1 != aux1) return;
+        if (p.sx == ST_INF) return NO_SLOT;
+        if ((uint64_t)st_dist(p.sx) + w != (uint64_t)aux0) return NO_SLOT;
+        if (A.use_lvl && p.lx + 1 != aux1) return NO_SLOT;
         pull = min(pull, (uint64_t)x);
+    }
+    return NO_SLOT;
+}
+
+constexpr uint32_t LSTK = 256;  // slots a wave keeps to visit itself in the same launch
+
+// A slot to visit next: kept on the wave's LDS stack when local continuation is
+// on and there is room (it is then visited later in this launch, unmarked),
+// else marked in the output bitmap for the next round.  `top` is wave-uniform.
+__device__ __forceinline__ void visit_next(const FArgs &A, uint32_t tgt, uint32_t *stk, uint32_t &top, WaveStats &S)
+{
+    const unsigned long long m = __ballot(tgt != NO_SLOT);
+    if (!m) return;
+    const uint32_t cnt = (uint32_t)__popcll(m);
+    if (A.local && top + cnt <= LSTK) {
+        if (tgt != NO_SLOT) stk[top + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull))] = tgt;
+        top += cnt;
+    } else {
+        if (tgt != NO_SLOT) mark(A.out_bm, tgt);
+        S.any = true;
     }
 }
 
@@ -220,6 +240,7 @@ struct TileSmem {
     uint32_t dbeg[FW][64];
     uint64_t b0[FW][64];
     unsigned long long pull[FW][64];  // SEED / PICK: per-owner minimum candidate
+    uint32_t stk[FW][LSTK];           // push modes: slots to visit in this launch
 };
 
 constexpr bool is_pull(int mode) { return mode == M_SEED || mode == M_PICK; }
@@ -265,6 +286,17 @@ __device__ __forceinline__ bool owner_prep(const FArgs &A, uint32_t os, uint32_t
     }
 }
 
+// A reservation [base, base + n) of the item list that does not fit: the part
+// below the capacity is read by the next launch, so it gets empty items.
+__device__ __forceinline__ void void_items(const FArgs &A, uint32_t base, uint32_t n)
+{
+    const uint32_t end = min(base + n, A.hcap);
+    for (uint32_t i = base + lane_id(); i < end; i += 64) {
+        A.out_hq[2 * (uint64_t)i] = 0ull;
+        A.out_hq[2 * (uint64_t)i + 1] = 0ull;
+    }
+}
+
 // Work of a launch: the set slots of its input bitmap and the items left by the
 // previous launch.  Set slots are compacted across bitmap words into tiles of up
 // to 64 slots, one per lane; a tile's rows are concatenated into one range of
@@ -296,7 +328,7 @@ __device__ __forceinline__ uint32_t select_bit(uint64_t w, uint32_t k)
 // its other entries; an item covers A.chunk entries of its range.
 template <int MODE, int EPT>
 __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start, bool fresh, TileSmem &sm,
-                         uint32_t wv, WaveStats &S)
+                         uint32_t wv, WaveStats &S, uint32_t &top)
 {
     const uint32_t lane = lane_id();
     uint32_t deg = 0, blen = 0, dbeg = 0, aux0 = 0, aux1 = 0;
@@ -335,7 +367,16 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
             if (lane == 0) base = atomicAdd(A.out_hcnt, tot);
             base = __shfl(base, 0);
             if (base + tot > A.hcap) {
-                if (lane == 0) atomicOr(&A.c->err, ERR_QUEUE);
+                // Push modes may visit a slot more than once per launch (its own
+                // continuation): the list can fill; the slot is then visited in
+                // the next round instead.  Pull modes visit each slot once.
+                void_items(A, base, tot);
+                if constexpr (MODE == M_RELAX || MODE == M_INVAL || MODE == M_TBFS) {
+                    if (defer) mark(A.out_bm, os);
+                    S.any = true;
+                } else {
+                    if (lane == 0) atomicOr(&A.c->err, ERR_QUEUE);
+                }
             } else {
                 const uint64_t it0 = base + ic - nch;
                 for (uint32_t c = 0; c < nch; ++c) {
@@ -403,10 +444,12 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
 #pragma unroll
         for (int j = 0; j < EPT; ++j) {
             uint64_t pull = ~0ull;
-            apply<MODE>(A, ows[j], a0[j], a1[j], ent[j], pre[j], S, pull);
+            const uint32_t tgt = apply<MODE>(A, ows[j], a0[j], a1[j], ent[j], pre[j], S, pull);
             if constexpr (is_pull(MODE)) {
                 if (pull != ~0ull) atomicMin(&sm.pull[wv][oi[j]], (unsigned long long)pull);
             }
+            if constexpr (MODE == M_RELAX || MODE == M_INVAL || MODE == M_TBFS)
+                visit_next(A, tgt, sm.stk[wv], top, S);
         }
     }
     S.edges += end - start;
@@ -427,7 +470,192 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
     }
 }
 
+// Forests of 64 trees (K' = 64): bitmap word v is the tree mask of vertex v, so a
+// word is processed with lane = tree.  Each row entry is read once (the same
+// address in every lane) and the neighbour's 64 tree states are one contiguous
+// 512-byte gather; the marks for a neighbour are one ballot and one atomicOr.
+// A task covers FCH entries of the row; longer rows leave items for the rest.
+constexpr uint32_t FCH = 64;              // row entries per forest task
+constexpr int FE = 8;                     // row entries in flight per forest iteration
+constexpr uint32_t FMIN = 24;             // trees of a vertex that make the tree-vector path pay
+constexpr uint32_t FOREST_ITEM = 1u << 31;  // item tag: a piece of a tree-vector task
+
+template <int MODE>
+__device__ void do_forest(const FArgs &A, uint32_t v, uint64_t mask, uint32_t start, bool fresh, WaveStats &S)
+{
+    const uint32_t lane = lane_id();
+    v = __builtin_amdgcn_readfirstlane(v);
+    const uint32_t os = v * 64 + lane;
+    const VRec *r = &A.g.vr[v];
+    const uint64_t boff = r->boff;
+    const uint32_t blen = r->blen;
+    const uint64_t dd = r->dbl;
+    const uint64_t dbeg = dd >> 32;
+    const uint32_t deg = blen + (uint32_t)dd;
+    bool act = (mask >> lane) & 1ull;
+    uint32_t d = 0, l = 0;
+    if (act) {
+        if constexpr (MODE == M_RELAX || MODE == M_TBFS || MODE == M_PICK || MODE == M_OFCHECK) {
+            const uint64_t su = A.st[os];
+            d = st_dist(su);
+            act = su != ST_INF;
+            if constexpr (MODE == M_TBFS) {
+                l = A.lvl[os];
+                act = act && l != LVL_INF;
+            }
+            if constexpr (MODE == M_PICK) {
+                act = act && st_parent(su) != v;
+                l = A.use_lvl ? A.lvl[os] : 0u;
+            }
+        }
+    }
+    if (fresh) S.front += (uint32_t)__popcll(mask);
+    const unsigned long long amask = __ballot(act);
+    if (!amask) return;
+    uint32_t end = min(start + FCH, deg);
+    if (fresh && deg > FCH) {  // the rest of the row as items for the next launch
+        const uint32_t nit = (deg - 1) / FCH;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(A.out_hcnt, nit);
+        base = __shfl(base, 0);
+        if (base + nit > A.hcap) {
+            void_items(A, base, nit);
+            if constexpr (MODE == M_RELAX || MODE == M_INVAL || MODE == M_TBFS) {
+                if (lane == 0) atomicOr((unsigned long long *)&A.out_bm[v], mask);
+                S.any = true;
+                end = start;  // the whole row is visited in the next round
+            } else {
+                if (lane == 0) atomicOr(&A.c->err, ERR_QUEUE);
+            }
+        } else {
+            for (uint32_t i = lane; i < nit; i += 64) {
+                A.out_hq[2 * (uint64_t)(base + i)] = mask;
+                A.out_hq[2 * (uint64_t)(base + i) + 1] = ((uint64_t)v << 32) | FOREST_ITEM | ((i + 1) * FCH);
+            }
+        }
+    }
+    uint64_t best = ~0ull;  // pull modes: this lane's best candidate
+    for (uint32_t eb = start; eb < end; eb += FE) {
+        uint64_t ent[FE], sx[FE];
+        uint32_t lx[FE];
+#pragma unroll
+        for (int j = 0; j < FE; ++j) {
+            const uint32_t e = eb + j;
+            ent[j] = e < end ? (e < blen ? A.g.adj[boff + e] : A.g.pool[dbeg + (e - blen)]) : ((uint64_t)v << 32);
+        }
+#pragma unroll
+        for (int j = 0; j < FE; ++j) {
+            const uint32_t x = e_col(ent[j]);
+            const bool use = x != v && (MODE == M_INVAL || !(e_w(ent[j]) & W_DEAD));
+            sx[j] = act && use ? A.st[x * 64 + lane] : ST_INF;
+            lx[j] = LVL_INF;
+            if constexpr (MODE == M_TBFS) lx[j] = act && use ? A.lvl[x * 64 + lane] : LVL_INF;
+            if constexpr (MODE == M_PICK) {
+                if (A.use_lvl) lx[j] = act && use ? A.lvl[x * 64 + lane] : LVL_INF;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < FE; ++j) {
+            const uint32_t x = e_col(ent[j]);
+            const uint32_t w = e_w(ent[j]);
+            const bool use = act && x != v && (MODE == M_INVAL || !(w & W_DEAD));
+            const uint32_t xs = x * 64 + lane;
+            bool hit = false;
+            if constexpr (MODE == M_RELAX) {
+                if (use) {
+                    const uint64_t nd = (uint64_t)d + w;
+                    if (nd >= DIST_INF) {
+                        if (sx[j] == ST_INF) atomicOr(&A.c->err, ERR_DIST_SUSPECT);
+                    } else {
+                        const uint64_t cand = pack_state((uint32_t)nd, v);
+                        if (cand < sx[j] && st_parent(sx[j]) != x) {
+                            atomicMin((unsigned long long *)&A.st[xs], (unsigned long long)cand);
+                            hit = true;
+                            S.marks++;
+                        }
+                    }
+                }
+            } else if constexpr (MODE == M_INVAL) {
+                if (use && sx[j] != ST_INF && st_parent(sx[j]) == v) {
+                    A.st[xs] = ST_INF;
+                    hit = true;
+                }
+            } else if constexpr (MODE == M_SEED) {
+                if (use && sx[j] != ST_INF) {
+                    const uint64_t nd = (uint64_t)st_dist(sx[j]) + w;
+                    if (nd >= DIST_INF)
+                        atomicOr(&A.c->err, ERR_DIST_SUSPECT);
+                    else
+                        best = min(best, pack_state((uint32_t)nd, x));
+                }
+            } else if constexpr (MODE == M_TBFS) {
+                if (use && sx[j] != ST_INF && st_parent(sx[j]) != x && (uint64_t)d + w == (uint64_t)st_dist(sx[j]) &&
+                    l + 1 < lx[j]) {
+                    atomicMin(&A.lvl[xs], l + 1);
+                    hit = true;
+                }
+            } else if constexpr (MODE == M_PICK) {
+                if (use && sx[j] != ST_INF && (uint64_t)st_dist(sx[j]) + w == (uint64_t)d &&
+                    (!A.use_lvl || lx[j] + 1 == l))
+                    best = min(best, (uint64_t)x);
+            } else if constexpr (MODE == M_OFCHECK) {
+                if (use && sx[j] == ST_INF) atomicOr(&A.c->err, ERR_DIST_OVERFLOW);
+            }
+            if constexpr (MODE == M_RELAX || MODE == M_INVAL || MODE == M_TBFS) {
+                const unsigned long long m = __ballot(hit);
+                if (m) {
+                    if (lane == 0) {
+                        atomicOr((unsigned long long *)&A.out_bm[x], m);
+                        if constexpr (MODE == M_INVAL) atomicOr((unsigned long long *)&A.inv_bm[x], m);
+                    }
+                    S.any = true;
+                }
+            }
+        }
+    }
+    S.edges += (uint64_t)(end - start) * (uint32_t)__popcll(amask);
+    if constexpr (MODE == M_SEED) {
+        const bool hit = act && best != ~0ull && best < A.st[os];
+        if (hit) atomicMin((unsigned long long *)&A.st[os], (unsigned long long)best);
+        const unsigned long long m = __ballot(hit);
+        if (m) {
+            if (lane == 0) atomicOr((unsigned long long *)&A.out_bm[v], m);
+            S.any = true;
+        }
+    } else if constexpr (MODE == M_PICK) {
+        if (act && best != ~0ull) atomicMin(&A.best[os], (uint32_t)best);
+    }
+}
+
+// Visits the slots on the wave's stack, 64 at a time, for at most A.local
+// tiles; what is left is marked for the next round.
 template <int MODE, int EPT>
+__device__ void drain(const FArgs &A, TileSmem &sm, uint32_t wv, WaveStats &S, uint32_t &top)
+{
+    if constexpr (MODE == M_RELAX || MODE == M_INVAL || MODE == M_TBFS) {
+        const uint32_t lane = lane_id();
+        uint32_t budget = A.local;
+        while (top) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (budget == 0) {
+                for (uint32_t i = lane; i < top; i += 64) mark(A.out_bm, sm.stk[wv][i]);
+                S.any = true;
+                top = 0;
+                break;
+            }
+            --budget;
+            const uint32_t n = min(64u, top);
+            const bool valid = lane < n;
+            const uint32_t os = valid ? sm.stk[wv][top - n + lane] : 0u;
+            top -= n;
+            do_slots<MODE, EPT>(A, os, valid, 0u, true, sm, wv, S, top);
+        }
+    }
+}
+
+template <int MODE, int EPT, bool FOREST>
 __global__ __launch_bounds__(FB) void k_frontier(FArgs A)
 {
     __shared__ TileSmem sm;
@@ -468,8 +696,21 @@ __global__ __launch_bounds__(FB) void k_frontier(FArgs A)
     // without scanning further.
     if (H == 0 && A.in_act && *A.in_act == 0) return;
     WaveStats S{0, 0, 0, false};
+    uint32_t top = 0;  // slots on this wave's stack (wave-uniform)
     for (uint32_t pb = 0; pb < nblk; pb += nw * 8) {
         if (pb) word = load_word(pb, wi);
+        if constexpr (FOREST && MODE != M_EMAP) {
+            {  // forests of 64 trees: one vertex per word
+                const bool wide = (uint32_t)__popcll(word) >= FMIN;
+                unsigned long long nz = __ballot(wide);
+                while (nz) {
+                    const uint32_t l = (uint32_t)__builtin_ctzll(nz);
+                    nz &= nz - 1;
+                    do_forest<MODE>(A, (uint32_t)__shfl((int)wi, (int)l), __shfl(word, (int)l), 0u, true, S);
+                }
+                if (wide) word = 0;  // the other words go through slot compaction
+            }
+        }
         const uint32_t cnt = (uint32_t)__popcll(word);
         uint32_t incl = cnt;
 #pragma unroll
@@ -492,14 +733,22 @@ __global__ __launch_bounds__(FB) void k_frontier(FArgs A)
             const uint32_t owi = (uint32_t)__shfl((int)wi, (int)o);
             const bool valid = q < T;
             const uint32_t os = valid ? owi * 64 + select_bit(ow, q - oex) : 0u;
-            do_slots<MODE, EPT>(A, os, valid, 0u, true, sm, wv, S);
+            do_slots<MODE, EPT>(A, os, valid, 0u, true, sm, wv, S, top);
+            drain<MODE, EPT>(A, sm, wv, S, top);
         }
     }
     for (uint32_t it = gw; it < H; it += nw) {
         const uint64_t mask = A.in_hq[2 * (uint64_t)it];
         const uint64_t ws = A.in_hq[2 * (uint64_t)it + 1];
         const uint32_t widx = (uint32_t)(ws >> 32);
-        do_slots<MODE, EPT>(A, widx * 64 + lane, (mask >> lane) & 1ull, (uint32_t)ws, false, sm, wv, S);
+        if constexpr (FOREST && MODE != M_EMAP) {
+            if ((uint32_t)ws & FOREST_ITEM) {
+                do_forest<MODE>(A, widx, mask, (uint32_t)ws & ~FOREST_ITEM, false, S);
+                continue;
+            }
+        }
+        do_slots<MODE, EPT>(A, widx * 64 + lane, (mask >> lane) & 1ull, (uint32_t)ws, false, sm, wv, S, top);
+        drain<MODE, EPT>(A, sm, wv, S, top);
     }
     const uint32_t marks = wave_sum(S.marks);
     const bool any = __ballot(S.any) != 0;
@@ -882,13 +1131,24 @@ __global__ __launch_bounds__(RB) void k_apply_best(uint32_t nslots, uint32_t ksh
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + RB - 1) / RB); }
 
+template <int MODE, int EPT, bool FOREST>
+void launch_kernel(const FArgs &a, int grid, hipStream_t s, hipEvent_t start, hipEvent_t stop)
+{
+    if (start || stop)
+        hipExtLaunchKernelGGL((k_frontier<MODE, EPT, FOREST>), dim3(grid), dim3(FB), 0, s, start, stop, 0, a);
+    else
+        hipLaunchKernelGGL((k_frontier<MODE, EPT, FOREST>), dim3(grid), dim3(FB), 0, s, a);
+}
+
+// Slot layouts with 64 trees per vertex use the variant with the tree-vector
+// path (its registers would lower the occupancy of every other launch).
 template <int MODE, int EPT>
 void launch_mode(const FArgs &a, int grid, hipStream_t s, hipEvent_t start, hipEvent_t stop)
 {
-    if (start || stop)
-        hipExtLaunchKernelGGL((k_frontier<MODE, EPT>), dim3(grid), dim3(FB), 0, s, start, stop, 0, a);
+    if (MODE != M_EMAP && a.kshift == 6)
+        launch_kernel<MODE, 4, MODE != M_EMAP>(a, grid, s, start, stop);
     else
-        hipLaunchKernelGGL((k_frontier<MODE, EPT>), dim3(grid), dim3(FB), 0, s, a);
+        launch_kernel<MODE, EPT, false>(a, grid, s, start, stop);
 }
 
 template <int EPT>

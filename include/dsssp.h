@@ -109,6 +109,8 @@ typedef struct {
     int32_t edges_per_thread; /* adjacency entries in flight per lane: 1, 2 or 4       */
     int32_t tile_entries;     /* row entries a wave takes per tile of up to 64 frontier
                                  slots; slots past it are deferred as pieces         */
+    int32_t local_tiles;      /* tiles of slots a wave has just marked that it visits
+                                 itself in the same launch (-1: none)                 */
 } dsssp_tuning;
 
 /* ---- library ---------------------------------------------------------- */

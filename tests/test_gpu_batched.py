@@ -52,6 +52,36 @@ def test_forest_matches_plain_and_oracle(K):
             assert np.array_equal(pp, par) and np.array_equal(pd, dist)
 
 
+@pytest.mark.parametrize("K,wlo", [(40, 0), (64, 1)])
+def test_forest64_layout(K, wlo):
+    """K' = 64 (one vertex per frontier word): the tree-vector path, with zero
+    weights (level repair and parent pick run as forests too) and with hub rows
+    split into pieces."""
+    rng = np.random.default_rng(700 + K)
+    a, b, w = ds.gen_rmat(10, 16, wlo=wlo, whi=6 if wlo == 0 else 1000, seed=K)
+    n = 1 << 10
+    srcs = rng.integers(0, n, K).astype(np.int32)
+    batches, ca, cb, cw = [], a, b, w
+    for ins in (0.5, 0.0, 1.0, 0.3):
+        c = ds.gen_changes(n, ca, cb, cw, 900, ins, wlo, 6 if wlo == 0 else 1000, seed=int(rng.integers(1 << 30)))
+        batches.append(c)
+        ca, cb, cw = oracle.apply_changes(n, ca, cb, cw, c)
+    say(f"forest64 K={K} wlo={wlo}")
+    g = ds.Graph(n, a, b, w)
+    f = ds.Tree(g, sources=srcs)
+    for c in batches:
+        ds.update(g, f, c)
+    csr = oracle.CSR.from_edges(n, ca, cb, cw)
+    zero = wlo == 0
+    for j in range(K):
+        par, dist = f.read(j)
+        _, dref = oracle.dijkstra(csr, [int(srcs[j])])
+        assert np.array_equal(dist, dref), j
+        assert np.array_equal(par, oracle.canonical_parents(csr, dref, [int(srcs[j])], rule=1 if zero else 0)), j
+    pp, pd = plain_result(n, a, b, w, int(srcs[0]), batches)
+    assert np.array_equal(pp, f.read(0)[0]) and np.array_equal(pd, f.read(0)[1])
+
+
 def test_forest_with_invalidation_heavy_batches():
     rng = np.random.default_rng(9)
     n = 3000
