@@ -41,7 +41,7 @@ using namespace dsssp;
 
 namespace {
 
-dsssp_tuning g_tuning = {FRONTIER_BLOCK, 2048, 256, 256, 4, 0, 0, 4, 1024, 4};
+dsssp_tuning g_tuning = {FRONTIER_BLOCK, 1024, 256, 256, 4, 0, 0, 4, 1024, 2};
 std::mutex g_tuning_mu;
 thread_local int t_device = 0;
 
@@ -100,6 +100,7 @@ int check_device(int dev)
 inline uint32_t words_of(uint64_t bits) { return (uint32_t)((bits + 63) / 64); }
 
 constexpr uint32_t INIT_DCAP = 4;  // initial delta block capacity of every row
+constexpr int64_t LOCAL_MAX = 32;  // largest continuation budget of a round (tiles)
 
 }  // namespace
 
@@ -222,6 +223,9 @@ FArgs round_args(dsssp_tree *t, const FArgs &base, int64_t i)
     a.out_hcnt = &t->c->hcnt[(ci + 1) % 3];
     a.zero_act = &t->c->act[(ci + 2) % 3];
     a.zero_hcnt = &t->c->hcnt[(ci + 2) % 3];
+    // Deep updates (high-diameter graphs) run many rounds of small frontiers:
+    // the continuation budget doubles every 16 rounds, up to LOCAL_MAX tiles.
+    if (a.local) a.local = (uint32_t)std::min<int64_t>(LOCAL_MAX, (int64_t)a.local << std::min<int64_t>(i / 16, 8));
     return a;
 }
 
@@ -252,11 +256,13 @@ void single_pass(dsssp_tree *t, int mode, FArgs a)
     launch_frontier(mode, b, grid, s);
 }
 
-// Instantiated hipGraph of rounds first .. first+group-1 of `mode`.
+// Instantiated hipGraph of rounds first .. first+group-1 of `mode`.  Round
+// arguments depend on the round index mod 6 and, through the continuation
+// budget, on the index itself below round 128.
 int captured_rounds(dsssp_tree *t, int mode, int64_t first, int group, int grid, const FArgs &base,
                     hipGraphExec_t *out)
 {
-    const int phase = (int)(first % 6);
+    const int phase = first < 128 ? (int)first + 6 : (int)(first % 6);
     for (auto &e : t->graphs)
         if (e.mode == mode && e.phase == phase && e.group == group && e.grid == grid &&
             !memcmp(&e.args, &base, sizeof base)) {

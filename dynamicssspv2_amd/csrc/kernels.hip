@@ -655,8 +655,10 @@ __device__ void drain(const FArgs &A, TileSmem &sm, uint32_t wv, WaveStats &S, u
     }
 }
 
+// At least 6 waves per SIMD: the register budget is capped so that occupancy
+// hides the latency of the random gathers.
 template <int MODE, int EPT, bool FOREST>
-__global__ __launch_bounds__(FB) void k_frontier(FArgs A)
+__global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_frontier(FArgs A)
 {
     __shared__ TileSmem sm;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -734,8 +736,8 @@ __global__ __launch_bounds__(FB) void k_frontier(FArgs A)
             const bool valid = q < T;
             const uint32_t os = valid ? owi * 64 + select_bit(ow, q - oex) : 0u;
             do_slots<MODE, EPT>(A, os, valid, 0u, true, sm, wv, S, top);
-            drain<MODE, EPT>(A, sm, wv, S, top);
         }
+        drain<MODE, EPT>(A, sm, wv, S, top);
     }
     for (uint32_t it = gw; it < H; it += nw) {
         const uint64_t mask = A.in_hq[2 * (uint64_t)it];
@@ -748,8 +750,8 @@ __global__ __launch_bounds__(FB) void k_frontier(FArgs A)
             }
         }
         do_slots<MODE, EPT>(A, widx * 64 + lane, (mask >> lane) & 1ull, (uint32_t)ws, false, sm, wv, S, top);
-        drain<MODE, EPT>(A, sm, wv, S, top);
     }
+    drain<MODE, EPT>(A, sm, wv, S, top);
     const uint32_t marks = wave_sum(S.marks);
     const bool any = __ballot(S.any) != 0;
     if (lane == 0) {
