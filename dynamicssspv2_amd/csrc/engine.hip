@@ -137,6 +137,11 @@ struct dsssp_graph {
     int attached = 0;
     bool destroy_pending = false;
     std::mutex mu;
+    // dsssp_update from host records reuses one device batch and one pinned
+    // staging buffer (grown on demand) instead of allocating per call.
+    dsssp_batch *upd = nullptr;
+    dsssp_change *stage = nullptr;
+    int64_t stage_cap = 0;
 
     GView view() const { return GView{vr, adj, pool, emap, emap_slots - 1}; }
 };
@@ -173,6 +178,7 @@ struct dsssp_batch {
     bool zero_w = false;
     int4 *rec = nullptr;
     uint32_t *tslot = nullptr;  // per record copy: position in its row's delta block
+    int64_t cap = 0;            // records rec / tslot hold
 };
 
 namespace {
@@ -505,8 +511,9 @@ void scrub(dsssp_tree *t)
 }
 
 // (Re)builds the edge map for `entries` directed entries plus headroom for
-// `extra` more: every live entry of every row is entered by one identity pass
-// of the frontier kernel (rows scanned whole by their wave, no chunk lists).
+// `extra` more: every live entry of every row is entered by an identity pass of
+// the frontier kernel, whose long rows and over-budget tiles go to a temporary
+// item list that a second launch spreads over all waves.
 int build_emap(dsssp_graph *g, uint64_t entries, uint64_t extra)
 {
     const uint64_t slots = emap_slots_for(entries + extra);
@@ -517,19 +524,50 @@ int build_emap(dsssp_graph *g, uint64_t entries, uint64_t extra)
         g->emap_slots = slots;
     }
     launch_fill_u64(g->emap, ~0ull, slots, g->stream);
+    const uint32_t heavy = 1024, chunk = 1024, budget = 4096;
+    const uint64_t E = g->E + g->delta_entries;
+    const uint64_t items = 2 * E / chunk + E / heavy + 63 * E / budget + 1024;
+    if (items > 0xFFFFFFFFull) return fail(DSSSP_ERR_ARG, "edge map build: item list would exceed 2^32 entries");
+    uint64_t *list = nullptr;
+    Counters *c = nullptr;
+    if (int s = dev_alloc((void **)&list, 16 * items, "edge map items")) return s;
+    if (int s = dev_alloc((void **)&c, sizeof(Counters), "edge map counters")) {
+        dev_free(list);
+        return s;
+    }
+    launch_fill_u32((uint32_t *)c, 0u, sizeof(Counters) / 4, g->stream);
     FArgs a;
     memset(&a, 0, sizeof a);
     a.g = g->view();
+    a.c = c;
     a.nslots = (uint32_t)g->n;
     a.nwords = g->nw;
+    a.hcap = (uint32_t)items;
     a.identity = 1;
     a.tiles = 1;
-    a.heavy = 0xFFFFFFFFu;
-    a.budget = 0xFFFFFFFFu;
-    a.chunk = 64;
+    a.heavy = heavy;
+    a.chunk = chunk;
+    a.budget = budget;
     a.ept = 4;
-    launch_frontier(M_EMAP, a, std::max(1, tuning().grid), g->stream);
-    HIP_TRY(hipGetLastError());
+    a.out_hq = list;
+    a.out_hcnt = &c->aux_hcnt;
+    const int grid = std::max(1, tuning().grid);
+    launch_frontier(M_EMAP, a, grid, g->stream);
+    FArgs b = a;
+    b.tiles = 0;
+    b.in_hq = list;
+    b.in_hcnt = &c->aux_hcnt;
+    b.out_hq = nullptr;
+    b.out_hcnt = nullptr;
+    launch_frontier(M_EMAP, b, grid, g->stream);
+    Counters h;
+    const hipError_t e1 = hipMemcpyAsync(&h, c, sizeof(Counters), hipMemcpyDeviceToHost, g->stream);
+    const hipError_t e2 = hipStreamSynchronize(g->stream);
+    dev_free(list);
+    dev_free(c);
+    if (e1 != hipSuccess || e2 != hipSuccess)
+        return fail(DSSSP_ERR_HIP, "edge map build failed: %s", hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+    if (h.err & ERR_QUEUE) return fail(DSSSP_ERR_STATE, "edge map build: item list overflow (internal)");
     g->emap_used = entries;
     return DSSSP_OK;
 }
@@ -761,6 +799,12 @@ static void graph_release(dsssp_graph *g)
     dev_free(g->pool);
     dev_free(g->tbm);
     dev_free(g->emap);
+    if (g->upd) {
+        dev_free(g->upd->rec);
+        dev_free(g->upd->tslot);
+        delete g->upd;
+    }
+    if (g->stage) (void)hipHostFree(g->stage);
     dev_free(g->wsum);
     dev_free(g->bsum);
     dev_free(g->bbase);
@@ -1277,16 +1321,47 @@ extern "C" int dsssp_update_batch(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b,
 extern "C" int dsssp_update(dsssp_graph *g, dsssp_tree *t, const dsssp_change *chg, int64_t k, dsssp_stats *stats)
 {
     clear_error();
-    if (!g || !t) return fail(DSSSP_ERR_ARG, "null argument");
+    if (!g || !t || k < 0 || (k > 0 && !chg)) return fail(DSSSP_ERR_ARG, "bad argument");
     if (t->g != g) return fail(DSSSP_ERR_ARG, "tree belongs to another graph");
     const auto t0 = std::chrono::steady_clock::now();
-    dsssp_batch *b = nullptr;
-    if (int s = dsssp_batch_upload(g, chg, k, &b)) return s;
+    int64_t ni, nd;
+    bool zw;
+    if (int s = validate_changes(g->n, chg, k, ni, nd, zw)) return s;
+    if (k == 0) {
+        if (stats) *stats = dsssp_stats{};
+        return DSSSP_OK;
+    }
+    DevGuard dg(g->device);
+    if (!g->upd) {
+        g->upd = new dsssp_batch();
+        g->upd->g = g;  // owned by the graph, not attached to it
+    }
+    dsssp_batch *b = g->upd;
+    if (b->cap < k) {
+        const int64_t cap = std::max<int64_t>(k, 2 * b->cap);
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        dev_free(b->rec);
+        dev_free(b->tslot);
+        b->cap = 0;
+        if (int s = dev_alloc((void **)&b->rec, sizeof(int4) * (size_t)cap, "batch")) return s;
+        if (int s = dev_alloc((void **)&b->tslot, sizeof(uint32_t) * 2 * (size_t)cap, "batch slots")) return s;
+        b->cap = cap;
+    }
+    if (g->stage_cap < k) {
+        const int64_t cap = std::max<int64_t>(k, 2 * g->stage_cap);
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        if (g->stage) (void)hipHostFree(g->stage);
+        g->stage = nullptr;
+        g->stage_cap = 0;
+        HIP_TRY(hipHostMalloc((void **)&g->stage, sizeof(dsssp_change) * (size_t)cap, hipHostMallocDefault));
+        g->stage_cap = cap;
+    }
+    memcpy(g->stage, chg, sizeof(dsssp_change) * (size_t)k);
+    HIP_TRY(hipMemcpyAsync(b->rec, g->stage, sizeof(int4) * (size_t)k, hipMemcpyHostToDevice, g->stream));
+    b->k = k;
+    b->n_ins = ni;
+    b->n_del = nd;
+    b->zero_w = zw;
     const double h2d = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    int s = b->k ? update_impl(g, t, b, stats, h2d) : DSSSP_OK;
-    if (!b->k && stats) *stats = dsssp_stats{};
-    const std::string keep = s ? last_error() : std::string();
-    dsssp_batch_destroy(b);
-    if (s) fail(s, "%s", keep.c_str());
-    return s;
+    return update_impl(g, t, b, stats, h2d);
 }
