@@ -63,7 +63,7 @@ def parse():
     p.add_argument("--copy-probe", action="store_true",
                    help="only copy 1 GiB device-to-device five times (calibrates HBM byte counters), then exit")
     for knob in ("grid", "heavy_degree", "chunk", "round_batch", "use_graph", "edges_per_thread", "tile_entries",
-                 "local_tiles"):
+                 "local_tiles", "delta"):
         p.add_argument("--" + knob.replace("_", "-"), type=int, default=None, help="tuning knob (dsssp_tuning)")
     return p.parse_args()
 
@@ -469,7 +469,7 @@ def main():
     import dynamicssspv2_amd as ds
     ds.set_device(0)
     knobs = {k: getattr(args, k) for k in ("grid", "heavy_degree", "chunk", "round_batch", "use_graph",
-                                           "edges_per_thread", "tile_entries", "local_tiles")
+                                           "edges_per_thread", "tile_entries", "local_tiles", "delta")
              if getattr(args, k) is not None}
     if knobs:
         ds.set_tuning(**knobs)

@@ -76,10 +76,13 @@ class ServerTotals(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad0"}
 
 
+DELTA_AUTO = -2  # Tuning.delta: ordered relaxation chosen per tree (include/dsssp.h)
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [(f, ctypes.c_int32) for f in ("block", "grid", "heavy_degree", "chunk", "round_batch",
                                               "kernel_timing", "use_graph", "edges_per_thread",
-                                              "tile_entries", "local_tiles")]
+                                              "tile_entries", "local_tiles", "delta")]
 
 
 def build(force: bool = True) -> None:

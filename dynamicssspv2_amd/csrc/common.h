@@ -85,6 +85,11 @@ constexpr int STAT_SLOTS = 2048;
 struct Counters {
     uint32_t act[3];    // a round marked its output bitmap; indexed by round % 3
     uint32_t hcnt[3];   // heavy chunks queued for the next round; round % 3
+    // Distance-ordered relaxation (a threshold on the slots a round visits),
+    // indexed by round % 3 like act:
+    uint32_t near[3];      // a round marked a slot below its threshold
+    uint32_t thr[3];       // the threshold a round used
+    uint32_t fenc[3][64];  // ~(smallest distance marked at or above it), max-reduced over 64 words
     uint32_t aux_hcnt;  // heavy chunks queued by a single-pass stage
     uint32_t err;
     unsigned long long stats[NSTAT];

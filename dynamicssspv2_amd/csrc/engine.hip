@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cinttypes>
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -41,7 +42,7 @@ using namespace dsssp;
 
 namespace {
 
-dsssp_tuning g_tuning = {FRONTIER_BLOCK, 1024, 256, 256, 4, 0, 0, 4, 1024, 2};
+dsssp_tuning g_tuning = {FRONTIER_BLOCK, 1024, 256, 256, 4, 0, 0, 4, 1024, 2, DSSSP_DELTA_AUTO};
 std::mutex g_tuning_mu;
 thread_local int t_device = 0;
 
@@ -101,6 +102,10 @@ inline uint32_t words_of(uint64_t bits) { return (uint32_t)((bits + 63) / 64); }
 
 constexpr uint32_t INIT_DCAP = 4;  // initial delta block capacity of every row
 constexpr int64_t LOCAL_MAX = 32;  // largest continuation budget of a round (tiles)
+// Automatic ordered relaxation: trees whose distances span more than DEEP_STEPS
+// steps, or whose ordered construction took more than DEEP_ROUNDS rounds.
+constexpr uint32_t DEEP_STEPS = 64;
+constexpr int64_t DEEP_ROUNDS = 512;
 
 }  // namespace
 
@@ -129,6 +134,7 @@ struct dsssp_graph {
     GraphCtl *gc = nullptr;
     GraphCtl *gc_host = nullptr;  // pinned
     bool zero_w = false;          // some weight was ever 0: parents need the tight-BFS repair
+    uint32_t auto_delta = 1;      // step of ordered relaxation: twice the mean input weight
     hipEvent_t ev[8] = {};
     std::vector<hipEvent_t> kev;  // per-launch timing events (kernel_timing)
     // Trees and batches attached to this graph.  dsssp_graph_destroy while any
@@ -152,6 +158,10 @@ struct dsssp_tree {
     int32_t ntrees = 1;   // trees sharing the graph (a forest of K sources)
     uint32_t kshift = 0;  // slot = vertex << kshift | tree
     uint64_t nslots = 0;  // n << kshift
+    // Distances span many steps of the graph's auto_delta (a high-diameter
+    // graph): relaxation is distance-ordered when the tuning leaves it to the
+    // library.
+    bool deep = false;
     uint32_t nw = 0;      // words of a slot bitmap
     uint64_t *st = nullptr;
     uint64_t *fb[2] = {nullptr, nullptr};  // frontier bitmaps
@@ -210,7 +220,17 @@ FArgs base_args(dsssp_tree *t)
     a.ept = (uint32_t)tu.edges_per_thread;
     a.budget = (uint32_t)tu.tile_entries;
     a.local = tu.local_tiles > 0 ? (uint32_t)tu.local_tiles : 0u;
+    a.delta = 0;  // set for the relaxation rounds and the seeding before them
     return a;
+}
+
+// Step of the distance-ordered relaxation of tree t (0: off): the tuning's,
+// or in automatic mode the graph's step for deep trees.
+uint32_t relax_delta(const dsssp_tree *t)
+{
+    const int32_t d = tuning().delta;
+    if (d == DSSSP_DELTA_AUTO) return t->deep ? t->g->auto_delta : 0u;
+    return d > 0 ? (uint32_t)d : 0u;
 }
 
 // Arguments of round i of a loop phase; the producer stage before the loop
@@ -229,14 +249,19 @@ FArgs round_args(dsssp_tree *t, const FArgs &base, int64_t i)
     a.out_hcnt = &t->c->hcnt[(ci + 1) % 3];
     a.zero_act = &t->c->act[(ci + 2) % 3];
     a.zero_hcnt = &t->c->hcnt[(ci + 2) % 3];
+    a.ring = (uint32_t)ci;
     // Deep updates (high-diameter graphs) run many rounds of small frontiers:
     // the continuation budget doubles every 16 rounds, up to LOCAL_MAX tiles.
     if (a.local) a.local = (uint32_t)std::min<int64_t>(LOCAL_MAX, (int64_t)a.local << std::min<int64_t>(i / 16, 8));
     return a;
 }
 
-// Clears the round flags and chunk counts (act[3], hcnt[3]) before a producer.
-void zero_ring(dsssp_tree *t) { launch_fill_u32(&t->c->act[0], 0u, 6, t->g->stream); }
+// Clears the round flags, chunk counts and threshold ring (act .. fenc) before
+// a producer.
+void zero_ring(dsssp_tree *t)
+{
+    launch_fill_u32(&t->c->act[0], 0u, offsetof(Counters, aux_hcnt) / sizeof(uint32_t), t->g->stream);
+}
 
 // One pass of a single-pass mode (seed, parent pick) into bitmap 0 / flag 0.
 // Rows longer than the heavy threshold are queued as chunks on list 1 (idle
@@ -246,6 +271,7 @@ void single_pass(dsssp_tree *t, int mode, FArgs a)
     hipStream_t s = t->g->stream;
     const int grid = std::max(1, tuning().grid);
     launch_fill_u32(&t->c->aux_hcnt, 0u, 1, s);
+    a.ring = 2;  // output ring slot 0, read by round 0 of the loop that follows
     a.out_bm = t->fb[0];
     a.out_act = &t->c->act[0];
     a.out_hq = t->hq[1];
@@ -306,7 +332,12 @@ int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms)
     const int max_group = 64;
     const int grid = std::max(1, tu.grid);
     const bool timed = tu.kernel_timing && kernel_ms;
-    const FArgs base = base_args(t);
+    FArgs base = base_args(t);
+    if (mode == M_RELAX) base.delta = relax_delta(t);
+    // Each threshold move follows a round that visited nothing and settles at
+    // least the smallest marked slot, so ordered relaxation needs at most about
+    // twice the rounds of the plain one.
+    const int64_t max_rounds = (base.delta ? 3 : 1) * (int64_t)t->nslots + 64;
     int64_t i = 0;
     for (;;) {
         if (timed) {
@@ -340,8 +371,8 @@ int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms)
         }
         if (t->c_host->act[ci] == 0 && t->c_host->hcnt[ci] == 0) break;
         // Every round settles at least one more hop of every shortest path, so
-        // n + 64 rounds can only be exceeded by an internal error.
-        if (i > g->n + 64) return fail(DSSSP_ERR_STATE, "frontier did not empty after %lld rounds", (long long)i);
+        // more rounds than max_rounds can only come from an internal error.
+        if (i > max_rounds) return fail(DSSSP_ERR_STATE, "frontier did not empty after %lld rounds", (long long)i);
         group = std::min(group * 2, max_group);
     }
     return DSSSP_OK;
@@ -630,10 +661,12 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
         if (b->n_del > 0) {
             FArgs a = base_args(t);
             a.in_bm = t->ib;  // consumed, so clean for the next update
+            a.delta = relax_delta(t);
             single_pass(t, M_SEED, a);
         }
         if (b->n_ins > 0)
-            launch_ins_offer(b->rec, b->k, t->kshift, b->tslot, g->view(), t->st, t->fb[0], &t->c->act[0], t->c, s);
+            launch_ins_offer(b->rec, b->k, t->kshift, b->tslot, g->view(), t->st, t->fb[0], &t->c->act[0], t->c,
+                             relax_delta(t) != 0, s);
     }
     HIP_TRY(hipEventRecord(g->ev[4], s));
     // 5. relaxation
@@ -760,11 +793,12 @@ extern "C" int dsssp_set_tuning(const dsssp_tuning *t)
     if (t->edges_per_thread) nt.edges_per_thread = t->edges_per_thread;
     if (t->tile_entries) nt.tile_entries = t->tile_entries;
     if (t->local_tiles) nt.local_tiles = t->local_tiles;
+    if (t->delta) nt.delta = t->delta;
     nt.kernel_timing = t->kernel_timing;
     nt.use_graph = t->use_graph;
     if (nt.grid < 1 || nt.grid > 65536 || nt.heavy_degree < 64 || nt.chunk < 64 || nt.chunk % 64 ||
         nt.round_batch < 1 || nt.round_batch > 64 || nt.tile_entries < 64 || nt.local_tiles < -1 ||
-        nt.local_tiles > 1024 ||
+        nt.local_tiles > 1024 || nt.delta < DSSSP_DELTA_AUTO ||
         (nt.edges_per_thread != 1 && nt.edges_per_thread != 2 && nt.edges_per_thread != 4))
         return fail(DSSSP_ERR_ARG, "tuning out of range");
     g_tuning = nt;
@@ -859,7 +893,14 @@ extern "C" int dsssp_graph_create(int64_t n, int64_t m, const int32_t *a, const 
     g->device = t_device;
     g->n = n;
     g->E = adj.size();
-    for (int64_t i = 0; i < m && !g->zero_w; ++i) g->zero_w = w[i] == 0;
+    {
+        double sum = 0;
+        for (int64_t i = 0; i < m; ++i) {
+            sum += w[i];
+            if (w[i] == 0) g->zero_w = true;
+        }
+        g->auto_delta = (uint32_t)std::min(std::max(2.0 * sum / (double)std::max<int64_t>(m, 1), 1.0), 1e9);
+    }
     // Every row starts with a delta block of INIT_DCAP entries, so most
     // insertions land in place; the pool leaves room for the growth bound of
     // ensure_pool.
@@ -1019,6 +1060,7 @@ extern "C" int dsssp_graph_copy(dsssp_graph *dst, dsssp_graph *src)
     dst->delta_entries = src->delta_entries;
     dst->emap_used = src->emap_used;
     dst->zero_w = src->zero_w;
+    dst->auto_delta = src->auto_delta;
     return DSSSP_OK;
 }
 
@@ -1085,6 +1127,7 @@ static int tree_make(dsssp_graph *g, int32_t ntrees, const std::vector<uint32_t>
     if (((uint64_t)n << kshift) > 0xFFFFFF00ull)
         return fail(DSSSP_ERR_ARG, "n * trees must stay below 2^32 - 256 (n=%lld, trees=%d)", (long long)n, ntrees);
     std::vector<uint64_t> packed;
+    uint64_t max_dist = 0;
     if (parent) {
         if (!dist) return fail(DSSSP_ERR_ARG, "parent given without dist");
         packed.resize((size_t)n);
@@ -1101,6 +1144,7 @@ static int tree_make(dsssp_graph *g, int32_t ntrees, const std::vector<uint32_t>
             if (p == v && dist[v] != 0)
                 return fail(DSSSP_ERR_TREE, "root %lld has distance %lld, not 0", (long long)v, (long long)dist[v]);
             packed[v] = pack_state((uint32_t)dist[v], (uint32_t)p);
+            max_dist = std::max<uint64_t>(max_dist, (uint64_t)dist[v]);
         }
     } else {
         for (uint32_t r : roots)
@@ -1113,6 +1157,10 @@ static int tree_make(dsssp_graph *g, int32_t ntrees, const std::vector<uint32_t>
     t->ntrees = ntrees;
     t->kshift = kshift;
     t->nslots = (uint64_t)n << kshift;
+    // A given tree is deep when its distances span DEEP_STEPS automatic steps; a
+    // tree built from scratch is built ordered and is deep when that took more
+    // than DEEP_ROUNDS rounds.
+    t->deep = parent ? max_dist > (uint64_t)DEEP_STEPS * g->auto_delta : true;
     if (int s = tree_alloc(t)) {
         tree_release(t);
         return s;
@@ -1139,7 +1187,9 @@ static int tree_make(dsssp_graph *g, int32_t ntrees, const std::vector<uint32_t>
             zero_ring(t);
             launch_set_sources(t->st, d_roots, (uint32_t)roots.size(), t->kshift, per_tree, t->fb[0], &t->c->act[0],
                                g->stream);
+            launch_fill_u32(&t->c->fenc[0][0], ~0u, 1, g->stream);  // the roots were marked at distance 0
             s = run_rounds(t, M_RELAX, rounds, nullptr);
+            t->deep = rounds > DEEP_ROUNDS;
         }
         if (!s && g->zero_w) s = repick_parents(t, true, rounds);
     }
@@ -1257,6 +1307,7 @@ extern "C" int dsssp_tree_copy(dsssp_tree *dst, dsssp_tree *src)
     HIP_TRY(hipMemcpyAsync(dst->st, src->st, sizeof(uint64_t) * ns, hipMemcpyDeviceToDevice, dst->g->stream));
     HIP_TRY(hipStreamSynchronize(dst->g->stream));
     dst->src = src->src;
+    dst->deep = src->deep;
     return DSSSP_OK;
 }
 

@@ -60,6 +60,14 @@ struct FArgs {
     uint32_t ept;           // edges in flight per lane: 1, 2 or 4
     uint32_t budget;        // entries a fresh tile takes; later slots become items
     uint32_t local;         // push modes: tiles of newly marked slots a wave visits itself
+    // RELAX with delta > 0: a round visits only slots below its threshold and
+    // leaves the others marked; when the previous round marked nothing below
+    // its threshold, the threshold moves to the smallest marked distance plus
+    // delta.  SEED then records the distances it marks.  Ring slot `ring` of
+    // the Counters' near / thr / fenc is this launch's input, ring + 1 its
+    // output, ring + 2 is cleared.
+    uint32_t delta;
+    uint32_t ring;
 };
 
 constexpr int FRONTIER_BLOCK = 256;
@@ -91,8 +99,9 @@ void launch_classify(const int4 *rec, int64_t k, uint32_t kshift, uint64_t *st, 
                      uint32_t *out_act, hipStream_t s);
 // Tombstones every copy of each deleted entry, found through the edge map.
 void launch_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats, hipStream_t s);
+// With far set, the distances it marks go to ring slot 0 of c->fenc.
 void launch_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint32_t *tslot, GView g, uint64_t *st,
-                      uint64_t *out_bm, uint32_t *out_act, Counters *c, hipStream_t s);
+                      uint64_t *out_bm, uint32_t *out_act, Counters *c, bool far, hipStream_t s);
 // Roots: for j < count, vertex src[j] becomes a root (distance 0, parent itself)
 // of tree (per_tree ? j : 0) and is marked.
 void launch_set_sources(uint64_t *st, const uint32_t *src, uint32_t count, uint32_t kshift, bool per_tree,

@@ -116,7 +116,29 @@ struct WaveStats {
     uint64_t edges;  // wave-uniform
     uint32_t marks;  // per lane
     bool any;        // per lane: this lane set an output bit
+    // Distance-ordered relaxation: this round's threshold (wave-uniform; 0 in
+    // SEED, whose marks all count as far), and per lane whether a mark fell
+    // below it and ~(the smallest distance marked at or above it).
+    uint32_t thr;
+    bool near;
+    uint32_t fenc;
 };
+
+// Raises *p to v.  A relaxed load first skips the atomic when v is no record
+// (a stale read only costs an atomic).
+__device__ __forceinline__ void raise_u32(uint32_t *p, uint32_t v)
+{
+    if (v > __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(p, v);
+}
+
+// Bookkeeping of a slot marked for the next round with distance d.
+__device__ __forceinline__ void note_mark(WaveStats &S, uint32_t d)
+{
+    if (d < S.thr)
+        S.near = true;
+    else
+        S.fenc = max(S.fenc, ~d);
+}
 
 // Each edge is handled in two steps so that a lane can keep several edges in
 // flight: gather() issues the loads of the neighbour's state, apply() decides
@@ -155,7 +177,7 @@ constexpr uint32_t NO_SLOT = 0xFFFFFFFFu;
 
 template <int MODE>
 __device__ __forceinline__ uint32_t apply(const FArgs &A, uint32_t os, uint32_t aux0, uint32_t aux1, uint64_t ent,
-                                          const Pre &p, WaveStats &S, uint64_t &pull)
+                                          const Pre &p, WaveStats &S, uint64_t &pull, uint32_t &td)
 {
     if (p.skip) return NO_SLOT;
     const uint32_t u = os >> A.kshift;
@@ -174,6 +196,7 @@ __device__ __forceinline__ uint32_t apply(const FArgs &A, uint32_t os, uint32_t 
         if (cand >= p.sx || st_parent(p.sx) == x) return NO_SLOT;
         atomicMin((unsigned long long *)&A.st[xs], (unsigned long long)cand);
         S.marks++;
+        td = (uint32_t)nd;
         return xs;
     } else if constexpr (MODE == M_INVAL) {
         // x is a child of u in the pre-update tree: reset it and descend.
@@ -216,8 +239,11 @@ constexpr uint32_t LSTK = 256;  // slots a wave keeps to visit itself in the sam
 
 // A slot to visit next: kept on the wave's LDS stack when local continuation is
 // on and there is room (it is then visited later in this launch, unmarked),
-// else marked in the output bitmap for the next round.  `top` is wave-uniform.
-__device__ __forceinline__ void visit_next(const FArgs &A, uint32_t tgt, uint32_t *stk, uint32_t &top, WaveStats &S)
+// else marked in the output bitmap for the next round.  `top` is wave-uniform;
+// td is the target's new distance (RELAX).
+template <int MODE>
+__device__ __forceinline__ void visit_next(const FArgs &A, uint32_t tgt, uint32_t td, uint32_t *stk, uint32_t &top,
+                                           WaveStats &S)
 {
     const unsigned long long m = __ballot(tgt != NO_SLOT);
     if (!m) return;
@@ -226,7 +252,12 @@ __device__ __forceinline__ void visit_next(const FArgs &A, uint32_t tgt, uint32_
         if (tgt != NO_SLOT) stk[top + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull))] = tgt;
         top += cnt;
     } else {
-        if (tgt != NO_SLOT) mark(A.out_bm, tgt);
+        if (tgt != NO_SLOT) {
+            mark(A.out_bm, tgt);
+            if constexpr (MODE == M_RELAX) {
+                if (A.delta) note_mark(S, td);
+            }
+        }
         S.any = true;
     }
 }
@@ -342,8 +373,18 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
         deg = blen + (uint32_t)dd;
         if (!owner_prep<MODE>(A, os, aux0, aux1)) deg = 0;
     }
+    bool later = false;  // at or above the round's threshold: left for a later round
+    if constexpr (MODE == M_RELAX) {
+        if (fresh && A.delta && deg && aux0 >= S.thr) {
+            mark(A.out_bm, os);
+            S.any = true;
+            S.fenc = max(S.fenc, ~aux0);
+            later = true;
+            deg = 0;
+        }
+    }
     if (fresh) {
-        S.front += (uint32_t)__popcll(__ballot(valid));
+        S.front += (uint32_t)__popcll(__ballot(valid && !later));
         bool defer = deg > A.heavy;
         {  // rows ending past the budget, the tile's first row excepted
             uint32_t ib = defer ? 0u : deg;
@@ -372,7 +413,12 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
                 // the next round instead.  Pull modes visit each slot once.
                 void_items(A, base, tot);
                 if constexpr (MODE == M_RELAX || MODE == M_INVAL || MODE == M_TBFS) {
-                    if (defer) mark(A.out_bm, os);
+                    if (defer) {
+                        mark(A.out_bm, os);
+                        if constexpr (MODE == M_RELAX) {
+                            if (A.delta) note_mark(S, aux0);
+                        }
+                    }
                     S.any = true;
                 } else {
                     if (lane == 0) atomicOr(&A.c->err, ERR_QUEUE);
@@ -444,12 +490,13 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
 #pragma unroll
         for (int j = 0; j < EPT; ++j) {
             uint64_t pull = ~0ull;
-            const uint32_t tgt = apply<MODE>(A, ows[j], a0[j], a1[j], ent[j], pre[j], S, pull);
+            uint32_t td = 0;
+            const uint32_t tgt = apply<MODE>(A, ows[j], a0[j], a1[j], ent[j], pre[j], S, pull, td);
             if constexpr (is_pull(MODE)) {
                 if (pull != ~0ull) atomicMin(&sm.pull[wv][oi[j]], (unsigned long long)pull);
             }
             if constexpr (MODE == M_RELAX || MODE == M_INVAL || MODE == M_TBFS)
-                visit_next(A, tgt, sm.stk[wv], top, S);
+                visit_next<MODE>(A, tgt, td, sm.stk[wv], top, S);
         }
     }
     S.edges += end - start;
@@ -460,9 +507,11 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
         // One commit per owner with entries in this range.
         const uint32_t lo = sm.off[wv][lane];
         const bool mine = deg != 0 && lo < end && lo + deg > start;
-        if (mine && pull_commit<MODE>(A, os, sm.pull[wv][lane])) {
+        const uint64_t best = sm.pull[wv][lane];
+        if (mine && pull_commit<MODE>(A, os, best)) {
             mark(A.out_bm, os);
             S.any = true;
+            if (A.delta) note_mark(S, st_dist(best));
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -509,8 +558,20 @@ __device__ void do_forest(const FArgs &A, uint32_t v, uint64_t mask, uint32_t st
             }
         }
     }
-    if (fresh) S.front += (uint32_t)__popcll(mask);
+    if constexpr (MODE == M_RELAX) {
+        if (fresh && A.delta) {  // trees at or above the round's threshold: left for a later round
+            const bool later = act && d >= S.thr;
+            const unsigned long long lm = __ballot(later);
+            if (lm) {
+                if (lane == 0) atomicOr((unsigned long long *)&A.out_bm[v], lm);
+                S.any = true;
+                if (later) S.fenc = max(S.fenc, ~d);
+                act = act && !later;
+            }
+        }
+    }
     const unsigned long long amask = __ballot(act);
+    if (fresh) S.front += (uint32_t)__popcll(MODE == M_RELAX ? amask : mask);
     if (!amask) return;
     uint32_t end = min(start + FCH, deg);
     if (fresh && deg > FCH) {  // the rest of the row as items for the next launch
@@ -521,15 +582,18 @@ __device__ void do_forest(const FArgs &A, uint32_t v, uint64_t mask, uint32_t st
         if (base + nit > A.hcap) {
             void_items(A, base, nit);
             if constexpr (MODE == M_RELAX || MODE == M_INVAL || MODE == M_TBFS) {
-                if (lane == 0) atomicOr((unsigned long long *)&A.out_bm[v], mask);
+                if (lane == 0) atomicOr((unsigned long long *)&A.out_bm[v], amask);
                 S.any = true;
+                if constexpr (MODE == M_RELAX) {
+                    if (A.delta && act) note_mark(S, d);
+                }
                 end = start;  // the whole row is visited in the next round
             } else {
                 if (lane == 0) atomicOr(&A.c->err, ERR_QUEUE);
             }
         } else {
             for (uint32_t i = lane; i < nit; i += 64) {
-                A.out_hq[2 * (uint64_t)(base + i)] = mask;
+                A.out_hq[2 * (uint64_t)(base + i)] = amask;
                 A.out_hq[2 * (uint64_t)(base + i) + 1] = ((uint64_t)v << 32) | FOREST_ITEM | ((i + 1) * FCH);
             }
         }
@@ -572,6 +636,7 @@ __device__ void do_forest(const FArgs &A, uint32_t v, uint64_t mask, uint32_t st
                             atomicMin((unsigned long long *)&A.st[xs], (unsigned long long)cand);
                             hit = true;
                             S.marks++;
+                            if (A.delta) note_mark(S, (uint32_t)nd);
                         }
                     }
                 }
@@ -616,7 +681,10 @@ __device__ void do_forest(const FArgs &A, uint32_t v, uint64_t mask, uint32_t st
     S.edges += (uint64_t)(end - start) * (uint32_t)__popcll(amask);
     if constexpr (MODE == M_SEED) {
         const bool hit = act && best != ~0ull && best < A.st[os];
-        if (hit) atomicMin((unsigned long long *)&A.st[os], (unsigned long long)best);
+        if (hit) {
+            atomicMin((unsigned long long *)&A.st[os], (unsigned long long)best);
+            if (A.delta) note_mark(S, st_dist(best));
+        }
         const unsigned long long m = __ballot(hit);
         if (m) {
             if (lane == 0) atomicOr((unsigned long long *)&A.out_bm[v], m);
@@ -640,7 +708,13 @@ __device__ void drain(const FArgs &A, TileSmem &sm, uint32_t wv, WaveStats &S, u
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             if (budget == 0) {
-                for (uint32_t i = lane; i < top; i += 64) mark(A.out_bm, sm.stk[wv][i]);
+                for (uint32_t i = lane; i < top; i += 64) {
+                    const uint32_t slot = sm.stk[wv][i];
+                    mark(A.out_bm, slot);
+                    if constexpr (MODE == M_RELAX) {
+                        if (A.delta) note_mark(S, st_dist(A.st[slot]));
+                    }
+                }
                 S.any = true;
                 top = 0;
                 break;
@@ -661,11 +735,33 @@ template <int MODE, int EPT, bool FOREST>
 __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_frontier(FArgs A)
 {
     __shared__ TileSmem sm;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (A.zero_act) *A.zero_act = 0;
-        if (A.zero_hcnt) *A.zero_hcnt = 0;
-    }
     const uint32_t lane = lane_id();
+    if (blockIdx.x == 0 && threadIdx.x < 64) {
+        if (threadIdx.x == 0) {
+            if (A.zero_act) *A.zero_act = 0;
+            if (A.zero_hcnt) *A.zero_hcnt = 0;
+        }
+        if (A.delta) {
+            const uint32_t z = (A.ring + 2) % 3;
+            A.c->fenc[z][lane] = 0;
+            if (lane == 0) A.c->near[z] = 0;
+        }
+    }
+    // This round's threshold: the previous one, or, when the previous round
+    // marked nothing below it, the smallest distance it marked plus delta.
+    WaveStats S{0, 0, 0, false, 0u, false, 0u};
+    if constexpr (MODE == M_RELAX) {
+        if (A.delta) {
+            uint32_t fe = A.c->fenc[A.ring][lane];
+#pragma unroll
+            for (int d = 32; d > 0; d >>= 1) fe = max(fe, (uint32_t)__shfl_xor((int)fe, d));
+            uint32_t thr = A.c->thr[A.ring];
+            // (with nothing recorded at all, every slot is visited)
+            if (!A.c->near[A.ring]) thr = fe ? (uint32_t)min((uint64_t)~fe + A.delta, (uint64_t)DIST_INF) : DIST_INF;
+            S.thr = __builtin_amdgcn_readfirstlane(thr);
+            if (blockIdx.x == 0 && threadIdx.x == 0) A.c->thr[(A.ring + 1) % 3] = S.thr;
+        }
+    }
     const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t gw = blockIdx.x * FW + wv;
     const uint32_t nw = gridDim.x * FW;
@@ -697,7 +793,6 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_
     // A round after convergence has an empty bitmap and no items: it returns
     // without scanning further.
     if (H == 0 && A.in_act && *A.in_act == 0) return;
-    WaveStats S{0, 0, 0, false};
     uint32_t top = 0;  // slots on this wave's stack (wave-uniform)
     for (uint32_t pb = 0; pb < nblk; pb += nw * 8) {
         if (pb) word = load_word(pb, wi);
@@ -754,6 +849,19 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_
     drain<MODE, EPT>(A, sm, wv, S, top);
     const uint32_t marks = wave_sum(S.marks);
     const bool any = __ballot(S.any) != 0;
+    if constexpr (MODE == M_RELAX || MODE == M_SEED) {
+        if (A.delta) {
+            const bool nr = __ballot(S.near) != 0;
+            uint32_t fe = S.fenc;
+#pragma unroll
+            for (int d = 32; d > 0; d >>= 1) fe = max(fe, (uint32_t)__shfl_xor((int)fe, d));
+            if (lane == 0) {
+                const uint32_t o = (A.ring + 1) % 3;
+                if (nr) A.c->near[o] = 1u;
+                if (fe) raise_u32(&A.c->fenc[o][blockIdx.x & 63u], fe);
+            }
+        }
+    }
     if (lane == 0) {
         if (any && A.out_act) *A.out_act = 1u;
         if constexpr (MODE == M_RELAX) {
@@ -1024,12 +1132,13 @@ __global__ __launch_bounds__(RB) void k_classify(const int4 *rec, int64_t k, uin
 // directions (the role of insertEdge, gpuFunctions_undir.cuh:217-251).
 __global__ __launch_bounds__(RB) void k_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint32_t *tslot,
                                                   GView g, uint64_t *st, uint64_t *out_bm, uint32_t *out_act,
-                                                  Counters *c)
+                                                  Counters *c, uint32_t far)
 {
     const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
     const int64_t ri = i >> kshift;
     const uint32_t tree = (uint32_t)i & ((1u << kshift) - 1u);
     bool hit = false;
+    uint32_t fe = 0;  // ~(smallest distance marked)
     if (ri < k) {
         const int4 r = rec[ri];
         const uint32_t ta = tslot[2 * ri];
@@ -1052,6 +1161,7 @@ __global__ __launch_bounds__(RB) void k_ins_offer(const int4 *rec, int64_t k, ui
                         atomicMin((unsigned long long *)&st[sb_slot], (unsigned long long)cand);
                         mark(out_bm, sb_slot);
                         hit = true;
+                        fe = max(fe, ~(uint32_t)nd);
                     }
                 }
             }
@@ -1065,12 +1175,18 @@ __global__ __launch_bounds__(RB) void k_ins_offer(const int4 *rec, int64_t k, ui
                         atomicMin((unsigned long long *)&st[sa_slot], (unsigned long long)cand);
                         mark(out_bm, sa_slot);
                         hit = true;
+                        fe = max(fe, ~(uint32_t)nd);
                     }
                 }
             }
         }
     }
     if (__ballot(hit) != 0 && lane_id() == 0) *out_act = 1u;
+    if (far) {
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) fe = max(fe, (uint32_t)__shfl_xor((int)fe, d));
+        if (fe && lane_id() == 0) raise_u32(&c->fenc[0][blockIdx.x & 63u], fe);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1246,11 +1362,11 @@ void launch_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats
 }
 
 void launch_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint32_t *tslot, GView g, uint64_t *st,
-                      uint64_t *out_bm, uint32_t *out_act, Counters *c, hipStream_t s)
+                      uint64_t *out_bm, uint32_t *out_act, Counters *c, bool far, hipStream_t s)
 {
     if (k > 0)
         hipLaunchKernelGGL(k_ins_offer, dim3(blocks_for(k << kshift)), dim3(RB), 0, s, rec, k, kshift, tslot, g, st,
-                           out_bm, out_act, c);
+                           out_bm, out_act, c, far ? 1u : 0u);
 }
 
 void launch_set_sources(uint64_t *st, const uint32_t *src, uint32_t count, uint32_t kshift, bool per_tree,

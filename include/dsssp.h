@@ -111,7 +111,16 @@ typedef struct {
                                  slots; slots past it are deferred as pieces         */
     int32_t local_tiles;      /* tiles of slots a wave has just marked that it visits
                                  itself in the same launch (-1: none)                 */
+    int32_t delta;            /* distance-ordered relaxation: a round visits only
+                                 slots below a threshold that advances in steps of
+                                 delta once nothing below it is left.  -1: off (every
+                                 marked slot is visited in the next round);
+                                 DSSSP_DELTA_AUTO (default): twice the graph's mean
+                                 weight, for trees whose distances span more than 64
+                                 such steps (high-diameter graphs), else off         */
 } dsssp_tuning;
+
+#define DSSSP_DELTA_AUTO (-2)
 
 /* ---- library ---------------------------------------------------------- */
 const char *dsssp_last_error(void);
