@@ -162,6 +162,7 @@ struct dsssp_tree {
     // graph): relaxation is distance-ordered when the tuning leaves it to the
     // library.
     bool deep = false;
+    int64_t last_rounds[8] = {};  // rounds the last loop of each mode took
     uint32_t nw = 0;      // words of a slot bitmap
     uint64_t *st = nullptr;
     uint64_t *fb[2] = {nullptr, nullptr};  // frontier bitmaps
@@ -328,8 +329,14 @@ int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms)
     dsssp_graph *g = t->g;
     hipStream_t s = g->stream;
     const dsssp_tuning tu = tuning();
-    int group = std::max(1, tu.round_batch);
     const int max_group = 64;
+    // The first group covers about as many rounds as the last loop of this mode
+    // took, so that a loop as deep as the last one needs one convergence check;
+    // later groups start again from round_batch.  The estimate creeps down by
+    // one round per loop that converged within it.
+    const int rb = std::max(1, tu.round_batch);
+    const int first = (int)std::max<int64_t>(rb, std::min<int64_t>(t->last_rounds[mode], max_group));
+    int group = first;
     const int grid = std::max(1, tu.grid);
     const bool timed = tu.kernel_timing && kernel_ms;
     FArgs base = base_args(t);
@@ -373,8 +380,9 @@ int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms)
         // Every round settles at least one more hop of every shortest path, so
         // more rounds than max_rounds can only come from an internal error.
         if (i > max_rounds) return fail(DSSSP_ERR_STATE, "frontier did not empty after %lld rounds", (long long)i);
-        group = std::min(group * 2, max_group);
+        group = i == first && first < max_group ? rb : std::min(group * 2, max_group);
     }
+    t->last_rounds[mode] = i == first ? first - 1 : i;
     return DSSSP_OK;
 }
 

@@ -64,6 +64,12 @@ __device__ __forceinline__ void add_stat(unsigned long long *bstats, int which, 
 // 32-bit fingerprint of the key and the entry's position in its row (base
 // entries first, then the delta block; positions never change, as blocks move
 // in order).  A lookup verifies the entry itself, so it is exact; ~0 = empty.
+// The first EMAP_DSKIP delta entries of a row are not in the map: a deletion
+// scans them (a row's first delta entries share one or two 64-byte sectors),
+// so the common insertion, into a short delta row, claims no map slot.
+constexpr uint32_t EMAP_DSKIP = 8;
+
+__device__ __forceinline__ bool emap_keeps(uint32_t pos, uint32_t blen) { return pos < blen || pos - blen >= EMAP_DSKIP; }
 __device__ __forceinline__ uint64_t emap_hash(uint32_t row, uint32_t col, uint32_t w)
 {
     uint64_t h = ((uint64_t)row << 32 | col) * 0x9E3779B97F4A7C15ull;
@@ -87,7 +93,8 @@ __device__ __forceinline__ void emap_insert(const GView &g, uint32_t row, uint32
         if (atomicCAS((unsigned long long *)&g.emap[s], ~0ull, (unsigned long long)v) == ~0ull) return;
 }
 
-// Tombstones every live copy of (row, col, w); returns slots read.
+// Tombstones every live copy of (row, col, w); returns map slots and delta
+// entries read.
 __device__ __forceinline__ uint32_t emap_kill(const GView &g, uint32_t row, uint32_t col, uint32_t w)
 {
     const uint64_t h = emap_hash(row, col, w);
@@ -108,7 +115,13 @@ __device__ __forceinline__ uint32_t emap_kill(const GView &g, uint32_t row, uint
         const uint64_t e = *ep;
         if (e_key(e) == key && !(e & E_DEAD)) *ep = e | E_DEAD;
     }
-    return probes;
+    const uint32_t ns = min(dl, EMAP_DSKIP);
+    uint64_t *dp = &g.pool[d >> 32];
+    for (uint32_t i = 0; i < ns; ++i) {
+        const uint64_t e = dp[i];
+        if (e_key(e) == key && !(e & E_DEAD)) dp[i] = e | E_DEAD;
+    }
+    return probes + ns;
 }
 
 struct WaveStats {
@@ -481,7 +494,8 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
         if constexpr (MODE == M_EMAP) {
 #pragma unroll
             for (int j = 0; j < EPT; ++j)
-                if (act[j] && !(ent[j] & E_DEAD)) emap_insert(A.g, ows[j], e_col(ent[j]), e_w(ent[j]), pos[j]);
+                if (act[j] && !(ent[j] & E_DEAD) && emap_keeps(pos[j], sm.blen[wv][oi[j]]))
+                    emap_insert(A.g, ows[j], e_col(ent[j]), e_w(ent[j]), pos[j]);
             continue;
         }
         Pre pre[EPT];
@@ -920,7 +934,8 @@ __device__ __forceinline__ bool ins_copy(const int4 *rec, int64_t t, uint32_t &r
 }
 
 // Reserves a position with a returning atomicAdd on the row's delta length;
-// with room in the block the copy is written and entered in the edge map, else
+// with room in the block the copy is written (and entered in the edge map past
+// the row's first EMAP_DSKIP delta entries), else
 // it waits for the block to grow (the thread that reserved position == capacity
 // marks the row).
 __global__ __launch_bounds__(RB) void k_ins_place(const int4 *rec, int64_t k, GView g, uint32_t *tslot,
@@ -934,7 +949,7 @@ __global__ __launch_bounds__(RB) void k_ins_place(const int4 *rec, int64_t k, GV
     const uint32_t pos = (uint32_t)o, cap = r->dcap;
     if (pos < cap) {
         g.pool[(o >> 32) + pos] = pack_entry(col, w);
-        emap_insert(g, row, col, w, r->blen + pos);
+        if (pos >= EMAP_DSKIP) emap_insert(g, row, col, w, r->blen + pos);
         tslot[t] = pos;
     } else {
         tslot[t] = pos | SLOT_PENDING;
@@ -1068,7 +1083,7 @@ __global__ __launch_bounds__(RB) void k_ins_pending(const int4 *rec, int64_t k, 
     const uint32_t pos = tslot[t] & ~SLOT_PENDING;
     const VRec *r = &g.vr[row];
     g.pool[(r->dbl >> 32) + pos] = pack_entry(col, w);
-    emap_insert(g, row, col, w, r->blen + pos);
+    if (pos >= EMAP_DSKIP) emap_insert(g, row, col, w, r->blen + pos);
     tslot[t] = pos;
 }
 
