@@ -57,6 +57,7 @@ def parse():
     p.add_argument("--instances", type=int, default=16)
     p.add_argument("--requests", type=int, default=400)
     p.add_argument("--rate", type=float, default=0.0, help="served: arrivals per second (0: all at once)")
+    p.add_argument("--lanes", type=int, default=0, help="served: concurrent lanes (0: the library's default)")
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU Dijkstra baseline / exactness check")
     p.add_argument("--no-timing", action="store_true", help="no per-launch events (roofline unmeasured)")
@@ -379,7 +380,7 @@ def config5(args, ds, np):
     scales = [20 + (i * 5) // args.instances for i in range(args.instances)]  # 20..24 spread evenly
     if args.scale < 24:
         scales = [min(s, args.scale) for s in scales]
-    srv = ds.Server(mem_budget_bytes=96 << 30)
+    srv = ds.Server(mem_budget_bytes=96 << 30, lanes=args.lanes)
     insts = []
     for i, sc in enumerate(scales):
         a, b, w = ds.gen_rmat(sc, 16, (0.57, 0.19, 0.19), 1, 1000, seed=args.seed + i)
@@ -440,6 +441,7 @@ def config5(args, ds, np):
            "vs_baseline": None, "dtype": "u32", "data": "synthetic",
            "config": {"workload": f"served {len(insts)} rmat instances s{min(scales)}-s{max(scales)}, "
                                   f"{len(reqs)} requests k~logU[1,1e5]", "rate_per_s": args.rate,
+                      "lanes": args.lanes,
                       "scales": scales},
            "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "plain_changed_edges_per_s": plain,
            "speedup_vs_plain": served / plain, "dist_bitexact": same, "check": {"each_equals_plain": same},

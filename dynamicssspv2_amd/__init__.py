@@ -55,7 +55,7 @@ class Stats(ctypes.Structure):
 
 
 class ServerConfig(ctypes.Structure):
-    _fields_ = [("mem_budget_bytes", ctypes.c_uint64), ("max_merge", ctypes.c_int32), ("pad0", ctypes.c_int32),
+    _fields_ = [("mem_budget_bytes", ctypes.c_uint64), ("max_merge", ctypes.c_int32), ("lanes", ctypes.c_int32),
                 ("max_records", ctypes.c_int64)]
 
 
@@ -409,10 +409,11 @@ class Batch:
 
 
 class Server:
-    """Served form: resident instances + continuous-batching request loop."""
+    """Served form: resident instances + continuous-batching request loops, one
+    per lane (a device graph of some of the instances, with its own stream)."""
 
-    def __init__(self, mem_budget_bytes: int, max_merge: int = 0, max_records: int = 0):
-        cfg = ServerConfig(mem_budget_bytes, max_merge, 0, max_records)
+    def __init__(self, mem_budget_bytes: int, max_merge: int = 0, max_records: int = 0, lanes: int = 0):
+        cfg = ServerConfig(mem_budget_bytes, max_merge, lanes, max_records)
         h = ctypes.c_void_p()
         _check(lib().dsssp_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

@@ -225,6 +225,16 @@ FArgs base_args(dsssp_tree *t)
     return a;
 }
 
+// Workgroups of a frontier launch over a bitmap of nwords words: the tuning's
+// grid, or enough workgroups to read the whole bitmap in one pass (a wave
+// reads 64 words per pass), so that a nearly empty frontier of a large graph
+// costs one round trip to memory rather than one per pass.
+int frontier_grid(uint64_t nwords)
+{
+    const uint64_t one_pass = (nwords + 64 * (FRONTIER_BLOCK / 64) - 1) / (64 * (FRONTIER_BLOCK / 64));
+    return (int)std::max<uint64_t>((uint64_t)std::max(1, tuning().grid), std::min<uint64_t>(one_pass, 4096));
+}
+
 // Step of the distance-ordered relaxation of tree t (0: off): the tuning's,
 // or in automatic mode the graph's step for deep trees.
 uint32_t relax_delta(const dsssp_tree *t)
@@ -270,7 +280,7 @@ void zero_ring(dsssp_tree *t)
 void single_pass(dsssp_tree *t, int mode, FArgs a)
 {
     hipStream_t s = t->g->stream;
-    const int grid = std::max(1, tuning().grid);
+    const int grid = frontier_grid(t->nw);
     launch_fill_u32(&t->c->aux_hcnt, 0u, 1, s);
     a.ring = 2;  // output ring slot 0, read by round 0 of the loop that follows
     a.out_bm = t->fb[0];
@@ -337,7 +347,7 @@ int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms)
     const int rb = std::max(1, tu.round_batch);
     const int first = (int)std::max<int64_t>(rb, std::min<int64_t>(t->last_rounds[mode], max_group));
     int group = first;
-    const int grid = std::max(1, tu.grid);
+    const int grid = frontier_grid(t->nw);
     const bool timed = tu.kernel_timing && kernel_ms;
     FArgs base = base_args(t);
     if (mode == M_RELAX) base.delta = relax_delta(t);
@@ -590,7 +600,7 @@ int build_emap(dsssp_graph *g, uint64_t entries, uint64_t extra)
     a.ept = 4;
     a.out_hq = list;
     a.out_hcnt = &c->aux_hcnt;
-    const int grid = std::max(1, tuning().grid);
+    const int grid = frontier_grid(a.nwords);
     launch_frontier(M_EMAP, a, grid, g->stream);
     FArgs b = a;
     b.tiles = 0;

@@ -1,18 +1,21 @@
-// serve.cpp -- served form: resident instances and a continuous-batching
-// request loop on one GPU.
+// serve.cpp -- served form: resident instances and continuous-batching
+// request loops on one GPU.
 //
-// All admitted instances live in ONE device graph (vertex ranges laid end to
-// end) with ONE tree object holding one root per instance, so a merged update
-// that carries batches for several instances runs their work in the same
-// kernel launches.  A request loop thread takes, in arrival order, the oldest
-// pending request of every instance that has one (FIFO per instance), up to
-// max_merge requests / max_records records, concatenates them with vertex
-// offsets, uploads and applies them as one update, then wakes the waiters.
-// Requests that arrive while a merged update is on the GPU are picked up by
-// the next one.  Components never share vertices or edges, so each instance's
+// The admitted instances are spread over lanes (largest first, each to the
+// lane with the fewest bytes).  A lane is ONE device graph holding its
+// instances' vertex ranges end to end, with ONE tree object holding one root
+// per instance, its own stream and its own loop thread.  A lane's loop takes,
+// in arrival order, the oldest pending request of every one of its instances
+// that has one (FIFO per instance), up to max_merge requests / max_records
+// records, concatenates them with vertex offsets, uploads and applies them as
+// one update, then wakes the waiters.  Requests that arrive while a merged
+// update is on the GPU are picked up by the next one; the lanes' updates run
+// concurrently on their streams, so small updates of different lanes overlap on
+// the device.  Components never share vertices or edges, so each instance's
 // result is exactly what it would be alone.
 // The reference has no counterpart (one graph and one batch per process,
 // SSSPmainForUndirected.cu:27-335).
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -32,7 +35,9 @@ using sclock = std::chrono::steady_clock;
 namespace {
 
 struct Instance {
-    int64_t n = 0, m = 0, offset = 0;
+    int64_t n = 0, m = 0;
+    int32_t lane = 0;
+    int64_t offset = 0;  // first vertex in the lane's graph
     std::vector<int32_t> a, b, w;
     int32_t src = 0;
     uint64_t bytes = 0;
@@ -62,20 +67,29 @@ uint64_t instance_bytes(int64_t n, int64_t m)
 
 }  // namespace
 
+// One lane: a device graph holding some instances end to end, one tree with a
+// root per instance, its own stream (the graph's) and its own loop thread.
+struct Lane {
+    std::vector<int32_t> members;  // instance ids
+    int64_t N = 0;                 // vertices of the lane's graph
+    dsssp_graph *g = nullptr;
+    dsssp_tree *t = nullptr;
+    std::deque<Request *> pending;
+    std::condition_variable cv_work;
+    std::thread loop;
+};
+
 struct dsssp_server {
     dsssp_server_config cfg{};
     std::vector<Instance> inst;
+    std::vector<Lane> lanes;
     uint64_t used = 0;
-    dsssp_graph *g = nullptr;
-    dsssp_tree *t = nullptr;
     int device = 0;
     bool started = false, stopping = false;
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
-    std::deque<Request *> pending;
+    std::condition_variable cv_done;
     std::map<uint64_t, Request *> all;
     uint64_t next_ticket = 1;
-    std::thread loop;
     // totals
     int64_t launches = 0, records = 0, requests_done = 0;
     double busy_ms = 0;
@@ -83,18 +97,18 @@ struct dsssp_server {
 
 namespace {
 
-void serve_loop(dsssp_server *s)
+void serve_loop(dsssp_server *s, Lane *L)
 {
     (void)dsssp_set_device(s->device);
     for (;;) {
         std::vector<Request *> pick;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait(lk, [&] { return s->stopping || !s->pending.empty(); });
-            if (s->pending.empty() && s->stopping) return;
+            L->cv_work.wait(lk, [&] { return s->stopping || !L->pending.empty(); });
+            if (L->pending.empty() && s->stopping) return;
             std::vector<char> taken(s->inst.size(), 0);
             int64_t recs = 0;
-            for (auto it = s->pending.begin(); it != s->pending.end();) {
+            for (auto it = L->pending.begin(); it != L->pending.end();) {
                 Request *r = *it;
                 const bool room = (s->cfg.max_merge <= 0 || (int32_t)pick.size() < s->cfg.max_merge) &&
                                   (pick.empty() || s->cfg.max_records <= 0 ||
@@ -103,7 +117,7 @@ void serve_loop(dsssp_server *s)
                     taken[r->inst] = 1;
                     recs += (int64_t)r->chg.size();
                     pick.push_back(r);
-                    it = s->pending.erase(it);
+                    it = L->pending.erase(it);
                 } else {
                     taken[r->inst] = 1;  // keeps per-instance FIFO order
                     ++it;
@@ -124,7 +138,7 @@ void serve_loop(dsssp_server *s)
             }
         }
         dsssp_stats st{};
-        const int rc = dsssp_update(s->g, s->t, merged.data(), (int64_t)merged.size(), &st);
+        const int rc = dsssp_update(L->g, L->t, merged.data(), (int64_t)merged.size(), &st);
         const std::string err = rc ? dsssp_last_error() : std::string();
         const auto t1 = sclock::now();
         {
@@ -143,6 +157,24 @@ void serve_loop(dsssp_server *s)
             s->busy_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
         }
         s->cv_done.notify_all();
+    }
+}
+
+// Stops the loops and frees the lanes' device objects.
+void stop_lanes(dsssp_server *s)
+{
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+    }
+    for (Lane &L : s->lanes) L.cv_work.notify_all();
+    for (Lane &L : s->lanes)
+        if (L.loop.joinable()) L.loop.join();
+    for (Lane &L : s->lanes) {
+        if (L.t) dsssp_tree_destroy(L.t);
+        if (L.g) dsssp_graph_destroy(L.g);
+        L.t = nullptr;
+        L.g = nullptr;
     }
 }
 
@@ -179,7 +211,6 @@ extern "C" int dsssp_server_add_instance(dsssp_server *s, int64_t n, int64_t m, 
     Instance I;
     I.n = n;
     I.m = m;
-    I.offset = total;
     I.a.assign(a, a + m);
     I.b.assign(b, b + m);
     I.w.assign(w, w + m);
@@ -198,33 +229,65 @@ extern "C" int dsssp_server_start(dsssp_server *s)
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->started) return fail(DSSSP_ERR_STATE, "already started");
     if (s->inst.empty()) return fail(DSSSP_ERR_STATE, "no instance admitted");
-    int64_t N = 0, M = 0;
-    for (auto &x : s->inst) N += x.n, M += x.m;
-    std::vector<int32_t> A((size_t)M), B((size_t)M), W((size_t)M), roots;
-    int64_t e = 0;
-    for (auto &x : s->inst) {
-        const int32_t off = (int32_t)x.offset;
-        for (int64_t i = 0; i < x.m; ++i, ++e) {
-            A[e] = x.a[i] + off;
-            B[e] = x.b[i] + off;
-            W[e] = x.w[i];
+    const int ninst = (int)s->inst.size();
+    const int nl = std::max(1, std::min(ninst, s->cfg.lanes > 0 ? (int)s->cfg.lanes : 1));
+    // Largest instances first, each to the lane with the fewest bytes so far.
+    std::vector<int32_t> order((size_t)ninst);
+    for (int i = 0; i < ninst; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t x, int32_t y) { return s->inst[x].bytes > s->inst[y].bytes; });
+    std::vector<uint64_t> load((size_t)nl, 0);
+    std::vector<std::vector<int32_t>> members((size_t)nl);
+    for (int32_t i : order) {
+        const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[l] += s->inst[i].bytes;
+        members[l].push_back(i);
+    }
+    s->lanes = std::vector<Lane>((size_t)nl);
+    int rc = DSSSP_OK;
+    for (int l = 0; l < nl && !rc; ++l) {
+        Lane &L = s->lanes[l];
+        std::sort(members[l].begin(), members[l].end());
+        L.members = members[l];
+        int64_t N = 0, M = 0;
+        for (int32_t i : L.members) {
+            s->inst[i].lane = l;
+            s->inst[i].offset = N;
+            N += s->inst[i].n;
+            M += s->inst[i].m;
         }
-        roots.push_back(x.src + off);
+        L.N = N;
+        std::vector<int32_t> A((size_t)M), B((size_t)M), W((size_t)M), roots;
+        int64_t e = 0;
+        for (int32_t i : L.members) {
+            const Instance &x = s->inst[i];
+            const int32_t off = (int32_t)x.offset;
+            for (int64_t j = 0; j < x.m; ++j, ++e) {
+                A[e] = x.a[j] + off;
+                B[e] = x.b[j] + off;
+                W[e] = x.w[j];
+            }
+            roots.push_back(x.src + off);
+        }
+        rc = dsssp_graph_create(N, M, A.data(), B.data(), W.data(), DSSSP_GRAPH_DEFAULT, &L.g);
+        if (!rc) rc = dsssp_tree_create_roots(L.g, (int32_t)roots.size(), roots.data(), DSSSP_TREE_DEFAULT, &L.t);
+    }
+    if (rc) {
+        const std::string keep = dsssp_last_error();
+        for (Lane &L : s->lanes) {
+            if (L.t) dsssp_tree_destroy(L.t);
+            if (L.g) dsssp_graph_destroy(L.g);
+        }
+        s->lanes.clear();
+        return fail(rc, "%s", keep.c_str());
+    }
+    for (auto &x : s->inst) {
         std::vector<int32_t>().swap(x.a);
         std::vector<int32_t>().swap(x.b);
         std::vector<int32_t>().swap(x.w);
     }
-    int rc = dsssp_graph_create(N, M, A.data(), B.data(), W.data(), DSSSP_GRAPH_DEFAULT, &s->g);
-    if (rc) return rc;
-    rc = dsssp_tree_create_roots(s->g, (int32_t)roots.size(), roots.data(), DSSSP_TREE_DEFAULT, &s->t);
-    if (rc) {
-        std::string keep = dsssp_last_error();
-        dsssp_graph_destroy(s->g);
-        s->g = nullptr;
-        return fail(rc, "%s", keep.c_str());
-    }
     s->started = true;
-    s->loop = std::thread(serve_loop, s);
+    for (Lane &L : s->lanes) L.loop = std::thread(serve_loop, s, &L);
     return DSSSP_OK;
 }
 
@@ -249,11 +312,12 @@ extern "C" int dsssp_server_submit(dsssp_server *s, int32_t id, const dsssp_chan
     r->inst = id;
     r->chg.assign(chg, chg + k);
     r->submit = sclock::now();
-    s->pending.push_back(r);
+    Lane &L = s->lanes[s->inst[id].lane];
+    L.pending.push_back(r);
     s->all[r->ticket] = r;
     *ticket = r->ticket;
     lk.unlock();
-    s->cv_work.notify_one();
+    L.cv_work.notify_one();
     return DSSSP_OK;
 }
 
@@ -297,11 +361,10 @@ extern "C" int dsssp_server_read(dsssp_server *s, int32_t id, int32_t *parent, i
         return true;
     });
     const Instance &I = s->inst[id];
-    int64_t N = 0;
-    for (auto &x : s->inst) N += x.n;
-    std::vector<int32_t> par((size_t)N);
-    std::vector<int64_t> d((size_t)N);
-    const int rc = dsssp_tree_read(s->t, par.data(), d.data());
+    const Lane &L = s->lanes[I.lane];
+    std::vector<int32_t> par((size_t)L.N);
+    std::vector<int64_t> d((size_t)L.N);
+    const int rc = dsssp_tree_read(L.t, par.data(), d.data());
     if (rc) return rc;
     for (int64_t v = 0; v < I.n; ++v) {
         const int32_t p = par[I.offset + v];
@@ -323,7 +386,9 @@ extern "C" int dsssp_server_info(dsssp_server *s, dsssp_server_totals *tot)
     tot->records = s->records;
     tot->requests = s->requests_done;
     tot->busy_ms = s->busy_ms;
-    tot->pending = (int64_t)s->pending.size();
+    int64_t pend = 0;
+    for (const Lane &L : s->lanes) pend += (int64_t)L.pending.size();
+    tot->pending = pend;
     return DSSSP_OK;
 }
 
@@ -331,15 +396,8 @@ extern "C" int dsssp_server_destroy(dsssp_server *s)
 {
     clear_error();
     if (!s) return DSSSP_OK;
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->stopping = true;
-    }
-    s->cv_work.notify_all();
-    if (s->loop.joinable()) s->loop.join();
+    stop_lanes(s);
     for (auto &kv : s->all) delete kv.second;
-    if (s->t) dsssp_tree_destroy(s->t);
-    if (s->g) dsssp_graph_destroy(s->g);
     delete s;
     return DSSSP_OK;
 }

@@ -249,16 +249,18 @@ int dsssp_update_batch(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stat
 
 /* ---- served form -------------------------------------------------------------
  * Resident instances (each a graph with its own source) kept in HBM under a
- * memory budget, and a request loop that merges batches arriving for different
- * instances into shared launches (continuous batching).  Each instance's result
- * is bit-identical to updating it alone.  No reference counterpart: the
- * reference applies one batch to one graph per process
+ * memory budget, and request loops that merge batches arriving for different
+ * instances into shared launches (continuous batching).  The instances are
+ * spread over `lanes`, each one device graph with its own stream and loop
+ * thread, so the merged updates of different lanes overlap on the GPU.  Each
+ * instance's result is bit-identical to updating it alone.  No reference
+ * counterpart: the reference applies one batch to one graph per process
  * (SSSPmainForUndirected.cu:27-335). */
 typedef struct dsssp_server dsssp_server;
 typedef struct {
     uint64_t mem_budget_bytes;  /* device bytes all admitted instances may use      */
     int32_t max_merge;          /* most requests merged into one launch (0: no cap) */
-    int32_t pad0;
+    int32_t lanes;              /* concurrent lanes, at most one per instance (0: 1) */
     int64_t max_records;        /* most records per merged launch (0: no cap)       */
 } dsssp_server_config;
 typedef struct {
