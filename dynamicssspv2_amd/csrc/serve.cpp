@@ -100,6 +100,7 @@ namespace {
 void serve_loop(dsssp_server *s, Lane *L)
 {
     (void)dsssp_set_device(s->device);
+    std::vector<dsssp_change> merged;  // reused: grows to the largest merge
     for (;;) {
         std::vector<Request *> pick;
         {
@@ -125,7 +126,10 @@ void serve_loop(dsssp_server *s, Lane *L)
             }
         }
         const auto t0 = sclock::now();
-        std::vector<dsssp_change> merged;
+        size_t total = 0;
+        for (Request *r : pick) total += r->chg.size();
+        merged.clear();
+        merged.reserve(total);
         for (Request *r : pick) {
             r->start = t0;
             const int32_t off = (int32_t)s->inst[r->inst].offset;
@@ -295,10 +299,15 @@ extern "C" int dsssp_server_submit(dsssp_server *s, int32_t id, const dsssp_chan
 {
     clear_error();
     if (!s || !ticket || k < 0 || (k > 0 && !chg)) return fail(DSSSP_ERR_ARG, "bad argument");
-    std::unique_lock<std::mutex> lk(s->mu);
-    if (!s->started || s->stopping) return fail(DSSSP_ERR_STATE, "server not running");
-    if (id < 0 || id >= (int32_t)s->inst.size()) return fail(DSSSP_ERR_ARG, "unknown instance %d", id);
-    const int64_t n = s->inst[id].n;
+    int64_t n = 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (!s->started || s->stopping) return fail(DSSSP_ERR_STATE, "server not running");
+        if (id < 0 || id >= (int32_t)s->inst.size()) return fail(DSSSP_ERR_ARG, "unknown instance %d", id);
+        n = s->inst[id].n;  // fixed once started
+    }
+    // Validation and the copy of the records happen outside the lock, so a
+    // large submission does not hold up the loops or other submitters.
     for (int64_t i = 0; i < k; ++i) {
         const dsssp_change &c = chg[i];
         if (c.op != 0 && c.op != 1) continue;
@@ -308,9 +317,14 @@ extern "C" int dsssp_server_submit(dsssp_server *s, int32_t id, const dsssp_chan
         if (c.w < 0) return fail(DSSSP_ERR_WEIGHT, "change %lld: negative weight", (long long)i);
     }
     Request *r = new Request();
-    r->ticket = s->next_ticket++;
     r->inst = id;
     r->chg.assign(chg, chg + k);
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->stopping) {
+        delete r;
+        return fail(DSSSP_ERR_STATE, "server not running");
+    }
+    r->ticket = s->next_ticket++;
     r->submit = sclock::now();
     Lane &L = s->lanes[s->inst[id].lane];
     L.pending.push_back(r);
