@@ -77,6 +77,7 @@ struct Lane {
     std::deque<Request *> pending;
     std::condition_variable cv_work;
     std::thread loop;
+    std::mutex use;  // held while the lane's graph and tree are in use (one caller at a time)
 };
 
 struct dsssp_server {
@@ -142,8 +143,13 @@ void serve_loop(dsssp_server *s, Lane *L)
             }
         }
         dsssp_stats st{};
-        const int rc = dsssp_update(L->g, L->t, merged.data(), (int64_t)merged.size(), &st);
-        const std::string err = rc ? dsssp_last_error() : std::string();
+        int rc;
+        std::string err;
+        {
+            std::lock_guard<std::mutex> use(L->use);
+            rc = dsssp_update(L->g, L->t, merged.data(), (int64_t)merged.size(), &st);
+            if (rc) err = dsssp_last_error();
+        }
         const auto t1 = sclock::now();
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -375,10 +381,15 @@ extern "C" int dsssp_server_read(dsssp_server *s, int32_t id, int32_t *parent, i
         return true;
     });
     const Instance &I = s->inst[id];
-    const Lane &L = s->lanes[I.lane];
+    Lane &L = s->lanes[I.lane];
+    lk.unlock();  // the lane's loop may go on with other instances meanwhile
     std::vector<int32_t> par((size_t)L.N);
     std::vector<int64_t> d((size_t)L.N);
-    const int rc = dsssp_tree_read(L.t, par.data(), d.data());
+    int rc;
+    {
+        std::lock_guard<std::mutex> use(L.use);
+        rc = dsssp_tree_read(L.t, par.data(), d.data());
+    }
     if (rc) return rc;
     for (int64_t v = 0; v < I.n; ++v) {
         const int32_t p = par[I.offset + v];
