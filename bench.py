@@ -61,6 +61,8 @@ def parse():
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU Dijkstra baseline / exactness check")
     p.add_argument("--no-timing", action="store_true", help="no per-launch events (roofline unmeasured)")
+    p.add_argument("--autotune", action="store_true",
+                   help="plain configs: tune the launch knobs on the warm-up batches first (dsssp_autotune)")
     p.add_argument("--copy-probe", action="store_true",
                    help="only copy 1 GiB device-to-device five times (calibrates HBM byte counters), then exit")
     for knob in ("grid", "heavy_degree", "chunk", "round_batch", "use_graph", "edges_per_thread", "tile_entries",
@@ -186,6 +188,12 @@ def run_plain(args, ds, np, n, a, b, w, src, batches, label, cfg, traffic=False,
     g = ds.Graph(n, a, b, w)
     t = ds.Tree(g, src)
     res = [ds.Batch(g, c) for c in batches]
+    extra = {}
+    if args.autotune and args.warmup > 0:
+        t0 = time.perf_counter()
+        tun, tms = ds.autotune(g, t, res[:args.warmup])
+        log(f"{label}: autotuned in {time.perf_counter() - t0:.1f}s: {tun}, {tms:.3f} ms per warm-up batch")
+        extra["autotuned"] = tun
     stats, wall = timed_steps(ds, g, t, res, args.warmup, not args.no_timing)
     for r in res:
         r.close()
@@ -193,7 +201,6 @@ def run_plain(args, ds, np, n, a, b, w, src, batches, label, cfg, traffic=False,
     value = k * args.steps / wall
     log(f"{label}: {args.steps} steps in {wall:.3f}s -> {value:.3e} changed edges/s, "
         f"p50 {pct([s['t_update_ms'] for s in stats], 50):.3f} ms")
-    extra = {}
     if extra_batches:
         # The same workload continued without per-launch timing events, whose
         # dispatch-side recording adds a few microseconds to every launch.

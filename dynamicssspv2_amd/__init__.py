@@ -144,6 +144,8 @@ def lib():
         "dsssp_batch_upload": ([p, p, i64, pp], ctypes.c_int),
         "dsssp_batch_destroy": ([p], ctypes.c_int),
         "dsssp_update_batch": ([p, p, p, ctypes.POINTER(Stats)], ctypes.c_int),
+        "dsssp_autotune": ([p, p, pp, i32, i32, ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_double)],
+                           ctypes.c_int),
         "dsssp_server_create": ([ctypes.POINTER(ServerConfig), pp], ctypes.c_int),
         "dsssp_server_add_instance": ([p, i64, i64, p, p, p, i32, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
         "dsssp_server_start": ([p], ctypes.c_int),
@@ -458,6 +460,17 @@ class Server:
             self._h = ctypes.c_void_p()
 
     __del__ = close
+
+
+def autotune(graph: Graph, tree: Tree, batches, passes: int = 1) -> tuple:
+    """Coordinate descent over the launch knobs on resident sample batches
+    (``Batch`` objects of ``graph``); graph and tree end as they started.  Sets
+    the fastest tuning and returns it with its device ms per batch."""
+    hs = (ctypes.c_void_p * len(batches))(*[b._h.value for b in batches])
+    t = Tuning()
+    ms = ctypes.c_double()
+    _check(lib().dsssp_autotune(graph._h, tree._h, hs, len(batches), passes, ctypes.byref(t), ctypes.byref(ms)))
+    return {f: getattr(t, f) for f, _ in Tuning._fields_}, ms.value
 
 
 def update(graph: Graph, tree: Tree, changes) -> dict:

@@ -376,8 +376,9 @@ int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms)
         rounds += group;
         HIP_TRY(hipGetLastError());
         const int ci = (int)(i % 3);  // flag / chunk slot written by the last round
-        HIP_TRY(hipMemcpyAsync(&t->c_host->act[ci], &t->c->act[ci], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipMemcpyAsync(&t->c_host->hcnt[ci], &t->c->hcnt[ci], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        // act[ci] .. hcnt[ci] in one copy (hcnt follows act in Counters)
+        static_assert(offsetof(Counters, hcnt) == 3 * sizeof(uint32_t), "hcnt follows act");
+        HIP_TRY(hipMemcpyAsync(&t->c_host->act[ci], &t->c->act[ci], 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         if (timed) {
             for (int j = 0; j < group; ++j) {
@@ -1433,4 +1434,130 @@ extern "C" int dsssp_update(dsssp_graph *g, dsssp_tree *t, const dsssp_change *c
     b->zero_w = zw;
     const double h2d = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return update_impl(g, t, b, stats, h2d);
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: coordinate descent over the launch knobs on sample batches.  The
+// graph and tree are snapshotted and restored around every trial, so they end
+// as they started; every trial's final tree must digest to the first trial's
+// (the knobs never change results).  No reference counterpart.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct TuneRun {
+    dsssp_graph *g;
+    dsssp_tree *t;
+    dsssp_graph *snap = nullptr;
+    uint64_t *st_save = nullptr;
+    unsigned long long *digest = nullptr;
+    bool deep = false;
+    int64_t last_rounds[8] = {};
+
+    int restore()
+    {
+        if (int s = dsssp_graph_copy(g, snap)) return s;
+        DevGuard dg(g->device);
+        HIP_TRY(hipMemcpyAsync(t->st, st_save, sizeof(uint64_t) * t->nslots, hipMemcpyDeviceToDevice, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        t->deep = deep;
+        memcpy(t->last_rounds, last_rounds, sizeof last_rounds);
+        return DSSSP_OK;
+    }
+    // Applies the batches from the snapshot; *ms = summed device time, *dig =
+    // digest of the resulting tree.
+    int trial(dsssp_batch *const *batches, int32_t nb, double *ms, unsigned long long *dig)
+    {
+        if (int s = restore()) return s;
+        double sum = 0;
+        for (int32_t i = 0; i < nb; ++i) {
+            dsssp_stats st{};
+            if (int s = dsssp_update_batch(g, t, batches[i], &st)) return s;
+            sum += st.t_update_ms;
+        }
+        DevGuard dg(g->device);
+        launch_digest(t->st, t->nslots, digest, g->stream);
+        HIP_TRY(hipMemcpyAsync(dig, digest, sizeof *dig, hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        *ms = sum;
+        return DSSSP_OK;
+    }
+};
+
+}  // namespace
+
+extern "C" int dsssp_autotune(dsssp_graph *g, dsssp_tree *t, dsssp_batch *const *batches, int32_t nb,
+                              int32_t passes, dsssp_tuning *best, double *best_ms)
+{
+    clear_error();
+    if (!g || !t || !batches || nb < 1 || passes < 1) return fail(DSSSP_ERR_ARG, "bad argument");
+    if (t->g != g) return fail(DSSSP_ERR_ARG, "tree belongs to another graph");
+    for (int32_t i = 0; i < nb; ++i)
+        if (!batches[i] || batches[i]->g != g) return fail(DSSSP_ERR_ARG, "batch %d belongs to another graph", i);
+    DevGuard dg(g->device);
+    TuneRun R{g, t};
+    const dsssp_tuning start = tuning();
+    int s = dsssp_graph_clone(g, &R.snap);
+    if (!s) s = dev_alloc((void **)&R.st_save, sizeof(uint64_t) * t->nslots, "autotune snapshot");
+    if (!s) s = dev_alloc((void **)&R.digest, sizeof(unsigned long long), "autotune digest");
+    if (!s && hipMemcpy(R.st_save, t->st, sizeof(uint64_t) * t->nslots, hipMemcpyDeviceToDevice) != hipSuccess)
+        s = fail(DSSSP_ERR_HIP, "autotune snapshot failed");
+    R.deep = t->deep;
+    memcpy(R.last_rounds, t->last_rounds, sizeof R.last_rounds);
+    dsssp_tuning cur = start;
+    cur.kernel_timing = 0;
+    cur.use_graph = 0;
+    double cur_ms = 0;
+    unsigned long long ref = 0;
+    if (!s) s = dsssp_set_tuning(&cur);
+    if (!s) s = R.trial(batches, nb, &cur_ms, &ref);
+    // Candidate values per knob; each pass tries every other value of one knob
+    // with the others at their best so far, and keeps a value that is faster by
+    // more than 1 %.
+    struct Knob {
+        int32_t dsssp_tuning::*field;
+        std::vector<int32_t> values;
+    };
+    const std::vector<Knob> knobs = {
+        {&dsssp_tuning::grid, {512, 1024, 2048}},
+        {&dsssp_tuning::edges_per_thread, {2, 4}},
+        {&dsssp_tuning::heavy_degree, {128, 256, 512}},
+        {&dsssp_tuning::tile_entries, {512, 1024, 2048}},
+        {&dsssp_tuning::local_tiles, {1, 2, 4}},
+        {&dsssp_tuning::round_batch, {4, 8}},
+        {&dsssp_tuning::delta, {-1, DSSSP_DELTA_AUTO}},
+    };
+    for (int p = 0; p < passes && !s; ++p)
+        for (const Knob &k : knobs) {
+            for (int32_t v : k.values) {
+                if (s || v == cur.*k.field) continue;
+                dsssp_tuning trial = cur;
+                trial.*k.field = v;
+                double ms = 0;
+                unsigned long long dig = 0;
+                s = dsssp_set_tuning(&trial);
+                if (!s) s = R.trial(batches, nb, &ms, &dig);
+                if (!s && dig != ref) s = fail(DSSSP_ERR_STATE, "autotune: a knob setting changed the result");
+                if (!s && ms < 0.99 * cur_ms) {
+                    cur = trial;
+                    cur_ms = ms;
+                }
+            }
+        }
+    const std::string keep = s ? last_error() : std::string();
+    int r = R.restore();  // the graph and tree as they were
+    if (!s) s = r;
+    if (R.snap) dsssp_graph_destroy(R.snap);
+    dev_free(R.st_save);
+    dev_free(R.digest);
+    dsssp_tuning fin = s ? start : cur;
+    if (!s) {
+        fin.kernel_timing = start.kernel_timing;
+        fin.use_graph = start.use_graph;
+    }
+    (void)dsssp_set_tuning(&fin);
+    if (s) return fail(s, "%s", keep.empty() ? last_error() : keep.c_str());
+    if (best) *best = fin;
+    if (best_ms) *best_ms = cur_ms / nb;
+    return DSSSP_OK;
 }

@@ -1231,6 +1231,20 @@ __global__ __launch_bounds__(RB) void k_fill_u32(uint32_t *p, uint32_t v, uint64
     for (uint64_t i = (uint64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * RB) p[i] = v;
 }
 
+// Order-independent digest of p[0, n): the sum of a 64-bit mix of (index,
+// value), one atomicAdd per wave.
+__global__ __launch_bounds__(RB) void k_digest(const uint64_t *p, uint64_t n, unsigned long long *out)
+{
+    uint64_t acc = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * RB) {
+        uint64_t h = (p[i] ^ (i * 0x9E3779B97F4A7C15ull)) * 0xBF58476D1CE4E5B9ull;
+        acc += h ^ (h >> 31);
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) acc += (uint64_t)__shfl_xor((long long)acc, d);
+    if (lane_id() == 0) atomicAdd(out, (unsigned long long)acc);
+}
+
 // Roots (parent == self) start the tight-graph BFS at level 0.
 __global__ __launch_bounds__(RB) void k_lvl_init(uint32_t nslots, uint32_t kshift, const uint64_t *st, uint32_t *lvl,
                                                  uint32_t *best, uint64_t *out_bm, uint32_t *out_act)
@@ -1325,6 +1339,14 @@ void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s)
     if (n == 0) return;
     const unsigned g = (unsigned)std::min<uint64_t>((n + RB - 1) / RB, 8192);
     hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(RB), 0, s, p, v, n);
+}
+
+void launch_digest(const uint64_t *p, uint64_t n, unsigned long long *out, hipStream_t s)
+{
+    launch_fill_u64((uint64_t *)out, 0ull, 1, s);
+    if (n == 0) return;
+    const unsigned g = (unsigned)std::min<uint64_t>((n + RB - 1) / RB, 4096);
+    hipLaunchKernelGGL(k_digest, dim3(g), dim3(RB), 0, s, p, n, out);
 }
 
 void launch_reduce_stats(Counters *c, unsigned long long *bstats, hipStream_t s)

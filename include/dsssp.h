@@ -246,6 +246,16 @@ int dsssp_update(dsssp_graph *g, dsssp_tree *t, const dsssp_change *chg, int64_t
 int dsssp_batch_upload(dsssp_graph *g, const dsssp_change *chg, int64_t k, dsssp_batch **out);
 int dsssp_batch_destroy(dsssp_batch *b);
 int dsssp_update_batch(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stats);
+/* Autotuner (no reference counterpart; the reference has fixed launch
+ * shapes, SSSPmainForUndirected.cu:200-203,229-236,271-277): coordinate descent
+ * over grid, edges_per_thread, heavy_degree, tile_entries, local_tiles,
+ * round_batch and delta, each trial applying the nb resident batches to a
+ * snapshot of g and t (restored afterwards, so both end as they started) and
+ * timed on the device; a trial whose tree differs from the first one's is an
+ * error.  Sets the fastest tuning globally and returns it with its device time
+ * per batch.  Run it while no other update is in flight. */
+int dsssp_autotune(dsssp_graph *g, dsssp_tree *t, dsssp_batch *const *batches, int32_t nb, int32_t passes,
+                   dsssp_tuning *best, double *best_ms);
 
 /* ---- served form -------------------------------------------------------------
  * Resident instances (each a graph with its own source) kept in HBM under a

@@ -438,3 +438,40 @@ def test_op_tools_sssp_matches_reference_dijkstra(tmp_path):
         p, d = ds.read_sssp(str(out), n)
         _, ed = read_ref_sssp(os.path.join(GOLDEN, f"{name}_SSSP.txt"), n)
         assert np.array_equal(d, ed)
+
+
+def test_autotune_keeps_graph_tree_and_results():
+    """The autotuner leaves graph and tree as they were, returns a valid tuning,
+    and the batches then give the oracle's tree."""
+    old = ds.get_tuning()
+    try:
+        a, b, w = ds.gen_rmat(12, 16, wlo=1, whi=1000, seed=21)
+        n = 1 << 12
+        deg = np.bincount(np.concatenate([a, b]), minlength=n)
+        src = int(np.argmax(deg))
+        batches, ca, cb, cw = [], a, b, w
+        for i in range(2):
+            c = ds.gen_changes(n, ca, cb, cw, 1500, 0.5, 1, 1000, seed=30 + i)
+            batches.append(c)
+            ca, cb, cw = oracle.apply_changes(n, ca, cb, cw, c)
+        say("autotune rmat12")
+        g = ds.Graph(n, a, b, w)
+        t = ds.Tree(g, src)
+        p0, d0 = t.read()
+        e0 = edge_multiset(*g.export())
+        res = [ds.Batch(g, c) for c in batches]
+        tun, ms = ds.autotune(g, t, res)
+        assert ms > 0 and tun["grid"] >= 1 and tun["edges_per_thread"] in (1, 2, 4)
+        assert ds.get_tuning() == tun
+        p1, d1 = t.read()
+        assert np.array_equal(p0, p1) and np.array_equal(d0, d1)
+        assert np.array_equal(edge_multiset(*g.export()), e0)
+        m = Mirror(n, a, b, w)
+        for r, c in zip(res, batches):
+            ds.update(g, t, r)
+            m.apply(c)
+        check_tree(m, *t.read(), [src], "after autotune")
+        for r in res:
+            r.close()
+    finally:
+        ds.set_tuning(**old)
