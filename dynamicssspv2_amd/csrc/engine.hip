@@ -1388,20 +1388,9 @@ extern "C" int dsssp_update_batch(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b,
     return update_impl(g, t, b, stats, 0.0);
 }
 
-extern "C" int dsssp_update(dsssp_graph *g, dsssp_tree *t, const dsssp_change *chg, int64_t k, dsssp_stats *stats)
+// The graph's reusable device batch, with room for k records.
+static int ensure_batch(dsssp_graph *g, int64_t k)
 {
-    clear_error();
-    if (!g || !t || k < 0 || (k > 0 && !chg)) return fail(DSSSP_ERR_ARG, "bad argument");
-    if (t->g != g) return fail(DSSSP_ERR_ARG, "tree belongs to another graph");
-    const auto t0 = std::chrono::steady_clock::now();
-    int64_t ni, nd;
-    bool zw;
-    if (int s = validate_changes(g->n, chg, k, ni, nd, zw)) return s;
-    if (k == 0) {
-        if (stats) *stats = dsssp_stats{};
-        return DSSSP_OK;
-    }
-    DevGuard dg(g->device);
     if (!g->upd) {
         g->upd = new dsssp_batch();
         g->upd->g = g;  // owned by the graph, not attached to it
@@ -1417,6 +1406,12 @@ extern "C" int dsssp_update(dsssp_graph *g, dsssp_tree *t, const dsssp_change *c
         if (int s = dev_alloc((void **)&b->tslot, sizeof(uint32_t) * 2 * (size_t)cap, "batch slots")) return s;
         b->cap = cap;
     }
+    return DSSSP_OK;
+}
+
+// The graph's pinned staging buffer, with room for k records.
+static int ensure_stage(dsssp_graph *g, int64_t k)
+{
     if (g->stage_cap < k) {
         const int64_t cap = std::max<int64_t>(k, 2 * g->stage_cap);
         HIP_TRY(hipStreamSynchronize(g->stream));
@@ -1426,8 +1421,14 @@ extern "C" int dsssp_update(dsssp_graph *g, dsssp_tree *t, const dsssp_change *c
         HIP_TRY(hipHostMalloc((void **)&g->stage, sizeof(dsssp_change) * (size_t)cap, hipHostMallocDefault));
         g->stage_cap = cap;
     }
-    memcpy(g->stage, chg, sizeof(dsssp_change) * (size_t)k);
-    HIP_TRY(hipMemcpyAsync(b->rec, g->stage, sizeof(int4) * (size_t)k, hipMemcpyHostToDevice, g->stream));
+    return DSSSP_OK;
+}
+
+// Applies the k records whose upload to the device batch is enqueued.
+static int apply_uploaded(dsssp_graph *g, dsssp_tree *t, int64_t k, int64_t ni, int64_t nd, bool zw,
+                          dsssp_stats *stats, std::chrono::steady_clock::time_point t0)
+{
+    dsssp_batch *b = g->upd;
     b->k = k;
     b->n_ins = ni;
     b->n_del = nd;
@@ -1435,6 +1436,73 @@ extern "C" int dsssp_update(dsssp_graph *g, dsssp_tree *t, const dsssp_change *c
     const double h2d = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return update_impl(g, t, b, stats, h2d);
 }
+
+extern "C" int dsssp_update(dsssp_graph *g, dsssp_tree *t, const dsssp_change *chg, int64_t k, dsssp_stats *stats)
+{
+    clear_error();
+    if (!g || !t || k < 0 || (k > 0 && !chg)) return fail(DSSSP_ERR_ARG, "bad argument");
+    if (t->g != g) return fail(DSSSP_ERR_ARG, "tree belongs to another graph");
+    const auto t0 = std::chrono::steady_clock::now();
+    int64_t ni, nd;
+    bool zw;
+    if (int s = validate_changes(g->n, chg, k, ni, nd, zw)) return s;
+    if (k == 0) {
+        if (stats) *stats = dsssp_stats{};
+        return DSSSP_OK;
+    }
+    DevGuard dg(g->device);
+    if (int s = ensure_batch(g, k)) return s;
+    if (int s = ensure_stage(g, k)) return s;
+    memcpy(g->stage, chg, sizeof(dsssp_change) * (size_t)k);
+    HIP_TRY(hipMemcpyAsync(g->upd->rec, g->stage, sizeof(int4) * (size_t)k, hipMemcpyHostToDevice, g->stream));
+    return apply_uploaded(g, t, k, ni, nd, zw, stats, t0);
+}
+
+namespace dsssp {
+
+int pinned_alloc(void **p, size_t bytes)
+{
+    *p = nullptr;
+    if (hipHostMalloc(p, bytes, hipHostMallocPortable) != hipSuccess) {
+        *p = nullptr;
+        return fail(DSSSP_ERR_NOMEM, "pinned host allocation of %zu bytes failed: %s", bytes,
+                    hipGetErrorString(hipGetLastError()));
+    }
+    return DSSSP_OK;
+}
+
+void pinned_free(void *p)
+{
+    if (p) (void)hipHostFree(p);
+}
+
+int update_pinned(dsssp_graph *g, dsssp_tree *t, const dsssp_change *const *buf, const int64_t *len, int32_t nbuf,
+                  int64_t ni, int64_t nd, bool zw, dsssp_stats *stats)
+{
+    clear_error();
+    if (!g || !t || nbuf < 0 || (nbuf > 0 && (!buf || !len))) return fail(DSSSP_ERR_ARG, "bad argument");
+    if (t->g != g) return fail(DSSSP_ERR_ARG, "tree belongs to another graph");
+    const auto t0 = std::chrono::steady_clock::now();
+    int64_t k = 0;
+    for (int32_t i = 0; i < nbuf; ++i) k += len[i];
+    if (k == 0) {
+        if (stats) *stats = dsssp_stats{};
+        return DSSSP_OK;
+    }
+    DevGuard dg(g->device);
+    if (int s = ensure_batch(g, k)) return s;
+    dsssp_batch *b = g->upd;
+    int64_t pos = 0;
+    for (int32_t i = 0; i < nbuf; ++i) {
+        if (len[i])
+            HIP_TRY(hipMemcpyAsync(b->rec + pos, buf[i], sizeof(int4) * (size_t)len[i], hipMemcpyHostToDevice,
+                                   g->stream));
+        pos += len[i];
+    }
+    return apply_uploaded(g, t, k, ni, nd, zw, stats, t0);
+}
+
+}  // namespace dsssp
 
 // ---------------------------------------------------------------------------
 // Autotuner: coordinate descent over the launch knobs on sample batches.  The

@@ -8,7 +8,19 @@
 #include <thread>
 #include <vector>
 
+#include "../../include/dsssp.h"
+
 namespace dsssp {
+
+// Served form: applies the concatenation of nbuf pinned host record buffers to
+// (g, t), already in g's vertex ids and validated, with ni insertions, nd
+// deletions and zw set when an insertion has weight 0 (engine.hip).  The
+// buffers may be reused once it returns.
+int update_pinned(dsssp_graph *g, dsssp_tree *t, const dsssp_change *const *buf, const int64_t *len, int32_t nbuf,
+                  int64_t ni, int64_t nd, bool zw, dsssp_stats *stats);
+// Page-locked host memory (hipHostMalloc / hipHostFree).
+int pinned_alloc(void **p, size_t bytes);
+void pinned_free(void *p);
 
 // Records the calling thread's last error and returns code.
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));

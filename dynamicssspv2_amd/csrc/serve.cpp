@@ -46,7 +46,12 @@ struct Instance {
 struct Request {
     uint64_t ticket = 0;
     int32_t inst = 0;
-    std::vector<dsssp_change> chg;
+    // The records, staged at submission: pinned host memory, in the lane
+    // graph's vertex ids, counted.
+    dsssp_change *buf = nullptr;
+    int64_t cap = 0;   // > 0: a buffer of its own; 0: a slice of the arena
+    int64_t k = 0, ni = 0, nd = 0;
+    bool zw = false;
     sclock::time_point submit, start, finish;
     int status = DSSSP_OK;
     std::string error;
@@ -91,6 +96,19 @@ struct dsssp_server {
     std::condition_variable cv_done;
     std::map<uint64_t, Request *> all;
     uint64_t next_ticket = 1;
+    // Pinned staging of submitted records: slices of one arena, allocated in
+    // submission order and released as requests finish (the oldest live slice
+    // bounds the free space); a request that does not fit gets a buffer of its
+    // own, kept by capacity for reuse.
+    std::mutex pool_mu;
+    dsssp_change *arena = nullptr;
+    int64_t arena_cap = 0, head = 0;
+    struct Slice {
+        int64_t start, len;
+        bool live;
+    };
+    std::deque<Slice> slices;
+    std::multimap<int64_t, dsssp_change *> pool;
     // totals
     int64_t launches = 0, records = 0, requests_done = 0;
     double busy_ms = 0;
@@ -98,10 +116,73 @@ struct dsssp_server {
 
 namespace {
 
+constexpr int64_t ARENA_RECORDS = int64_t(1) << 22;  // 64 MiB of pinned staging
+
+// Pinned room for k records: an arena slice (*cap = 0) when one fits, else a
+// buffer of its own (from the pool, or a new one of a power-of-two capacity).
+int take_buffer(dsssp_server *s, int64_t k, dsssp_change **buf, int64_t *cap)
+{
+    {
+        std::lock_guard<std::mutex> lk(s->pool_mu);
+        if (s->arena && k > 0) {
+            if (s->slices.empty()) s->head = 0;
+            const int64_t tail = s->slices.empty() ? 0 : s->slices.front().start;
+            int64_t at = -1;
+            if (s->slices.empty() || s->head > tail) {  // free: [head, cap) and [0, tail)
+                if (s->head + k <= s->arena_cap)
+                    at = s->head;
+                else if (k <= tail)
+                    at = 0;
+            } else if (s->head + k <= tail) {  // wrapped: free [head, tail)
+                at = s->head;
+            }
+            if (at >= 0) {
+                s->slices.push_back({at, k, true});
+                s->head = at + k;
+                *buf = s->arena + at;
+                *cap = 0;
+                return DSSSP_OK;
+            }
+        }
+        auto it = s->pool.lower_bound(k);
+        if (it != s->pool.end()) {
+            *cap = it->first;
+            *buf = it->second;
+            s->pool.erase(it);
+            return DSSSP_OK;
+        }
+    }
+    int64_t c = 1024;
+    while (c < k) c *= 2;
+    void *p = nullptr;
+    if (int rc = pinned_alloc(&p, sizeof(dsssp_change) * (size_t)c)) return rc;
+    *buf = (dsssp_change *)p;
+    *cap = c;
+    return DSSSP_OK;
+}
+
+void give_buffer(dsssp_server *s, dsssp_change *buf, int64_t cap)
+{
+    if (!buf) return;
+    std::lock_guard<std::mutex> lk(s->pool_mu);
+    if (cap > 0) {
+        s->pool.emplace(cap, buf);
+        return;
+    }
+    const int64_t at = buf - s->arena;
+    for (auto &sl : s->slices)
+        if (sl.start == at && sl.live) {
+            sl.live = false;
+            break;
+        }
+    while (!s->slices.empty() && !s->slices.front().live) s->slices.pop_front();
+}
+
 void serve_loop(dsssp_server *s, Lane *L)
 {
     (void)dsssp_set_device(s->device);
-    std::vector<dsssp_change> merged;  // reused: grows to the largest merge
+    std::vector<const dsssp_change *> ptrs;  // reused per merged update
+    std::vector<int64_t> lens;
     for (;;) {
         std::vector<Request *> pick;
         {
@@ -113,11 +194,10 @@ void serve_loop(dsssp_server *s, Lane *L)
             for (auto it = L->pending.begin(); it != L->pending.end();) {
                 Request *r = *it;
                 const bool room = (s->cfg.max_merge <= 0 || (int32_t)pick.size() < s->cfg.max_merge) &&
-                                  (pick.empty() || s->cfg.max_records <= 0 ||
-                                   recs + (int64_t)r->chg.size() <= s->cfg.max_records);
+                                  (pick.empty() || s->cfg.max_records <= 0 || recs + r->k <= s->cfg.max_records);
                 if (!taken[r->inst] && room) {
                     taken[r->inst] = 1;
-                    recs += (int64_t)r->chg.size();
+                    recs += r->k;
                     pick.push_back(r);
                     it = L->pending.erase(it);
                 } else {
@@ -127,28 +207,31 @@ void serve_loop(dsssp_server *s, Lane *L)
             }
         }
         const auto t0 = sclock::now();
-        size_t total = 0;
-        for (Request *r : pick) total += r->chg.size();
-        merged.clear();
-        merged.reserve(total);
+        // The picked requests' staged records are uploaded one after another.
+        ptrs.clear();
+        lens.clear();
+        int64_t nrec = 0, ni = 0, nd = 0;
+        bool zw = false;
         for (Request *r : pick) {
             r->start = t0;
-            const int32_t off = (int32_t)s->inst[r->inst].offset;
-            for (dsssp_change c : r->chg) {
-                if (c.op == 0 || c.op == 1) {
-                    c.a += off;
-                    c.b += off;
-                }
-                merged.push_back(c);
-            }
+            ptrs.push_back(r->buf);
+            lens.push_back(r->k);
+            nrec += r->k;
+            ni += r->ni;
+            nd += r->nd;
+            zw = zw || r->zw;
         }
         dsssp_stats st{};
         int rc;
         std::string err;
         {
             std::lock_guard<std::mutex> use(L->use);
-            rc = dsssp_update(L->g, L->t, merged.data(), (int64_t)merged.size(), &st);
+            rc = update_pinned(L->g, L->t, ptrs.data(), lens.data(), (int32_t)pick.size(), ni, nd, zw, &st);
             if (rc) err = dsssp_last_error();
+        }
+        for (Request *r : pick) {  // the uploads are complete when the update returns
+            give_buffer(s, r->buf, r->cap);
+            r->buf = nullptr;
         }
         const auto t1 = sclock::now();
         {
@@ -162,7 +245,7 @@ void serve_loop(dsssp_server *s, Lane *L)
                 r->done = true;
             }
             s->launches += 1;
-            s->records += (int64_t)merged.size();
+            s->records += nrec;
             s->requests_done += (int64_t)pick.size();
             s->busy_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
         }
@@ -296,6 +379,16 @@ extern "C" int dsssp_server_start(dsssp_server *s)
         std::vector<int32_t>().swap(x.b);
         std::vector<int32_t>().swap(x.w);
     }
+    {
+        void *p = nullptr;
+        if (pinned_alloc(&p, sizeof(dsssp_change) * (size_t)ARENA_RECORDS) == DSSSP_OK) {
+            s->arena = (dsssp_change *)p;
+            s->arena_cap = ARENA_RECORDS;
+            // first touch now rather than in the first submissions
+            memset(s->arena, 0, sizeof(dsssp_change) * (size_t)ARENA_RECORDS);
+        }
+        clear_error();  // without the arena every request gets a buffer of its own
+    }
     s->started = true;
     for (Lane &L : s->lanes) L.loop = std::thread(serve_loop, s, &L);
     return DSSSP_OK;
@@ -305,28 +398,52 @@ extern "C" int dsssp_server_submit(dsssp_server *s, int32_t id, const dsssp_chan
 {
     clear_error();
     if (!s || !ticket || k < 0 || (k > 0 && !chg)) return fail(DSSSP_ERR_ARG, "bad argument");
-    int64_t n = 0;
+    int64_t n = 0, off = 0;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (!s->started || s->stopping) return fail(DSSSP_ERR_STATE, "server not running");
         if (id < 0 || id >= (int32_t)s->inst.size()) return fail(DSSSP_ERR_ARG, "unknown instance %d", id);
         n = s->inst[id].n;  // fixed once started
+        off = s->inst[id].offset;
     }
-    // Validation and the copy of the records happen outside the lock, so a
-    // large submission does not hold up the loops or other submitters.
-    for (int64_t i = 0; i < k; ++i) {
-        const dsssp_change &c = chg[i];
-        if (c.op != 0 && c.op != 1) continue;
-        if (c.a < 0 || c.a >= n || c.b < 0 || c.b >= n)
-            return fail(DSSSP_ERR_RANGE, "change %lld: vertex outside [0,%lld) of instance %d", (long long)i,
-                        (long long)n, id);
-        if (c.w < 0) return fail(DSSSP_ERR_WEIGHT, "change %lld: negative weight", (long long)i);
-    }
+    // Validation and staging happen here, in the submitting thread, outside the
+    // lock: the records are checked, shifted to the lane graph's ids and counted
+    // straight into a pinned buffer, so a loop only enqueues their upload.
     Request *r = new Request();
     r->inst = id;
-    r->chg.assign(chg, chg + k);
+    r->k = k;
+    if (k > 0) {
+        if (int rc = take_buffer(s, k, &r->buf, &r->cap)) {
+            delete r;
+            return rc;
+        }
+    }
+    for (int64_t i = 0; i < k; ++i) {
+        dsssp_change c = chg[i];
+        if (c.op == 0 || c.op == 1) {
+            if (c.a < 0 || c.a >= n || c.b < 0 || c.b >= n || c.w < 0) {
+                give_buffer(s, r->buf, r->cap);
+                delete r;
+                if (c.w < 0 && c.a >= 0 && c.a < n && c.b >= 0 && c.b < n)
+                    return fail(DSSSP_ERR_WEIGHT, "change %lld: negative weight", (long long)i);
+                return fail(DSSSP_ERR_RANGE, "change %lld: vertex outside [0,%lld) of instance %d", (long long)i,
+                            (long long)n, id);
+            }
+            c.a += (int32_t)off;
+            c.b += (int32_t)off;
+            if (c.op == 1) {
+                ++r->ni;
+                if (c.w == 0) r->zw = true;
+            } else {
+                ++r->nd;
+            }
+        }
+        r->buf[i] = c;
+    }
     std::unique_lock<std::mutex> lk(s->mu);
     if (s->stopping) {
+        lk.unlock();
+        give_buffer(s, r->buf, r->cap);
         delete r;
         return fail(DSSSP_ERR_STATE, "server not running");
     }
@@ -354,7 +471,7 @@ extern "C" int dsssp_server_wait(dsssp_server *s, uint64_t ticket, dsssp_request
     const std::string err = r->error;
     if (st) {
         st->instance = r->inst;
-        st->records = (int64_t)r->chg.size();
+        st->records = r->k;
         st->merged = r->merged;
         st->queue_ms = std::chrono::duration<double, std::milli>(r->start - r->submit).count();
         st->latency_ms = std::chrono::duration<double, std::milli>(r->finish - r->submit).count();
@@ -422,7 +539,12 @@ extern "C" int dsssp_server_destroy(dsssp_server *s)
     clear_error();
     if (!s) return DSSSP_OK;
     stop_lanes(s);
-    for (auto &kv : s->all) delete kv.second;
+    for (auto &kv : s->all) {
+        if (kv.second->cap > 0) pinned_free(kv.second->buf);
+        delete kv.second;
+    }
+    for (auto &kv : s->pool) pinned_free(kv.second);
+    pinned_free(s->arena);
     delete s;
     return DSSSP_OK;
 }
