@@ -393,15 +393,20 @@ def config5(args, ds, np):
     scales = [20 + (i * 5) // args.instances for i in range(args.instances)]  # 20..24 spread evenly
     if args.scale < 24:
         scales = [min(s, args.scale) for s in scales]
-    srv = ds.Server(mem_budget_bytes=96 << 30, lanes=args.lanes)
+    budget = 160 << 30
+    srv = ds.Server(mem_budget_bytes=budget, lanes=args.lanes)
     insts = []
     for i, sc in enumerate(scales):
         a, b, w = ds.gen_rmat(sc, 16, (0.57, 0.19, 0.19), 1, 1000, seed=args.seed + i)
         n = 1 << sc
         deg = np.bincount(np.concatenate([a, b]), minlength=n)
         src = int(np.argmax(deg))
+        try:
+            srv.add_instance(n, a, b, w, src)
+        except ds.DssspError as e:  # admission by memory budget
+            log(f"instance {i}: s{sc} not admitted ({e})")
+            continue
         insts.append((n, a, b, w, src))
-        srv.add_instance(n, a, b, w, src)
         log(f"instance {i}: s{sc} m={len(a)}")
     t0 = time.time()
     srv.start()

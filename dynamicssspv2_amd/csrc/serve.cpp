@@ -63,6 +63,11 @@ struct Request {
 // Device bytes of an instance: CSR, delta descriptors/capacities/counters and
 // bitmaps, the initial delta pool (E/2 + 12 n + 2^16 entries), the edge map, and the
 // tree (state, bitmaps, chunk lists).
+// What an instance adds to its lane graph's delta pool (half its entries plus
+// the initial delta blocks), which 32-bit offsets address: LANE_ROWS at most.
+uint64_t lane_rows(int64_t n, int64_t m) { return (uint64_t)m + 12 * (uint64_t)n; }
+constexpr uint64_t LANE_ROWS = 0x7FFFFFFFull - (1u << 20);
+
 uint64_t instance_bytes(int64_t n, int64_t m)
 {
     const uint64_t E = 2 * (uint64_t)m, N = (uint64_t)n;
@@ -294,6 +299,9 @@ extern "C" int dsssp_server_add_instance(dsssp_server *s, int64_t n, int64_t m, 
     if (s->started) return fail(DSSSP_ERR_STATE, "instances are admitted before dsssp_server_start");
     if (src < 0 || src >= n) return fail(DSSSP_ERR_RANGE, "source %d outside [0,%lld)", src, (long long)n);
     const uint64_t bytes = instance_bytes(n, m);
+    if (lane_rows(n, m) > LANE_ROWS)
+        return fail(DSSSP_ERR_BUDGET, "instance too large for one device graph (n=%lld, m=%lld)", (long long)n,
+                    (long long)m);
     if (s->used + bytes > s->cfg.mem_budget_bytes)
         return fail(DSSSP_ERR_BUDGET, "instance needs %llu bytes, %llu of %llu left", (unsigned long long)bytes,
                     (unsigned long long)(s->cfg.mem_budget_bytes - s->used),
@@ -323,19 +331,33 @@ extern "C" int dsssp_server_start(dsssp_server *s)
     if (s->started) return fail(DSSSP_ERR_STATE, "already started");
     if (s->inst.empty()) return fail(DSSSP_ERR_STATE, "no instance admitted");
     const int ninst = (int)s->inst.size();
-    const int nl = std::max(1, std::min(ninst, s->cfg.lanes > 0 ? (int)s->cfg.lanes : 1));
-    // Largest instances first, each to the lane with the fewest bytes so far.
+    const int want = std::max(1, std::min(ninst, s->cfg.lanes > 0 ? (int)s->cfg.lanes : 1));
+    // Largest instances first, each to the lane with the fewest bytes so far
+    // among those its rows still fit in (a device graph's delta pool is
+    // addressed with 32 bits); a new lane when none does.
     std::vector<int32_t> order((size_t)ninst);
     for (int i = 0; i < ninst; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(),
                      [&](int32_t x, int32_t y) { return s->inst[x].bytes > s->inst[y].bytes; });
-    std::vector<uint64_t> load((size_t)nl, 0);
-    std::vector<std::vector<int32_t>> members((size_t)nl);
+    std::vector<uint64_t> load, rows;
+    std::vector<std::vector<int32_t>> members;
     for (int32_t i : order) {
-        const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        load[l] += s->inst[i].bytes;
-        members[l].push_back(i);
+        const uint64_t r = lane_rows(s->inst[i].n, s->inst[i].m);
+        int best = -1;
+        if ((int)load.size() >= want)
+            for (int l = 0; l < (int)load.size(); ++l)
+                if (rows[l] + r <= LANE_ROWS && (best < 0 || load[l] < load[best])) best = l;
+        if (best < 0) {
+            best = (int)load.size();
+            load.push_back(0);
+            rows.push_back(0);
+            members.emplace_back();
+        }
+        load[best] += s->inst[i].bytes;
+        rows[best] += r;
+        members[best].push_back(i);
     }
+    const int nl = (int)members.size();
     s->lanes = std::vector<Lane>((size_t)nl);
     int rc = DSSSP_OK;
     for (int l = 0; l < nl && !rc; ++l) {

@@ -270,7 +270,8 @@ typedef struct dsssp_server dsssp_server;
 typedef struct {
     uint64_t mem_budget_bytes;  /* device bytes all admitted instances may use      */
     int32_t max_merge;          /* most requests merged into one launch (0: no cap) */
-    int32_t lanes;              /* concurrent lanes, at most one per instance (0: 1) */
+    int32_t lanes;              /* concurrent lanes (0: 1); more when the instances
+                                   do not fit one device graph's 32-bit pool offsets */
     int64_t max_records;        /* most records per merged launch (0: no cap)       */
 } dsssp_server_config;
 typedef struct {
