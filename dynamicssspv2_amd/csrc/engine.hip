@@ -1,12 +1,13 @@
 // engine.hip -- device objects and the update pipeline behind the C ABI.
 //
 // Update pipeline (one HIP stream per graph, no host round-trip per round):
-//   1. insert   k_ins_count / wordsum / scan / alloc / scatter / reset: the
-//               batch's insertions become delta-row entries in HBM (the
-//               reference appends them to its host adjacency before upload,
+//   1. insert   k_ins_place / wordsum / scan / alloc / pending: the batch's
+//               insertions become delta-row entries in HBM (the reference
+//               appends them to its host adjacency before upload,
 //               all_structure_undir.cuh:106-117)
-//   2. classify k_classify: tree-edge test + tombstones; roots of invalidated
-//               subtrees marked (deleteEdge, gpuFunctions_undir.cuh:179-215)
+//   2. classify k_classify: tree-edge test, roots of invalidated subtrees
+//               marked; k_tomb: every copy of each deleted entry tombstoned
+//               (deleteEdge, gpuFunctions_undir.cuh:179-215)
 //   3. invalidate rounds of k_frontier<M_INVAL>, descending the pre-update tree
 //               (updateNeighbors_del + filterAffectedNodes loop,
 //               SSSPmainForUndirected.cu:219-239, gpuFunctions_undir.cuh:125-144,312-349)

@@ -20,8 +20,13 @@
 //  * Distance and parent live in one uint64 (dist << 32 | parent) and change
 //    together through one 64-bit atomicMin, so the smaller distance wins and,
 //    among equal distances, the smaller parent id.  Every atomic in the edge
-//    loop is non-returning; the only returning atomics left are one per heavy
-//    row (chunk list) and the per-vertex insertion counters.
+//    loop is non-returning; the returning ones are one per tile with split
+//    rows (item list), the per-row insertion counters and the edge-map claims
+//    of delta entries past a row's first few.
+//  * A wave keeps the slots it has just marked on an LDS stack and visits a
+//    bounded number of tiles of them in the same launch; with a distance step
+//    (delta) a round visits only slots below a threshold that every workgroup
+//    derives from the previous round's marks.
 //  * Statistics go to one slot per workgroup and are summed once per update;
 //    no kernel issues device-scope fences.
 // The reference does one thread per frontier vertex with unsynchronised

@@ -98,12 +98,14 @@ typedef struct {
 /* Tuning knobs (zero fields keep the current value). */
 typedef struct {
     int32_t block;            /* threads per workgroup of frontier kernels (fixed 256) */
-    int32_t grid;             /* workgroups per frontier launch                        */
+    int32_t grid;             /* least workgroups per frontier launch (more when the
+                                 bitmap needs them to be read in one pass)          */
     int32_t heavy_degree;     /* a tile (the set slots of one 64-bit frontier word)
                                  with more row entries than this is split           */
     int32_t chunk;            /* entries per split piece (multiple of 64)              */
-    int32_t round_batch;      /* rounds launched before the first convergence check;
-                                 the group doubles (up to 64) while rounds continue   */
+    int32_t round_batch;      /* rounds per group between convergence checks; a
+                                 loop's first group follows the depth of the last
+                                 loop of its kind, later groups double up to 64      */
     int32_t kernel_timing;    /* 1: time every relaxation launch with events           */
     int32_t use_graph;        /* 1: launch round groups as captured hipGraphs          */
     int32_t edges_per_thread; /* adjacency entries in flight per lane: 1, 2 or 4       */
