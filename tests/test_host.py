@@ -182,6 +182,12 @@ def test_server_admission_by_budget():
         srv.add_instance(10, [0], [1], [1], 99)
     assert e.value.code == ds.ERR_RANGE
     srv.close()
+    # an instance whose rows one device graph's 32-bit pool offsets cannot address
+    big = ds.Server(mem_budget_bytes=1 << 50)
+    with pytest.raises(ds.DssspError) as e:
+        big.add_instance(200_000_000, [0], [1], [1], 0)
+    assert e.value.code == ds.ERR_BUDGET and "too large" in str(e.value)
+    big.close()
 
 
 def test_gen_changes_grid():
