@@ -66,7 +66,7 @@ def parse():
     p.add_argument("--copy-probe", action="store_true",
                    help="only copy 1 GiB device-to-device five times (calibrates HBM byte counters), then exit")
     for knob in ("grid", "heavy_degree", "chunk", "round_batch", "use_graph", "edges_per_thread", "tile_entries",
-                 "local_tiles", "delta"):
+                 "local_tiles", "delta", "sweep"):
         p.add_argument("--" + knob.replace("_", "-"), type=int, default=None, help="tuning knob (dsssp_tuning)")
     return p.parse_args()
 
@@ -148,6 +148,18 @@ def roofline(stats, traffic_applies=False):
     return out
 
 
+def full_sweep(fs):
+    """Roofline figures of dsssp_tree_relax_full: one relaxation launch over
+    every slot (32 B per slot, 16 B per scanned entry, 16 B per improvement, 8 B
+    per bitmap word) plus the empty convergence round(s) after it."""
+    ms = fs["t_relax_kernel_ms"]
+    ach = fs["alg_bytes_relax"] / (ms / 1e3) / 1e9 if ms > 0 else None
+    return {"kernel": "k_frontier<M_RELAX> over every slot", "achieved": ach, "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s", "frac": ach / HBM_PEAK_GBPS if ach else None, "kernel_ms": ms,
+            "launches": fs["relax_launches"], "alg_bytes": fs["alg_bytes_relax"],
+            "slots": fs["frontier_vertices"], "entries": fs["relax_edges"], "improved": fs["relax_improved"]}
+
+
 def line(args, value, wall, steps, cfg, stats, check, cpu, unit="changed edges/s", extra=None, traffic=False):
     import numpy as np
     lat = [s["t_update_ms"] for s in stats]
@@ -158,7 +170,7 @@ def line(args, value, wall, steps, cfg, stats, check, cpu, unit="changed edges/s
         "vs_baseline": None, "dtype": "u32", "data": "synthetic", "config": cfg,
         "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
         "update_alg_GBps": alg / (sum(lat) / 1e3) / 1e9 if sum(lat) > 0 else None,
-        "dist_bitexact": bool(check.get("gpu_recompute_equal", True)
+        "dist_bitexact": bool(check.get("gpu_recompute_equal", True) and check.get("full_sweep_fixed_point", True)
                               and check.get("dijkstra_mismatches", 0) == 0 and check.get("certificate", 0) == 0),
         "check": check, "roofline": roofline(stats, traffic), "cpu_baseline": cpu,
         "phases_ms": {ph: sum(s[f"t_{ph}_ms"] for s in stats) / max(1, len(stats))
@@ -211,13 +223,23 @@ def run_plain(args, ds, np, n, a, b, w, src, batches, label, cfg, traffic=False,
         extra["value_without_launch_events"] = k * len(extra_batches) / wall2
         extra["p50_ms_without_launch_events"] = pct([s["t_update_ms"] for s in st2], 50)
         log(f"{label}: without launch events {extra['value_without_launch_events']:.3e} changed edges/s")
+    # One full relaxation sweep over the converged tree: every live entry of
+    # every reachable vertex, rows in vertex order.  Nothing may change (a
+    # device-side fixed-point check); its launch is the kernel's streaming case.
+    ds.set_tuning(kernel_timing=0 if args.no_timing else 1)
+    fs = t.relax_full()
+    ds.set_tuning(kernel_timing=0)
+    extra["full_sweep"] = full_sweep(fs)
+    log(f"{label}: full sweep {fs['t_relax_kernel_ms']:.3f} ms kernel, "
+        f"{extra['full_sweep']['achieved']} GB/s algorithmic, improved {fs['relax_improved']}")
     par, dist = t.read()
     t0 = time.perf_counter()
     ft = ds.Tree(g, src)  # from scratch on the GPU: the recompute the update replaces
     recompute_ms = (time.perf_counter() - t0) * 1e3
     fpar, fdist = ft.read()
     del ft
-    check = {"gpu_recompute_equal": bool(np.array_equal(dist, fdist) and np.array_equal(par, fpar))}
+    check = {"gpu_recompute_equal": bool(np.array_equal(dist, fdist) and np.array_equal(par, fpar)),
+             "full_sweep_fixed_point": fs["relax_improved"] == 0}
     extra["gpu_recompute_ms"] = recompute_ms
     log(f"{label}: GPU tree from scratch on the final graph {recompute_ms:.2f} ms")
     cpu = None

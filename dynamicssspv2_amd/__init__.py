@@ -48,7 +48,8 @@ class Stats(ctypes.Structure):
         "frontier_vertices", "relax_edges", "relax_improved", "relax_enqueued", "other_edges", "probes")] + \
         [(f, ctypes.c_double) for f in (
             "t_h2d_ms", "t_update_ms", "t_insert_ms", "t_classify_ms", "t_invalidate_ms", "t_seed_ms",
-            "t_relax_ms", "t_relax_kernel_ms", "alg_bytes", "alg_bytes_relax")]
+            "t_relax_ms", "t_relax_kernel_ms", "alg_bytes", "alg_bytes_relax")] + \
+        [("relax_sweeps", ctypes.c_int64)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -82,7 +83,7 @@ DELTA_AUTO = -2  # Tuning.delta: ordered relaxation chosen per tree (include/dss
 class Tuning(ctypes.Structure):
     _fields_ = [(f, ctypes.c_int32) for f in ("block", "grid", "heavy_degree", "chunk", "round_batch",
                                               "kernel_timing", "use_graph", "edges_per_thread",
-                                              "tile_entries", "local_tiles", "delta")]
+                                              "tile_entries", "local_tiles", "delta", "sweep")]
 
 
 def build(force: bool = True) -> None:
@@ -136,6 +137,7 @@ def lib():
         "dsssp_tree_destroy": ([p], ctypes.c_int),
         "dsssp_tree_read": ([p, p, p], ctypes.c_int),
         "dsssp_tree_copy": ([p, p], ctypes.c_int),
+        "dsssp_tree_relax_full": ([p, ctypes.POINTER(Stats)], ctypes.c_int),
         "dsssp_tree_create_roots": ([p, i32, p, u32, pp], ctypes.c_int),
         "dsssp_forest_create": ([p, i32, p, u32, pp], ctypes.c_int),
         "dsssp_tree_count": ([p, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
@@ -389,6 +391,14 @@ class Tree:
 
     def copy_from(self, other: "Tree") -> None:
         _check(lib().dsssp_tree_copy(self._h, other._h))
+
+    def relax_full(self) -> dict:
+        """One full relaxation sweep (every reachable slot), then rounds until
+        nothing changes; on a converged tree nothing changes
+        (``relax_improved == 0``).  Returns the relaxation statistics."""
+        st = Stats()
+        _check(lib().dsssp_tree_relax_full(self._h, ctypes.byref(st)))
+        return st.as_dict()
 
 
 class Batch:

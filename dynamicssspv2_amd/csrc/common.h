@@ -77,6 +77,7 @@ enum Stat : int {
     ST_OTHER_EDGES,      // entries scanned by invalidation, seeding, repair
     ST_INVALIDATED,      // slots seeded (= invalidated by the update)
     ST_PROBES,           // entries read by deletion searches
+    ST_SWEEPS,           // relaxation rounds run as full sweeps (every slot visited)
     NSTAT
 };
 constexpr int STAT_SLOTS = 2048;
@@ -90,6 +91,9 @@ struct Counters {
     uint32_t near[3];      // a round marked a slot below its threshold
     uint32_t thr[3];       // the threshold a round used
     uint32_t fenc[3][64];  // ~(smallest distance marked at or above it), max-reduced over 64 words
+    // Push-vs-full-sweep: slots a relaxation round marked in its output bitmap,
+    // spread over 64 words (summed by the next round); round % 3.
+    uint32_t nmark[3][64];
     uint32_t aux_hcnt;  // heavy chunks queued by a single-pass stage
     uint32_t err;
     unsigned long long stats[NSTAT];

@@ -43,7 +43,7 @@ using namespace dsssp;
 
 namespace {
 
-dsssp_tuning g_tuning = {FRONTIER_BLOCK, 1024, 256, 256, 4, 0, 0, 4, 1024, 2, DSSSP_DELTA_AUTO};
+dsssp_tuning g_tuning = {FRONTIER_BLOCK, 1024, 256, 256, 4, 0, 0, 4, 1024, 2, DSSSP_DELTA_AUTO, -1};
 std::mutex g_tuning_mu;
 thread_local int t_device = 0;
 
@@ -335,7 +335,7 @@ int captured_rounds(dsssp_tree *t, int mode, int64_t first, int group, int grid,
 // Rounds run back to back on the stream (replayed from captured hipGraphs when
 // use_graph is set); the host reads one flag pair per group of rounds.  A round
 // after convergence reads an empty bitmap and is a no-op.
-int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms)
+int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms, bool full_first = false)
 {
     dsssp_graph *g = t->g;
     hipStream_t s = g->stream;
@@ -352,6 +352,19 @@ int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms)
     const bool timed = tu.kernel_timing && kernel_ms;
     FArgs base = base_args(t);
     if (mode == M_RELAX) base.delta = relax_delta(t);
+    // Push-vs-full-sweep (not with distance ordering, whose rounds visit only
+    // slots below a threshold, nor on the tree-vector path of 64-tree forests).
+    if (mode == M_RELAX && !base.delta && t->kshift < 6 && tu.sweep > 0)
+        base.sweep = (uint32_t)std::max<uint64_t>(1, t->nslots * (uint64_t)tu.sweep / 1000);
+    // Round i's arguments; with full_first, round 0 relaxes every slot.
+    auto args_of = [&](int64_t r) {
+        FArgs a = round_args(t, base, r);
+        if (full_first && r == 0) {
+            a.identity = 1;
+            a.in_act = nullptr;
+        }
+        return a;
+    };
     // Each threshold move follows a round that visited nothing and settles at
     // least the smallest marked slot, so ordered relaxation needs at most about
     // twice the rounds of the plain one.
@@ -365,13 +378,13 @@ int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms)
                 for (size_t j = old; j < g->kev.size(); ++j) HIP_TRY(hipEventCreate(&g->kev[j]));
             }
             for (int j = 0; j < group; ++j)
-                launch_frontier(mode, round_args(t, base, i + j), grid, s, g->kev[2 * j], g->kev[2 * j + 1]);
-        } else if (tu.use_graph) {
+                launch_frontier(mode, args_of(i + j), grid, s, g->kev[2 * j], g->kev[2 * j + 1]);
+        } else if (tu.use_graph && !(full_first && i == 0)) {
             hipGraphExec_t exec = nullptr;
             if (int e = captured_rounds(t, mode, i, group, grid, base, &exec)) return e;
             HIP_TRY(hipGraphLaunch(exec, s));
         } else {
-            for (int j = 0; j < group; ++j) launch_frontier(mode, round_args(t, base, i + j), grid, s);
+            for (int j = 0; j < group; ++j) launch_frontier(mode, args_of(i + j), grid, s);
         }
         i += group;
         rounds += group;
@@ -746,6 +759,7 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
                              8.0 * t->nw * (double)inv_rounds;
         S.alg_bytes_relax = relax;
         S.alg_bytes = relax + other;
+        S.relax_sweeps = (int64_t)c.stats[ST_SWEEPS];
         *stats = S;
     }
     return DSSSP_OK;
@@ -814,11 +828,12 @@ extern "C" int dsssp_set_tuning(const dsssp_tuning *t)
     if (t->tile_entries) nt.tile_entries = t->tile_entries;
     if (t->local_tiles) nt.local_tiles = t->local_tiles;
     if (t->delta) nt.delta = t->delta;
+    if (t->sweep) nt.sweep = t->sweep;
     nt.kernel_timing = t->kernel_timing;
     nt.use_graph = t->use_graph;
     if (nt.grid < 1 || nt.grid > 65536 || nt.heavy_degree < 64 || nt.chunk < 64 || nt.chunk % 64 ||
         nt.round_batch < 1 || nt.round_batch > 64 || nt.tile_entries < 64 || nt.local_tiles < -1 ||
-        nt.local_tiles > 1024 || nt.delta < DSSSP_DELTA_AUTO ||
+        nt.local_tiles > 1024 || nt.delta < DSSSP_DELTA_AUTO || nt.sweep < -1 || nt.sweep > 1000 ||
         (nt.edges_per_thread != 1 && nt.edges_per_thread != 2 && nt.edges_per_thread != 4))
         return fail(DSSSP_ERR_ARG, "tuning out of range");
     g_tuning = nt;
@@ -1331,6 +1346,63 @@ extern "C" int dsssp_tree_copy(dsssp_tree *dst, dsssp_tree *src)
     return DSSSP_OK;
 }
 
+extern "C" int dsssp_tree_relax_full(dsssp_tree *t, dsssp_stats *stats)
+{
+    clear_error();
+    if (!t) return fail(DSSSP_ERR_ARG, "null tree");
+    dsssp_graph *g = t->g;
+    DevGuard dg(g->device);
+    hipStream_t s = g->stream;
+    if (int e = ensure_hq(t, g->E + g->delta_entries)) return e;
+    int64_t rounds = 0, repair = 0;
+    double kernel_ms = 0;
+    // Not a measure of update depth: kept, and the first group is round_batch.
+    const int64_t keep_rounds = t->last_rounds[M_RELAX];
+    t->last_rounds[M_RELAX] = 0;
+    HIP_TRY(hipEventRecord(g->ev[0], s));
+    launch_fill_u32((uint32_t *)t->c, 0u, sizeof(Counters) / 4, s);
+    launch_fill_u64(t->fb[0], 0ull, t->nw, s);
+    launch_fill_u64(t->fb[1], 0ull, t->nw, s);
+    int code = run_rounds(t, M_RELAX, rounds, &kernel_ms, true);
+    t->last_rounds[M_RELAX] = keep_rounds;
+    HIP_TRY(hipEventRecord(g->ev[1], s));
+    if (!code && g->zero_w) code = repick_parents(t, true, repair);
+    HIP_TRY(hipEventRecord(g->ev[2], s));
+    launch_reduce_stats(t->c, t->bstats, s);
+    HIP_TRY(hipMemcpyAsync(t->c_host, t->c, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipGetLastError());
+    if (!code) code = confirm_overflow(t);
+    if (!code) code = finish_errors(t, 0);
+    if (code) {
+        const std::string keep = last_error();
+        scrub(t);
+        return fail(code, "%s", keep.c_str());
+    }
+    if (stats) {
+        const Counters &c = *t->c_host;
+        dsssp_stats S{};
+        S.relax_rounds = rounds;
+        S.relax_launches = rounds;
+        S.frontier_vertices = (int64_t)c.stats[ST_RELAX_FRONT];
+        S.relax_edges = (int64_t)c.stats[ST_RELAX_EDGES];
+        S.relax_improved = (int64_t)c.stats[ST_RELAX_MARKS];
+        S.relax_enqueued = (int64_t)c.stats[ST_RELAX_MARKS];
+        S.other_edges = (int64_t)c.stats[ST_OTHER_EDGES];
+        S.relax_sweeps = (int64_t)c.stats[ST_SWEEPS] + 1;
+        S.t_relax_ms = elapsed(g->ev[0], g->ev[2]);
+        S.t_update_ms = S.t_relax_ms;
+        S.t_relax_kernel_ms = kernel_ms;
+        // The same accounting as an update's relaxation (update_impl).
+        const double relax = 32.0 * (double)c.stats[ST_RELAX_FRONT] + 16.0 * (double)c.stats[ST_RELAX_EDGES] +
+                             16.0 * (double)c.stats[ST_RELAX_MARKS] + 8.0 * t->nw * (double)rounds;
+        S.alg_bytes_relax = relax;
+        S.alg_bytes = relax + 16.0 * (double)c.stats[ST_OTHER_EDGES];
+        *stats = S;
+    }
+    return DSSSP_OK;
+}
+
 extern "C" int dsssp_batch_upload(dsssp_graph *g, const dsssp_change *chg, int64_t k, dsssp_batch **out)
 {
     clear_error();
@@ -1595,6 +1667,7 @@ extern "C" int dsssp_autotune(dsssp_graph *g, dsssp_tree *t, dsssp_batch *const 
         {&dsssp_tuning::local_tiles, {1, 2, 4}},
         {&dsssp_tuning::round_batch, {4, 8}},
         {&dsssp_tuning::delta, {-1, DSSSP_DELTA_AUTO}},
+        {&dsssp_tuning::sweep, {-1, 100, 400}},
     };
     for (int p = 0; p < passes && !s; ++p)
         for (const Knob &k : knobs) {

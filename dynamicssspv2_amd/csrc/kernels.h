@@ -27,8 +27,8 @@ enum Mode : int {
 };
 
 // Arguments of one frontier launch.  The input frontier is a bitmap over
-// state slots (or every slot when identity is set) plus a list of heavy-row
-// chunks; marks go to the output bitmap.
+// state slots (or every slot when identity is set, or in a full-sweep round)
+// plus a list of heavy-row chunks; marks go to the output bitmap.
 struct FArgs {
     GView g;
     uint64_t *st;
@@ -68,6 +68,12 @@ struct FArgs {
     // output, ring + 2 is cleared.
     uint32_t delta;
     uint32_t ring;
+    // RELAX with delta off (push-vs-full-sweep): when the previous round marked
+    // at least `sweep` slots (ring slot `ring` of Counters::nmark), this round
+    // relaxes every slot, its rows read in vertex order, instead of the marked
+    // ones; it still consumes the input bitmap.  0: never.  Rounds count their
+    // marks into ring slot ring + 1 only when sweep is set.
+    uint32_t sweep;
 };
 
 constexpr int FRONTIER_BLOCK = 256;

@@ -140,6 +140,7 @@ struct WaveStats {
     uint32_t thr;
     bool near;
     uint32_t fenc;
+    uint32_t outm;  // wave-uniform: slots marked in the output bitmap (RELAX with sweep set)
 };
 
 // Raises *p to v.  A relaxed load first skips the atomic when v is no record
@@ -276,6 +277,7 @@ __device__ __forceinline__ void visit_next(const FArgs &A, uint32_t tgt, uint32_
                 if (A.delta) note_mark(S, td);
             }
         }
+        S.outm += cnt;
         S.any = true;
     }
 }
@@ -735,6 +737,7 @@ __device__ void drain(const FArgs &A, TileSmem &sm, uint32_t wv, WaveStats &S, u
                     }
                 }
                 S.any = true;
+                S.outm += top;
                 top = 0;
                 break;
             }
@@ -765,10 +768,11 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_
             A.c->fenc[z][lane] = 0;
             if (lane == 0) A.c->near[z] = 0;
         }
+        if (MODE == M_RELAX && A.sweep) A.c->nmark[(A.ring + 2) % 3][lane] = 0;
     }
     // This round's threshold: the previous one, or, when the previous round
     // marked nothing below it, the smallest distance it marked plus delta.
-    WaveStats S{0, 0, 0, false, 0u, false, 0u};
+    WaveStats S{0, 0, 0, false, 0u, false, 0u, 0u};
     if constexpr (MODE == M_RELAX) {
         if (A.delta) {
             uint32_t fe = A.c->fenc[A.ring][lane];
@@ -779,6 +783,20 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_
             if (!A.c->near[A.ring]) thr = fe ? (uint32_t)min((uint64_t)~fe + A.delta, (uint64_t)DIST_INF) : DIST_INF;
             S.thr = __builtin_amdgcn_readfirstlane(thr);
             if (blockIdx.x == 0 && threadIdx.x == 0) A.c->thr[(A.ring + 1) % 3] = S.thr;
+        }
+    }
+    // Push-vs-full-sweep: after a round that marked at least A.sweep slots,
+    // every slot is relaxed (rows in vertex order, tiles of 64 consecutive
+    // slots) instead of the marked ones.  Every wave sums the same 64 words,
+    // written by the previous launch, so the choice is the same grid-wide.
+    bool sweep = false;
+    if constexpr (MODE == M_RELAX && !FOREST) {
+        if (A.sweep) {
+            unsigned long long tot = A.c->nmark[A.ring][lane];
+#pragma unroll
+            for (int d = 32; d > 0; d >>= 1) tot += __shfl_xor(tot, d);
+            sweep = __builtin_amdgcn_readfirstlane((uint32_t)(tot >= A.sweep)) != 0;
+            if (sweep && blockIdx.x == 0 && threadIdx.x == 0) add_stat(A.bstats, ST_SWEEPS, 1);
         }
     }
     const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -794,9 +812,10 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_
         wi = blk * 8 + (lane & 7);
         uint64_t word = 0;
         if (blk < nblk && wi < A.nwords) {
-            if (A.identity) {
+            if (A.identity || sweep) {
                 const uint32_t lo = wi * 64;
                 word = A.nslots - lo >= 64 ? ~0ull : ((1ull << (A.nslots - lo)) - 1ull);
+                if (sweep && A.in_bm[wi]) A.in_bm[wi] = 0;  // consumed all the same
             } else {
                 word = A.in_bm[wi];
                 if (word && A.consume) A.in_bm[wi] = 0;
@@ -883,6 +902,9 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_
     }
     if (lane == 0) {
         if (any && A.out_act) *A.out_act = 1u;
+        if constexpr (MODE == M_RELAX && !FOREST) {
+            if (A.sweep && S.outm) atomicAdd(&A.c->nmark[(A.ring + 1) % 3][gw & 63u], S.outm);
+        }
         if constexpr (MODE == M_RELAX) {
             add_stat(A.bstats, ST_RELAX_FRONT, S.front);
             add_stat(A.bstats, ST_RELAX_EDGES, S.edges);

@@ -93,6 +93,7 @@ typedef struct {
     double t_relax_kernel_ms;       /* sum of relaxation kernel durations      */
     double alg_bytes;               /* algorithmic HBM bytes, whole update     */
     double alg_bytes_relax;         /* algorithmic HBM bytes, relaxation only  */
+    int64_t relax_sweeps;           /* relaxation rounds run as full sweeps    */
 } dsssp_stats;
 
 /* Tuning knobs (zero fields keep the current value). */
@@ -120,6 +121,11 @@ typedef struct {
                                  DSSSP_DELTA_AUTO (default): twice the graph's mean
                                  weight, for trees whose distances span more than 64
                                  such steps (high-diameter graphs), else off         */
+    int32_t sweep;            /* push-vs-full-sweep threshold, per mille of the slots:
+                                 a relaxation round after one that marked at least
+                                 this share relaxes every slot, rows read in vertex
+                                 order, instead of the marked ones.  -1: never.
+                                 Only without distance ordering and below 64 trees */
 } dsssp_tuning;
 
 #define DSSSP_DELTA_AUTO (-2)
@@ -213,6 +219,15 @@ int dsssp_tree_create(dsssp_graph *g, int32_t src, const int32_t *parent, const 
 int dsssp_tree_destroy(dsssp_tree *t);
 int dsssp_tree_read(dsssp_tree *t, int32_t *parent, int64_t *dist);
 int dsssp_tree_copy(dsssp_tree *dst, dsssp_tree *src);
+/* One full relaxation sweep: every live entry of every reachable slot is
+ * relaxed once, rows in vertex order, then relaxation rounds run until nothing
+ * changes (the relaxation of updateNeighbors, gpuFunctions_undir.cuh:370-433,
+ * started from every vertex instead of the affected ones; the reference has
+ * no such entry point).  Distances that are lengths of real paths converge to
+ * the shortest ones; on a converged tree nothing changes and
+ * stats->relax_improved is 0, a device-side check that no live edge shortens
+ * any distance.  stats: the relaxation fields and times only. */
+int dsssp_tree_relax_full(dsssp_tree *t, dsssp_stats *stats);
 /* One shortest-path forest grown from several roots at once (each root has
  * distance 0 and parent itself), computed on the GPU.  Used for many graphs
  * held in one device graph, one root per component.  No reference

@@ -23,6 +23,26 @@ def test_library_exports_every_header_symbol():
     assert ds.lib().dsssp_version().startswith(b"dsssp")
 
 
+def test_tuning_knobs_validated():
+    """Host-side checks of dsssp_set_tuning (no device call): the
+    push-vs-full-sweep threshold is -1 (never) or 1..1000 per mille, and a
+    rejected setting leaves the tuning as it was."""
+    old = ds.get_tuning()
+    assert old["sweep"] == -1  # default: push rounds only
+    try:
+        ds.set_tuning(sweep=250)
+        assert ds.get_tuning()["sweep"] == 250
+        for bad in (-2, 1001):
+            with pytest.raises(ds.DssspError) as e:
+                ds.set_tuning(sweep=bad)
+            assert e.value.code == ds.ERR_ARG
+            assert ds.get_tuning()["sweep"] == 250
+        ds.set_tuning(sweep=-1)
+        assert ds.get_tuning() == old
+    finally:
+        ds.set_tuning(**old)
+
+
 @pytest.mark.parametrize("name", ["toy", "r300", "r2000", "sparse1500"])
 def test_readers_match_plain_python(name):
     n = GOLDEN_N[name]
