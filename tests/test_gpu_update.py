@@ -318,6 +318,37 @@ def test_heavy_rows_and_chunks(ept):
         ds.set_tuning(**old)
 
 
+@pytest.mark.parametrize("ept", [1, 4])
+def test_budget_deferred_rows(ept):
+    """Smallest tile budget and chunk (64 entries) under a heavy threshold no
+    row reaches: most rows leave their tiles for the budget and are processed
+    as items by the next launch.  Every frontier mode goes through that path:
+    the edge-map build, the tree from scratch, invalidation, seeding,
+    relaxation, and (zero weights) the parent repair."""
+    old = ds.get_tuning()
+    ds.set_tuning(heavy_degree=1 << 20, tile_entries=64, chunk=64, grid=64, edges_per_thread=ept)
+    try:
+        a, b, w = ds.gen_rmat(13, 16, wlo=1, whi=1000, seed=91)
+        n = 1 << 13
+        rng = np.random.default_rng(92)
+        batches, ca, cb, cw = [], a, b, w
+        for ins in (0.5, 0.0, 1.0):
+            chg = ds.gen_changes(n, ca, cb, cw, 3000, ins, 1, 1000, seed=int(rng.integers(1 << 30)))
+            batches.append(chg)
+            ca, cb, cw = oracle.apply_changes(n, ca, cb, cw, chg)
+        deg = np.bincount(np.concatenate([a, b]), minlength=n)
+        run_sequence(n, a, b, w, batches, src=int(np.argmax(deg)), what=f"rmat13-budget-ept{ept}")
+        za, zb, zw = random_multigraph(rng, 3000, 24000, 0, 4)
+        zbatches, ca, cb, cw = [], za, zb, zw
+        for _ in range(2):
+            chg = random_batch(rng, 3000, ca, cb, cw, 800, 0.5, 0, 4)
+            zbatches.append(chg)
+            ca, cb, cw = oracle.apply_changes(3000, ca, cb, cw, chg)
+        run_sequence(3000, za, zb, zw, zbatches, src=1, what=f"zero-weights-budget-ept{ept}")
+    finally:
+        ds.set_tuning(**old)
+
+
 def test_config1_full_size():
     """n=100,000, m=1,000,000, w U[1,100], src 0, batch 10,000 (50/50), deletions
     from the non-spanning-tree edges; the oracle is Dijkstra on the updated graph."""
