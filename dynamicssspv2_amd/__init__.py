@@ -49,7 +49,7 @@ class Stats(ctypes.Structure):
         [(f, ctypes.c_double) for f in (
             "t_h2d_ms", "t_update_ms", "t_insert_ms", "t_classify_ms", "t_invalidate_ms", "t_seed_ms",
             "t_relax_ms", "t_relax_kernel_ms", "alg_bytes", "alg_bytes_relax")] + \
-        [("relax_sweeps", ctypes.c_int64)]
+        [("relax_sweeps", ctypes.c_int64), ("t_sweep_ms", ctypes.c_double), ("alg_bytes_sweep", ctypes.c_double)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}

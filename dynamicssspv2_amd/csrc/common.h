@@ -53,7 +53,9 @@ struct VRec {
     uint32_t blen;  // base entries
     uint32_t dcap;  // delta block capacity
     uint64_t dbl;   // delta block: pool offset << 32 | length
-    uint64_t pad;
+    uint32_t blo;   // base entries whose neighbour id is below the vertex's own
+                    // (a prefix of the sorted row; positions never move)
+    uint32_t pad;
 };
 static_assert(sizeof(VRec) == 32, "VRec is 32 bytes");
 
@@ -78,6 +80,7 @@ enum Stat : int {
     ST_INVALIDATED,      // slots seeded (= invalidated by the update)
     ST_PROBES,           // entries read by deletion searches
     ST_SWEEPS,           // relaxation rounds run as full sweeps (every slot visited)
+    ST_SWEEP_GATHERS,    // entries of an M_SWEEP pass that read their neighbour's state
     NSTAT
 };
 constexpr int STAT_SLOTS = 2048;

@@ -103,6 +103,11 @@ inline uint32_t words_of(uint64_t bits) { return (uint32_t)((bits + 63) / 64); }
 
 constexpr uint32_t INIT_DCAP = 4;  // initial delta block capacity of every row
 constexpr int64_t LOCAL_MAX = 32;  // largest continuation budget of a round (tiles)
+// The full edge pass: rows longer than SWEEP_PIECE entries go to the item
+// launch in pieces of that size; both launches have (at least) as many
+// workgroups as the device holds at once at 6 waves per SIMD (6 per CU).
+constexpr uint32_t SWEEP_PIECE = 512;
+constexpr int SWEEP_WG_PER_CU = 6;
 // Automatic ordered relaxation: trees whose distances span more than DEEP_STEPS
 // steps, or whose ordered construction took more than DEEP_ROUNDS rounds.
 constexpr uint32_t DEEP_STEPS = 64;
@@ -278,10 +283,10 @@ void zero_ring(dsssp_tree *t)
 // One pass of a single-pass mode (seed, parent pick) into bitmap 0 / flag 0.
 // Rows longer than the heavy threshold are queued as chunks on list 1 (idle
 // here) by the first launch and processed by a second, chunk-only launch.
-void single_pass(dsssp_tree *t, int mode, FArgs a)
+void single_pass(dsssp_tree *t, int mode, FArgs a, int min_grid = 0)
 {
     hipStream_t s = t->g->stream;
-    const int grid = frontier_grid(t->nw);
+    const int grid = std::max(frontier_grid(t->nw), min_grid);
     launch_fill_u32(&t->c->aux_hcnt, 0u, 1, s);
     a.ring = 2;  // output ring slot 0, read by round 0 of the loop that follows
     a.out_bm = t->fb[0];
@@ -948,8 +953,13 @@ extern "C" int dsssp_graph_create(int64_t n, int64_t m, const int32_t *a, const 
     DevGuard dg(g->device);
     std::vector<VRec> vr((size_t)n);
     parallel_ranges(n, [&](int64_t lo, int64_t hi, int) {
-        for (int64_t v = lo; v < hi; ++v)
-            vr[v] = VRec{rp[v], (uint32_t)(rp[v + 1] - rp[v]), INIT_DCAP, ((uint64_t)v * INIT_DCAP) << 32, 0ull};
+        for (int64_t v = lo; v < hi; ++v) {
+            const uint64_t *row = adj.data() + rp[v], *end = adj.data() + rp[v + 1];
+            const uint32_t blo = (uint32_t)(std::partition_point(row, end, [v](uint64_t e) {
+                                                return e_col(e) < (uint32_t)v;
+                                            }) - row);
+            vr[v] = VRec{rp[v], (uint32_t)(rp[v + 1] - rp[v]), INIT_DCAP, ((uint64_t)v * INIT_DCAP) << 32, blo, 0u};
+        }
     });
     if (hipMemcpy(g->vr, vr.data(), sizeof(VRec) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess ||
         (g->E && hipMemcpy(g->adj, adj.data(), sizeof(uint64_t) * g->E, hipMemcpyHostToDevice) != hipSuccess)) {
@@ -1363,7 +1373,37 @@ extern "C" int dsssp_tree_relax_full(dsssp_tree *t, dsssp_stats *stats)
     launch_fill_u32((uint32_t *)t->c, 0u, sizeof(Counters) / 4, s);
     launch_fill_u64(t->fb[0], 0ull, t->nw, s);
     launch_fill_u64(t->fb[1], 0ull, t->nw, s);
-    int code = run_rounds(t, M_RELAX, rounds, &kernel_ms, true);
+    // The pass's own statistics (front, entries, marks, gathers) and time.
+    uint64_t sw[4] = {0, 0, 0, 0};
+    double sweep_ms = 0;
+    int code = DSSSP_OK;
+    if (t->kshift < 6) {
+        // Each live edge once, both directions, from its higher-id endpoint's
+        // row; the slots it lowers are marked in bitmap 0 for round 0.
+        FArgs a = base_args(t);
+        a.identity = 1;
+        a.consume = 0;
+        a.local = 0;
+        a.delta = 0;
+        a.heavy = a.chunk = SWEEP_PIECE;
+        int cus = 0;
+        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+        HIP_TRY(hipEventRecord(g->ev[3], s));
+        single_pass(t, M_SWEEP, a, cus * SWEEP_WG_PER_CU);
+        HIP_TRY(hipEventRecord(g->ev[4], s));
+        launch_reduce_stats(t->c, t->bstats, s);
+        HIP_TRY(hipMemcpyAsync(t->c_host, t->c, sizeof(Counters), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        sweep_ms = elapsed(g->ev[3], g->ev[4]);
+        const Counters &c = *t->c_host;
+        sw[0] = c.stats[ST_RELAX_FRONT];
+        sw[1] = c.stats[ST_RELAX_EDGES];
+        sw[2] = c.stats[ST_RELAX_MARKS];
+        sw[3] = c.stats[ST_SWEEP_GATHERS];
+        code = run_rounds(t, M_RELAX, rounds, &kernel_ms);
+    } else {
+        code = run_rounds(t, M_RELAX, rounds, &kernel_ms, true);  // round 0 pushes from every slot
+    }
     t->last_rounds[M_RELAX] = keep_rounds;
     HIP_TRY(hipEventRecord(g->ev[1], s));
     if (!code && g->zero_w) code = repick_parents(t, true, repair);
@@ -1392,12 +1432,18 @@ extern "C" int dsssp_tree_relax_full(dsssp_tree *t, dsssp_stats *stats)
         S.relax_sweeps = (int64_t)c.stats[ST_SWEEPS] + 1;
         S.t_relax_ms = elapsed(g->ev[0], g->ev[2]);
         S.t_update_ms = S.t_relax_ms;
-        S.t_relax_kernel_ms = kernel_ms;
-        // The same accounting as an update's relaxation (update_impl).
-        const double relax = 32.0 * (double)c.stats[ST_RELAX_FRONT] + 16.0 * (double)c.stats[ST_RELAX_EDGES] +
-                             16.0 * (double)c.stats[ST_RELAX_MARKS] + 8.0 * t->nw * (double)rounds;
-        S.alg_bytes_relax = relax;
-        S.alg_bytes = relax + 16.0 * (double)c.stats[ST_OTHER_EDGES];
+        // The pass: per slot 32 B (row fields + own state), per scanned entry
+        // 8 B, per entry that read its neighbour's state 8 B more, per
+        // improvement 16 B (state atomicMin + bitmap word).
+        S.t_sweep_ms = sweep_ms;
+        S.alg_bytes_sweep = 32.0 * (double)sw[0] + 8.0 * (double)sw[1] + 8.0 * (double)sw[3] + 16.0 * (double)sw[2];
+        S.t_relax_kernel_ms = kernel_ms + (tuning().kernel_timing ? sweep_ms : 0.0);
+        // The rounds: the same accounting as an update's relaxation (update_impl).
+        const double relax = 32.0 * (double)(c.stats[ST_RELAX_FRONT] - sw[0]) +
+                             16.0 * (double)(c.stats[ST_RELAX_EDGES] - sw[1]) +
+                             16.0 * (double)(c.stats[ST_RELAX_MARKS] - sw[2]) + 8.0 * t->nw * (double)rounds;
+        S.alg_bytes_relax = relax + S.alg_bytes_sweep;
+        S.alg_bytes = S.alg_bytes_relax + 16.0 * (double)c.stats[ST_OTHER_EDGES];
         *stats = S;
     }
     return DSSSP_OK;

@@ -24,6 +24,11 @@ enum Mode : int {
     M_PICK = 4,   // smallest-id tight neighbour (optionally one level up) as parent
     M_OFCHECK = 5,  // a reachable slot with a live edge to an unreachable one: overflow
     M_EMAP = 6,     // enter every live entry into the edge map (identity pass)
+    // One pass over every live edge, both directions at once, from the row of
+    // its higher-id endpoint only (base rows are sorted by neighbour, so that
+    // part is a prefix): push (d[v]+w, v) to the neighbour, pull (d[u]+w, u)
+    // into the owner.  Identity frontier; not with 64 trees per vertex.
+    M_SWEEP = 7,
 };
 
 // Arguments of one frontier launch.  The input frontier is a bitmap over

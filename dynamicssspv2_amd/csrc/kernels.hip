@@ -141,6 +141,7 @@ struct WaveStats {
     bool near;
     uint32_t fenc;
     uint32_t outm;  // wave-uniform: slots marked in the output bitmap (RELAX with sweep set)
+    uint32_t gath;  // per lane: SWEEP entries that read their neighbour's state
 };
 
 // Raises *p to v.  A relaxed load first skips the atomic when v is no record
@@ -175,6 +176,9 @@ __device__ __forceinline__ Pre gather(const FArgs &A, uint32_t os, uint64_t ent)
     const uint32_t u = os >> A.kshift;
     const uint32_t x = e_col(ent);
     if (x == u) return p;  // a self-loop never carries a shortest path
+    if constexpr (MODE == M_SWEEP) {
+        if (x > u) return p;  // relaxed from the row of x, the higher id
+    }
     if constexpr (MODE != M_INVAL) {
         if (e_w(ent) & W_DEAD) return p;
     }
@@ -216,6 +220,30 @@ __device__ __forceinline__ uint32_t apply(const FArgs &A, uint32_t os, uint32_t 
         atomicMin((unsigned long long *)&A.st[xs], (unsigned long long)cand);
         S.marks++;
         td = (uint32_t)nd;
+        return xs;
+    } else if constexpr (MODE == M_SWEEP) {
+        // Edge {u, x}, x < u, in both directions.  Pull: the owner's candidate
+        // (reduced per owner by the caller).  aux0 is the owner's distance
+        // (DIST_INF when unreachable).
+        S.gath++;
+        if (p.sx != ST_INF) {
+            const uint64_t nd = (uint64_t)st_dist(p.sx) + w;
+            if (nd < DIST_INF)
+                pull = min(pull, pack_state((uint32_t)nd, x));
+            else if (aux0 == DIST_INF)
+                atomicOr(&A.c->err, ERR_DIST_SUSPECT);
+        }
+        // Push, as RELAX: the neighbour's candidate through the owner.
+        if (aux0 == DIST_INF) return NO_SLOT;
+        const uint64_t nd = (uint64_t)aux0 + w;
+        if (nd >= DIST_INF) {
+            if (p.sx == ST_INF) atomicOr(&A.c->err, ERR_DIST_SUSPECT);
+            return NO_SLOT;
+        }
+        const uint64_t cand = pack_state((uint32_t)nd, u);
+        if (cand >= p.sx || st_parent(p.sx) == x) return NO_SLOT;
+        atomicMin((unsigned long long *)&A.st[xs], (unsigned long long)cand);
+        S.marks++;
         return xs;
     } else if constexpr (MODE == M_INVAL) {
         // x is a child of u in the pre-update tree: reset it and descend.
@@ -267,7 +295,7 @@ __device__ __forceinline__ void visit_next(const FArgs &A, uint32_t tgt, uint32_
     const unsigned long long m = __ballot(tgt != NO_SLOT);
     if (!m) return;
     const uint32_t cnt = (uint32_t)__popcll(m);
-    if (A.local && top + cnt <= LSTK) {
+    if (MODE != M_SWEEP && A.local && top + cnt <= LSTK) {  // (a sweep's marks all go to the bitmap)
         if (tgt != NO_SLOT) stk[top + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull))] = tgt;
         top += cnt;
     } else {
@@ -294,7 +322,7 @@ struct TileSmem {
     uint32_t stk[FW][LSTK];           // push modes: slots to visit in this launch
 };
 
-constexpr bool is_pull(int mode) { return mode == M_SEED || mode == M_PICK; }
+constexpr bool is_pull(int mode) { return mode == M_SEED || mode == M_PICK || mode == M_SWEEP; }
 
 // Owner side of a pull mode: the reduced candidate `best` of slot os.  Returns
 // true when it lowered the slot (SEED), so that the slot is marked.
@@ -304,6 +332,12 @@ __device__ __forceinline__ bool pull_commit(const FArgs &A, uint32_t os, uint64_
     if (best == ~0ull) return false;
     if constexpr (MODE == M_SEED) {
         if (best >= A.st[os]) return false;
+        atomicMin((unsigned long long *)&A.st[os], (unsigned long long)best);
+        return true;
+    } else if constexpr (MODE == M_SWEEP) {
+        const uint64_t cur = A.st[os];
+        // a root (parent itself) never changes
+        if (best >= cur || st_parent(cur) == (os >> A.kshift)) return false;
         atomicMin((unsigned long long *)&A.st[os], (unsigned long long)best);
         return true;
     } else {
@@ -327,6 +361,9 @@ __device__ __forceinline__ bool owner_prep(const FArgs &A, uint32_t os, uint32_t
         return su != ST_INF && aux1 != LVL_INF;
     } else if constexpr (MODE == M_OFCHECK) {
         return A.st[os] != ST_INF;
+    } else if constexpr (MODE == M_SWEEP) {
+        aux0 = st_dist(A.st[os]);  // an unreachable owner still pulls
+        return true;
     } else if constexpr (MODE == M_PICK) {
         const uint64_t su = A.st[os];
         aux0 = st_dist(su);
@@ -390,6 +427,7 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
         blen = r->blen;
         const uint64_t dd = r->dbl;
         dbeg = (uint32_t)(dd >> 32);
+        if constexpr (MODE == M_SWEEP) blen = r->blo;  // base entries below the owner's id
         deg = blen + (uint32_t)dd;
         if (!owner_prep<MODE>(A, os, aux0, aux1)) deg = 0;
     }
@@ -516,7 +554,7 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
             if constexpr (is_pull(MODE)) {
                 if (pull != ~0ull) atomicMin(&sm.pull[wv][oi[j]], (unsigned long long)pull);
             }
-            if constexpr (MODE == M_RELAX || MODE == M_INVAL || MODE == M_TBFS)
+            if constexpr (MODE == M_RELAX || MODE == M_INVAL || MODE == M_TBFS || MODE == M_SWEEP)
                 visit_next<MODE>(A, tgt, td, sm.stk[wv], top, S);
         }
     }
@@ -532,6 +570,7 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
         if (mine && pull_commit<MODE>(A, os, best)) {
             mark(A.out_bm, os);
             S.any = true;
+            if constexpr (MODE == M_SWEEP) S.marks++;
             if (A.delta) note_mark(S, st_dist(best));
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -772,7 +811,7 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_
     }
     // This round's threshold: the previous one, or, when the previous round
     // marked nothing below it, the smallest distance it marked plus delta.
-    WaveStats S{0, 0, 0, false, 0u, false, 0u, 0u};
+    WaveStats S{0, 0, 0, false, 0u, false, 0u, 0u, 0u};
     if constexpr (MODE == M_RELAX) {
         if (A.delta) {
             uint32_t fe = A.c->fenc[A.ring][lane];
@@ -886,6 +925,7 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_
     }
     drain<MODE, EPT>(A, sm, wv, S, top);
     const uint32_t marks = wave_sum(S.marks);
+    const uint32_t gath = MODE == M_SWEEP ? wave_sum(S.gath) : 0u;
     const bool any = __ballot(S.any) != 0;
     if constexpr (MODE == M_RELAX || MODE == M_SEED) {
         if (A.delta) {
@@ -909,6 +949,11 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_
             add_stat(A.bstats, ST_RELAX_FRONT, S.front);
             add_stat(A.bstats, ST_RELAX_EDGES, S.edges);
             add_stat(A.bstats, ST_RELAX_MARKS, marks);
+        } else if constexpr (MODE == M_SWEEP) {
+            add_stat(A.bstats, ST_RELAX_FRONT, S.front);
+            add_stat(A.bstats, ST_RELAX_EDGES, S.edges);
+            add_stat(A.bstats, ST_RELAX_MARKS, marks);
+            add_stat(A.bstats, ST_SWEEP_GATHERS, gath);
         } else if constexpr (MODE == M_EMAP) {
             // no statistics: the map is built outside updates
         } else {
@@ -1335,6 +1380,7 @@ void launch_frontier_ept(int mode, const FArgs &a, int grid, hipStream_t s, hipE
     case M_TBFS: launch_mode<M_TBFS, EPT>(a, grid, s, start, stop); break;
     case M_PICK: launch_mode<M_PICK, EPT>(a, grid, s, start, stop); break;
     case M_OFCHECK: launch_mode<M_OFCHECK, EPT>(a, grid, s, start, stop); break;
+    case M_SWEEP: launch_kernel<M_SWEEP, EPT, false>(a, grid, s, start, stop); break;  // never 64 trees
     default: launch_mode<M_EMAP, EPT>(a, grid, s, start, stop); break;
     }
 }

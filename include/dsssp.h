@@ -94,6 +94,8 @@ typedef struct {
     double alg_bytes;               /* algorithmic HBM bytes, whole update     */
     double alg_bytes_relax;         /* algorithmic HBM bytes, relaxation only  */
     int64_t relax_sweeps;           /* relaxation rounds run as full sweeps    */
+    double t_sweep_ms;              /* dsssp_tree_relax_full: its edge pass    */
+    double alg_bytes_sweep;         /* and that pass's algorithmic HBM bytes   */
 } dsssp_stats;
 
 /* Tuning knobs (zero fields keep the current value). */
@@ -219,14 +221,17 @@ int dsssp_tree_create(dsssp_graph *g, int32_t src, const int32_t *parent, const 
 int dsssp_tree_destroy(dsssp_tree *t);
 int dsssp_tree_read(dsssp_tree *t, int32_t *parent, int64_t *dist);
 int dsssp_tree_copy(dsssp_tree *dst, dsssp_tree *src);
-/* One full relaxation sweep: every live entry of every reachable slot is
- * relaxed once, rows in vertex order, then relaxation rounds run until nothing
- * changes (the relaxation of updateNeighbors, gpuFunctions_undir.cuh:370-433,
- * started from every vertex instead of the affected ones; the reference has
- * no such entry point).  Distances that are lengths of real paths converge to
- * the shortest ones; on a converged tree nothing changes and
- * stats->relax_improved is 0, a device-side check that no live edge shortens
- * any distance.  stats: the relaxation fields and times only. */
+/* One full relaxation pass, then relaxation rounds until nothing changes (the
+ * relaxation of updateNeighbors, gpuFunctions_undir.cuh:370-433, started from
+ * every vertex instead of the affected ones; the reference has no such entry
+ * point).  The pass relaxes every live edge in both directions once, from the
+ * row of its higher-id endpoint (for 64 trees per vertex: every entry of every
+ * reachable slot, pushed).  Distances that are lengths of real paths converge
+ * to the shortest ones; on a converged tree nothing changes and, with
+ * positive weights, stats->relax_improved is 0: a device-side check that no
+ * live edge shortens any distance (with zero weights, equal-distance parents
+ * may move to smaller ids, and the parent repair restores the canonical ones).  stats: the relaxation fields and times, and the pass's own
+ * device time and algorithmic bytes (t_sweep_ms, alg_bytes_sweep). */
 int dsssp_tree_relax_full(dsssp_tree *t, dsssp_stats *stats);
 /* One shortest-path forest grown from several roots at once (each root has
  * distance 0 and parent itself), computed on the GPU.  Used for many graphs

@@ -16,7 +16,7 @@ import pytest
 
 import dynamicssspv2_amd as ds
 import oracle
-from tests._util import random_batch, random_connected
+from tests._util import random_batch, random_connected, random_multigraph
 from tests.test_gpu_update import Mirror, check_tree, run_sequence
 
 pytestmark = pytest.mark.gpu
@@ -88,6 +88,42 @@ def test_sweep_equals_push_on_rmat_with_split_rows():
         assert np.array_equal(d0, d1) and np.array_equal(p0, p1), i
 
 
+def test_sweep_rounds_from_captured_graphs():
+    """Round groups with the sweep threshold set, replayed from captured
+    hipGraphs (the sweep choice is made on the device from the previous
+    round's mark counts), equal the eager push-only result on every update of
+    a sequence."""
+    a, b, w = ds.gen_rmat(12, 16, seed=41)
+    n = 1 << 12
+    rng = np.random.default_rng(41)
+    batches, ca, cb, cw = [], a, b, w
+    for i in range(5):
+        c = ds.gen_changes(n, ca, cb, cw, 1500, [0.0, 0.5, 1.0][i % 3], 1, 1000, seed=int(rng.integers(1 << 30)))
+        batches.append(c)
+        ca, cb, cw = oracle.apply_changes(n, ca, cb, cw, c)
+    results, sweeps = {}, 0
+    for label, kw in (("eager push", dict(sweep=-1, use_graph=0)),
+                      ("graph sweep", dict(sweep=1, use_graph=1, round_batch=2, local_tiles=-1))):
+        say(f"sweep rounds {label}")
+        with tuned(delta=-1, **kw):
+            g = ds.Graph(n, a, b, w)
+            t = ds.Tree(g, 0)
+            seq = [t.read()]
+            for c in batches:
+                st = ds.update(g, t, c)
+                if kw["sweep"] > 0:
+                    sweeps += st["relax_sweeps"]
+                seq.append(t.read())
+            results[label] = seq
+    assert sweeps > 0
+    for (p0, d0), (p1, d1) in zip(results["eager push"], results["graph sweep"]):
+        assert np.array_equal(p0, p1) and np.array_equal(d0, d1)
+    m = Mirror(n, a, b, w)
+    for c in batches:
+        m.apply(c)
+    check_tree(m, *results["graph sweep"][-1], [0], "graph sweep")
+
+
 @pytest.mark.parametrize("K", [3, 8, 64])
 def test_sweep_forest(K):
     """Forests: K' = 4, 8 sweep over slots vertex*K'+tree; K = 64 (tree-vector
@@ -128,6 +164,7 @@ def test_relax_full_on_converged_tree_changes_nothing():
     assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
     assert st["relax_improved"] == 0, st
     assert st["frontier_vertices"] == n and st["relax_sweeps"] >= 1, st
+    assert st["t_sweep_ms"] > 0 and st["alg_bytes_sweep"] > 0, st
     # the rows of the reachable vertices were scanned
     m = Mirror(n, a, b, w)
     m.apply(chg)
@@ -137,6 +174,73 @@ def test_relax_full_on_converged_tree_changes_nothing():
     ds.update(g, t, chg2)
     m.apply(chg2)
     check_tree(m, *t.read(), [5], "after relax_full")
+
+
+def test_relax_full_multigraph_and_forests():
+    """The edge pass on a multigraph (zero weights, parallel edges, self-loops,
+    several components) and on forests of 3 trees (slots vertex*4+tree) and 64
+    trees (the push path): nothing changes on converged trees, which match the
+    oracle.  With zero weights the packed minimum also moves equal-distance
+    parents to smaller ids, which the parent repair undoes, so there the tree,
+    not the improvement count, is what stays the same."""
+    rng = np.random.default_rng(1700)
+    n = 2000
+    a, b, w = random_multigraph(rng, n, 12000, 0, 5)
+    say("relax_full multigraph")
+    g = ds.Graph(n, a, b, w)
+    t = ds.Tree(g, 4)
+    chg = random_batch(rng, n, a, b, w, 600, 0.5, 0, 5)
+    ds.update(g, t, chg)
+    before = t.read()
+    t.relax_full()
+    after = t.read()
+    assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+    m = Mirror(n, a, b, w)
+    m.apply(chg)
+    check_tree(m, *after, [4], "relax_full multigraph")
+    ra, rb, rw = ds.gen_rmat(11, 16, wlo=1, whi=1000, seed=1701)
+    rn = 1 << 11
+    for K in (3, 64):
+        say(f"relax_full forest K={K}")
+        srcs = rng.integers(0, rn, K).astype(np.int32)
+        gf = ds.Graph(rn, ra, rb, rw)
+        f = ds.Tree(gf, sources=srcs)
+        st = f.relax_full()
+        assert st["relax_improved"] == 0, (K, st)
+        csr = oracle.CSR.from_edges(rn, ra, rb, rw)
+        for j in range(K):
+            par, dist = f.read(j)
+            _, dref = oracle.dijkstra(csr, [int(srcs[j])])
+            assert np.array_equal(dist, dref), (K, j)
+
+
+@pytest.mark.parametrize("zero", [False, True])
+def test_relax_full_converges_upper_bounds_zero_weights(zero):
+    """Upper bounds from a spanning subgraph on a multigraph-like graph (zero
+    weights allowed): the pass plus rounds (and the parent repair) give the
+    oracle's distances and canonical parents."""
+    rng = np.random.default_rng(1800 + zero)
+    n = 1500
+    a, b, w = random_connected(rng, n, 7000, 0 if zero else 1, 50)
+    src = 3
+    sub = np.arange(len(a)) < n - 1
+    csr_sub = oracle.CSR.from_edges(n, a[sub], b[sub], w[sub])
+    _, dsub = oracle.dijkstra(csr_sub, [src])
+    psub = oracle.canonical_parents(csr_sub, dsub, [src], rule=1 if zero else 0)
+    say(f"relax_full upper bounds zero={zero}")
+    g = ds.Graph(n, a, b, w)
+    t = ds.Tree(g, src, parent=psub, dist=dsub)
+    st = t.relax_full()
+    assert st["relax_improved"] > 0
+    par, dist = t.read()
+    m = Mirror(n, a, b, w)
+    if zero:
+        check_tree(m, par, dist, [src], "relax_full upper bounds zero")  # repaired: canonical parents
+    else:
+        csr = m.csr()
+        _, dref = oracle.dijkstra(csr, [src])
+        assert np.array_equal(dist, dref)
+        assert oracle.certificate(csr, dist, par, [src])[0] == 0
 
 
 @pytest.mark.parametrize("sweep", [-1, 1])
