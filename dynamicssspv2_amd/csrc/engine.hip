@@ -108,6 +108,11 @@ constexpr int64_t LOCAL_MAX = 32;  // largest continuation budget of a round (ti
 // workgroups as the device holds at once at 6 waves per SIMD (6 per CU).
 constexpr uint32_t SWEEP_PIECE = 512;
 constexpr int SWEEP_WG_PER_CU = 6;
+// A tree built from scratch starts with full edge passes while a pass still
+// lowers at least 1/SCRATCH_PASS_SHARE of the slots (low-diameter graphs),
+// at most SCRATCH_PASSES, then relaxation rounds finish it.
+constexpr int SCRATCH_PASSES = 4;
+constexpr uint64_t SCRATCH_PASS_SHARE = 8;
 // Automatic ordered relaxation: trees whose distances span more than DEEP_STEPS
 // steps, or whose ordered construction took more than DEEP_ROUNDS rounds.
 constexpr uint32_t DEEP_STEPS = 64;
@@ -303,6 +308,47 @@ void single_pass(dsssp_tree *t, int mode, FArgs a, int min_grid = 0)
     b.out_hq = nullptr;
     b.out_hcnt = nullptr;
     launch_frontier(mode, b, grid, s);
+}
+
+// Arguments of the full edge pass (M_SWEEP) and its least grid.
+FArgs sweep_args(dsssp_tree *t, int &grid)
+{
+    FArgs a = base_args(t);
+    a.identity = 1;
+    a.consume = 0;
+    a.local = 0;
+    a.delta = 0;
+    a.heavy = a.chunk = SWEEP_PIECE;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, t->g->device) != hipSuccess) cus = 0;
+    grid = cus * SWEEP_WG_PER_CU;
+    return a;
+}
+
+// Full edge passes at the start of a tree built from scratch (the roots are
+// marked in bitmap 0; the passes add the slots they lower), while a pass
+// lowers at least 1/SCRATCH_PASS_SHARE of the slots.  Relaxation rounds
+// started from bitmap 0 then reach the same fixed point as without them.
+int scratch_passes(dsssp_tree *t, int &passes)
+{
+    passes = 0;
+    if (t->kshift >= 6) return DSSSP_OK;
+    hipStream_t s = t->g->stream;
+    int grid = 0;
+    const FArgs a = sweep_args(t, grid);
+    unsigned long long prev = 0;
+    for (int k = 0; k < SCRATCH_PASSES; ++k) {
+        single_pass(t, M_SWEEP, a, grid);
+        ++passes;
+        launch_reduce_stats(t->c, t->bstats, s);
+        HIP_TRY(hipMemcpyAsync(&t->c_host->stats[ST_RELAX_MARKS], &t->c->stats[ST_RELAX_MARKS],
+                               sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        const unsigned long long total = t->c_host->stats[ST_RELAX_MARKS];
+        if ((total - prev) * SCRATCH_PASS_SHARE < t->nslots) break;
+        prev = total;
+    }
+    return DSSSP_OK;
 }
 
 // Instantiated hipGraph of rounds first .. first+group-1 of `mode`.  Round
@@ -1233,7 +1279,9 @@ static int tree_make(dsssp_graph *g, int32_t ntrees, const std::vector<uint32_t>
             launch_set_sources(t->st, d_roots, (uint32_t)roots.size(), t->kshift, per_tree, t->fb[0], &t->c->act[0],
                                g->stream);
             launch_fill_u32(&t->c->fenc[0][0], ~0u, 1, g->stream);  // the roots were marked at distance 0
-            s = run_rounds(t, M_RELAX, rounds, nullptr);
+            int passes = 0;
+            s = scratch_passes(t, passes);
+            if (!s) s = run_rounds(t, M_RELAX, rounds, nullptr);
             t->deep = rounds > DEEP_ROUNDS;
         }
         if (!s && g->zero_w) s = repick_parents(t, true, rounds);
@@ -1380,16 +1428,10 @@ extern "C" int dsssp_tree_relax_full(dsssp_tree *t, dsssp_stats *stats)
     if (t->kshift < 6) {
         // Each live edge once, both directions, from its higher-id endpoint's
         // row; the slots it lowers are marked in bitmap 0 for round 0.
-        FArgs a = base_args(t);
-        a.identity = 1;
-        a.consume = 0;
-        a.local = 0;
-        a.delta = 0;
-        a.heavy = a.chunk = SWEEP_PIECE;
-        int cus = 0;
-        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+        int grid = 0;
+        const FArgs a = sweep_args(t, grid);
         HIP_TRY(hipEventRecord(g->ev[3], s));
-        single_pass(t, M_SWEEP, a, cus * SWEEP_WG_PER_CU);
+        single_pass(t, M_SWEEP, a, grid);
         HIP_TRY(hipEventRecord(g->ev[4], s));
         launch_reduce_stats(t->c, t->bstats, s);
         HIP_TRY(hipMemcpyAsync(t->c_host, t->c, sizeof(Counters), hipMemcpyDeviceToHost, s));
@@ -1400,8 +1442,12 @@ extern "C" int dsssp_tree_relax_full(dsssp_tree *t, dsssp_stats *stats)
         sw[1] = c.stats[ST_RELAX_EDGES];
         sw[2] = c.stats[ST_RELAX_MARKS];
         sw[3] = c.stats[ST_SWEEP_GATHERS];
+        // Distance-ordered rounds start from a threshold at distance 0 (an
+        // empty ring would mean no threshold at all).
+        if (relax_delta(t)) launch_fill_u32(&t->c->fenc[0][0], ~0u, 1, s);
         code = run_rounds(t, M_RELAX, rounds, &kernel_ms);
     } else {
+        if (relax_delta(t)) launch_fill_u32(&t->c->fenc[0][0], ~0u, 1, s);
         code = run_rounds(t, M_RELAX, rounds, &kernel_ms, true);  // round 0 pushes from every slot
     }
     t->last_rounds[M_RELAX] = keep_rounds;

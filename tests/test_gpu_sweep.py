@@ -243,6 +243,31 @@ def test_relax_full_converges_upper_bounds_zero_weights(zero):
         assert oracle.certificate(csr, dist, par, [src])[0] == 0
 
 
+def test_relax_full_ordered_on_a_deep_grid():
+    """From a tree holding only its root, relax_full on a deep grid with a
+    distance step scans fewer entries than without one (its rounds start from
+    a threshold at distance 0), with the same exact result."""
+    side = 200
+    a, b, w = ds.gen_grid(side, 0.01, 1, 10000, seed=19)
+    n = side * side
+    par = np.full(n, -1, np.int32)
+    dist = np.full(n, ds.INF, np.int64)
+    par[0], dist[0] = 0, 0
+    out = {}
+    for delta in (-1, 5000):
+        say(f"relax_full grid delta={delta}")
+        with tuned(delta=delta):
+            g = ds.Graph(n, a, b, w)
+            t = ds.Tree(g, 0, parent=par, dist=dist)
+            st = t.relax_full()
+            out[delta] = (t.read(), st["relax_edges"])
+    (p0, d0), e0 = out[-1]
+    (p1, d1), e1 = out[5000]
+    assert np.array_equal(d0, d1) and np.array_equal(p0, p1)
+    assert e1 < e0, (e1, e0)
+    check_tree(Mirror(n, a, b, w), p1, d1, [0], "relax_full grid")
+
+
 @pytest.mark.parametrize("sweep", [-1, 1])
 def test_relax_full_converges_upper_bounds(sweep):
     """A given tree whose distances are lengths of real paths but not shortest
