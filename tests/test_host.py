@@ -245,3 +245,25 @@ def test_op_tools_apply_and_ce(tmp_path):
                                             "3"]))
     c = ds.read_changes(str(ce), 300)
     assert len(c) == 100 and int(np.sum(c[:, 3] == 1)) == 40 and c[:, 2].max() <= 99
+
+
+def test_op_main_argument_and_input_errors(tmp_path):
+    """The drop-in CLI (SSSPmainForUndirected.cu:17-32 argv): a wrong argument
+    count prints the usage with status 2; an unreadable input is named on
+    stderr with a non-zero status (the reference prints and exits 0).  Both are
+    decided on the host before any device call."""
+    cli = os.path.join(os.path.dirname(ds.LIB_PATH), "op_main")
+    r = subprocess.run([cli], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage: op_main" in (r.stdout + r.stderr)
+    missing = str(tmp_path / "no_such_cE.txt")
+    r = subprocess.run([cli, os.path.join(GOLDEN, "toy_graph.txt"), "6", "6", os.path.join(GOLDEN, "toy_SSSP.txt"),
+                        missing], capture_output=True, text=True)
+    assert r.returncode != 0 and missing in r.stderr
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus above 1 exits with a message."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run(["python", os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0 and "one GPU" in (r.stdout + r.stderr)
