@@ -230,8 +230,10 @@ int dsssp_tree_copy(dsssp_tree *dst, dsssp_tree *src);
  * to the shortest ones; on a converged tree nothing changes and, with
  * positive weights, stats->relax_improved is 0: a device-side check that no
  * live edge shortens any distance (with zero weights, equal-distance parents
- * may move to smaller ids, and the parent repair restores the canonical ones).  stats: the relaxation fields and times, and the pass's own
- * device time and algorithmic bytes (t_sweep_ms, alg_bytes_sweep). */
+ * may move to smaller ids, and the parent repair restores the canonical ones).
+ * stats: the relaxation fields and times, and the pass's own device time and
+ * algorithmic bytes (t_sweep_ms, alg_bytes_sweep; zero for 64 trees per
+ * vertex, whose first round is timed with the others). */
 int dsssp_tree_relax_full(dsssp_tree *t, dsssp_stats *stats);
 /* One shortest-path forest grown from several roots at once (each root has
  * distance 0 and parent itself), computed on the GPU.  Used for many graphs
