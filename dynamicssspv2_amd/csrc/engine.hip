@@ -1367,11 +1367,18 @@ extern "C" int dsssp_forest_read(dsssp_tree *t, int32_t tree, int32_t *parent, i
     const size_t stride = (size_t)1 << t->kshift;
     HIP_TRY(hipStreamSynchronize(t->g->stream));
     std::vector<uint64_t> st(n);
-    if (stride == 1)
+    if (stride == 1) {
         HIP_TRY(hipMemcpy(st.data(), t->st, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
-    else
-        HIP_TRY(hipMemcpy2D(st.data(), sizeof(uint64_t), t->st + tree, sizeof(uint64_t) * stride, sizeof(uint64_t), n,
-                            hipMemcpyDeviceToHost));
+    } else {
+        // the tree's states gathered into one contiguous buffer on the device
+        uint64_t *col = nullptr;
+        if (int s = dev_alloc((void **)&col, sizeof(uint64_t) * n, "forest read")) return s;
+        launch_gather_tree(t->st, n, t->kshift, (uint32_t)tree, col, t->g->stream);
+        hipError_t e = hipMemcpyAsync(st.data(), col, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, t->g->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(t->g->stream);
+        dev_free(col);
+        if (e != hipSuccess) return fail(DSSSP_ERR_HIP, "forest read failed: %s", hipGetErrorString(e));
+    }
     for (size_t v = 0; v < n; ++v) {
         if (st[v] == ST_INF) {
             parent[v] = -1;

@@ -90,6 +90,9 @@ void launch_frontier(int mode, const FArgs &a, int grid, hipStream_t s, hipEvent
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 void launch_reduce_stats(Counters *c, unsigned long long *bstats, hipStream_t s);
+// out[v] = st[v << kshift | tree] for v < n (one tree of a forest, contiguous).
+void launch_gather_tree(const uint64_t *st, uint64_t n, uint32_t kshift, uint32_t tree, uint64_t *out,
+                        hipStream_t s);
 // *out = an order-independent digest of p[0, n) (equal arrays, equal digests).
 void launch_digest(const uint64_t *p, uint64_t n, unsigned long long *out, hipStream_t s);
 // Insertion of a batch into the delta blocks.  Thread 2i+s places copy s of

@@ -1303,6 +1303,14 @@ __global__ __launch_bounds__(RB) void k_fill_u32(uint32_t *p, uint32_t v, uint64
     for (uint64_t i = (uint64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * RB) p[i] = v;
 }
 
+// out[v] = st[v << kshift | tree]: one tree's states, contiguous.
+__global__ __launch_bounds__(RB) void k_gather_tree(const uint64_t *st, uint64_t n, uint32_t kshift, uint32_t tree,
+                                                    uint64_t *out)
+{
+    for (uint64_t v = (uint64_t)blockIdx.x * RB + threadIdx.x; v < n; v += (uint64_t)gridDim.x * RB)
+        out[v] = st[(v << kshift) | tree];
+}
+
 // Order-independent digest of p[0, n): the sum of a 64-bit mix of (index,
 // value), one atomicAdd per wave.
 __global__ __launch_bounds__(RB) void k_digest(const uint64_t *p, uint64_t n, unsigned long long *out)
@@ -1420,6 +1428,14 @@ void launch_digest(const uint64_t *p, uint64_t n, unsigned long long *out, hipSt
     if (n == 0) return;
     const unsigned g = (unsigned)std::min<uint64_t>((n + RB - 1) / RB, 4096);
     hipLaunchKernelGGL(k_digest, dim3(g), dim3(RB), 0, s, p, n, out);
+}
+
+void launch_gather_tree(const uint64_t *st, uint64_t n, uint32_t kshift, uint32_t tree, uint64_t *out,
+                        hipStream_t s)
+{
+    if (n == 0) return;
+    const unsigned g = (unsigned)std::min<uint64_t>((n + RB - 1) / RB, 8192);
+    hipLaunchKernelGGL(k_gather_tree, dim3(g), dim3(RB), 0, s, st, n, kshift, tree, out);
 }
 
 void launch_reduce_stats(Counters *c, unsigned long long *bstats, hipStream_t s)

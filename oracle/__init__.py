@@ -44,6 +44,8 @@ def lib():
         i64, i32, p = ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p
         L.orc_build_csr.argtypes = [i64, i64, p, p, p, p, p, p]
         L.orc_build_csr.restype = ctypes.c_int
+        L.orc_build_csr_any.argtypes = [i64, i64, p, p, p, p, p, p]
+        L.orc_build_csr_any.restype = ctypes.c_int
         L.orc_djk.argtypes = [i64, p, p, p, i32, i64, p, p]
         L.orc_djk.restype = i64
         L.orc_dijkstra_lazy.argtypes = [i64, p, p, p, p, i32, p, p]
@@ -82,15 +84,20 @@ class CSR:
         self.rp, self.col, self.wt = rp, col, wt
 
     @classmethod
-    def from_edges(cls, n, a, b, w) -> "CSR":
+    def from_edges(cls, n, a, b, w, ordered: bool = True) -> "CSR":
+        """ordered=False: rows in an unspecified order, built by all threads; for
+        the order-independent checks (distances, canonical parents, certificate)."""
         a, b, w = _i32(a), _i32(b), _i32(w)
         m = len(a)
-        rp = np.zeros(n + 1, np.int64)
-        col = np.zeros(max(2 * m, 1), np.int32)
-        wt = np.zeros(max(2 * m, 1), np.int32)
-        rc = lib().orc_build_csr(n, m, _ptr(a), _ptr(b), _ptr(w), _ptr(rp), _ptr(col), _ptr(wt))
-        if rc != 0:
+        rp = np.empty(n + 1, np.int64)
+        col = np.empty(max(2 * m, 1), np.int32)
+        wt = np.empty(max(2 * m, 1), np.int32)
+        build = lib().orc_build_csr if ordered else lib().orc_build_csr_any
+        rc = build(n, m, _ptr(a), _ptr(b), _ptr(w), _ptr(rp), _ptr(col), _ptr(wt))
+        if rc == -1:
             raise ValueError("edge endpoint outside [0, n)")
+        if rc != 0:
+            raise MemoryError
         return cls(n, rp, col[: 2 * m], wt[: 2 * m])
 
 

@@ -43,6 +43,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define ORC_INF INT64_MAX
 
@@ -76,6 +79,45 @@ int orc_build_csr(int64_t n, int64_t m, const int32_t *a, const int32_t *b,
         col[p] = b[i];
         wt[p] = w[i];
         p = fill[b[i]]++;
+        col[p] = a[i];
+        wt[p] = w[i];
+    }
+    free(fill);
+    return 0;
+}
+
+/* The same adjacency with each row in an unspecified order, built by all
+ * threads (atomic row cursors).  For the order-independent checks (Dijkstra
+ * distances, canonical parents, the certificate) on graphs of 10^8+ edges.
+ * Returns 0, -1 if an endpoint is outside [0,n), -2 on allocation failure. */
+int orc_build_csr_any(int64_t n, int64_t m, const int32_t *a, const int32_t *b,
+                      const int32_t *w, int64_t *rp, int32_t *col, int32_t *wt)
+{
+    int64_t i, bad = 0;
+    int64_t *fill;
+#pragma omp parallel for schedule(static)
+    for (i = 0; i <= n; i++) rp[i] = 0;
+#pragma omp parallel for schedule(static) reduction(+ : bad)
+    for (i = 0; i < m; i++) {
+        if (a[i] < 0 || a[i] >= n || b[i] < 0 || b[i] >= n) {
+            bad++;
+            continue;
+        }
+        __atomic_fetch_add(&rp[a[i] + 1], 1, __ATOMIC_RELAXED);
+        __atomic_fetch_add(&rp[b[i] + 1], 1, __ATOMIC_RELAXED);
+    }
+    if (bad) return -1;
+    for (i = 0; i < n; i++) rp[i + 1] += rp[i];
+    fill = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    if (!fill) return -2;
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < n; i++) fill[i] = rp[i];
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < m; i++) {
+        int64_t p = __atomic_fetch_add(&fill[a[i]], 1, __ATOMIC_RELAXED);
+        col[p] = b[i];
+        wt[p] = w[i];
+        p = __atomic_fetch_add(&fill[b[i]], 1, __ATOMIC_RELAXED);
         col[p] = a[i];
         wt[p] = w[i];
     }
@@ -300,11 +342,15 @@ int64_t orc_apply_changes(int64_t n, int64_t m, const int32_t *a, const int32_t 
                           const int32_t *w, int64_t k, const int32_t *chg,
                           int32_t *oa, int32_t *ob, int32_t *ow)
 {
-    int64_t i, cnt = 0, nd = 0;
+    int64_t i, cnt = 0, nd = 0, total = m + k;
+    int64_t *part;
     del_key *dk;
+    char *keep;
+    int nt = 1, used = 1;
     (void)n;
     dk = (del_key *)malloc(sizeof(del_key) * (size_t)(k > 0 ? k : 1));
-    if (!dk) return -1;
+    keep = (char *)malloc((size_t)(total > 0 ? total : 1));
+    if (!dk || !keep) { free(dk); free(keep); return -1; }
     for (i = 0; i < k; i++) {
         if (chg[4 * i + 3] == 0) {
             int32_t x = chg[4 * i], y = chg[4 * i + 1];
@@ -315,23 +361,62 @@ int64_t orc_apply_changes(int64_t n, int64_t m, const int32_t *a, const int32_t 
         }
     }
     qsort(dk, (size_t)nd, sizeof(del_key), del_key_cmp);
-    for (i = 0; i < m + k; i++) {
+    /* Survival of each edge is independent of the others: decided by all
+     * threads, then compacted in the original order (the result is the same
+     * as one sequential pass). */
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < total; i++) {
         int32_t x, y, ww;
         del_key key;
         if (i < m) {
             x = a[i]; y = b[i]; ww = w[i];
         } else {
             const int32_t *c = chg + 4 * (i - m);
+            keep[i] = 0;
             if (c[3] != 1) continue;
             x = c[0]; y = c[1]; ww = c[2];
         }
         key.lo = x < y ? x : y;
         key.hi = x < y ? y : x;
         key.w = ww;
-        if (nd > 0 && bsearch(&key, dk, (size_t)nd, sizeof(del_key), del_key_cmp)) continue;
-        oa[cnt] = x; ob[cnt] = y; ow[cnt] = ww;
-        cnt++;
+        keep[i] = !(nd > 0 && bsearch(&key, dk, (size_t)nd, sizeof(del_key), del_key_cmp));
     }
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    part = (int64_t *)calloc((size_t)nt + 1, sizeof(int64_t));
+    if (!part) { free(dk); free(keep); return -1; }
+#pragma omp parallel num_threads(nt)
+    {
+        int t = 0, j, nth = 1;
+        int64_t lo, hi, c = 0, p;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+        nth = omp_get_num_threads();
+#endif
+#pragma omp single
+        used = nth;
+        lo = total * t / nth;
+        hi = total * (t + 1) / nth;
+        for (p = lo; p < hi; p++) c += keep[p];
+        part[t + 1] = c;
+#pragma omp barrier
+        c = 0;
+        for (j = 0; j <= t; j++) c += part[j];
+        for (p = lo; p < hi; p++) {
+            if (!keep[p]) continue;
+            if (p < m) {
+                oa[c] = a[p]; ob[c] = b[p]; ow[c] = w[p];
+            } else {
+                const int32_t *q = chg + 4 * (p - m);
+                oa[c] = q[0]; ob[c] = q[1]; ow[c] = q[2];
+            }
+            c++;
+        }
+    }
+    for (i = 1; i <= used; i++) cnt += part[i];
+    free(part);
+    free(keep);
     free(dk);
     return cnt;
 }
@@ -497,6 +582,7 @@ int64_t orc_canonical_parents(int64_t n, const int64_t *rp, const int32_t *col,
     if (!is_src) return -1;
     for (i = 0; i < nsrc; i++) is_src[srcs[i]] = 1;
     if (rule == 0) {
+#pragma omp parallel for schedule(dynamic, 4096) private(j) reduction(+ : bad)
         for (i = 0; i < n; i++) {
             int32_t best = -1;
             if (is_src[i]) { parent[i] = (int32_t)i; continue; }
@@ -529,6 +615,7 @@ int64_t orc_canonical_parents(int64_t n, const int64_t *rp, const int32_t *col,
                 }
             }
         }
+#pragma omp parallel for schedule(dynamic, 4096) private(j) reduction(+ : bad)
         for (i = 0; i < n; i++) {
             int32_t best = -1;
             if (is_src[i]) { parent[i] = (int32_t)i; continue; }
@@ -575,28 +662,51 @@ int orc_certificate(int64_t n, const int64_t *rp, const int32_t *col, const int3
         is_src[s] = 1;
         if (dist[s] != 0 || parent[s] != s) { *where = s; free(is_src); return 1; }
     }
-    for (i = 0; i < n; i++) {
-        if (dist[i] == ORC_INF) continue;
-        for (j = rp[i]; j < rp[i + 1]; j++) {
-            int32_t v = col[j];
-            if (wt[j] < 0) continue;
-            if (dist[v] > dist[i] + wt[j]) { *where = v; free(is_src); return 2; }
+    /* C2 and C3/C5 are checked by all threads; the failure reported is the
+     * one a sequential pass in vertex order meets first. */
+    {
+        int64_t first = n;
+#pragma omp parallel for schedule(dynamic, 4096) private(j) reduction(min : first)
+        for (i = 0; i < n; i++) {
+            if (dist[i] == ORC_INF || i >= first) continue;
+            for (j = rp[i]; j < rp[i + 1]; j++) {
+                if (wt[j] < 0) continue;
+                if (dist[col[j]] > dist[i] + wt[j]) { first = i; break; }
+            }
+        }
+        if (first < n) {
+            for (j = rp[first]; j < rp[first + 1]; j++) {
+                int32_t v = col[j];
+                if (wt[j] >= 0 && dist[v] > dist[first] + wt[j]) { *where = v; break; }
+            }
+            free(is_src);
+            return 2;
         }
     }
-    for (i = 0; i < n; i++) {
-        int32_t p;
-        int ok = 0;
-        if (dist[i] == ORC_INF) {
-            if (parent[i] != -1) { *where = i; free(is_src); return 5; }
-            continue;
+    {
+        int64_t first = n;
+#pragma omp parallel for schedule(dynamic, 4096) private(j) reduction(min : first)
+        for (i = 0; i < n; i++) {
+            int32_t p;
+            int ok = 0;
+            if (i >= first) continue;
+            if (dist[i] == ORC_INF) {
+                if (parent[i] != -1) first = i;
+                continue;
+            }
+            if (is_src[i]) continue;
+            p = parent[i];
+            if (p < 0 || p >= n || p == i || dist[p] == ORC_INF) { first = i; continue; }
+            for (j = rp[i]; j < rp[i + 1]; j++) {
+                if (col[j] == p && wt[j] >= 0 && dist[p] + wt[j] == dist[i]) { ok = 1; break; }
+            }
+            if (!ok) first = i;
         }
-        if (is_src[i]) continue;
-        p = parent[i];
-        if (p < 0 || p >= n || p == i || dist[p] == ORC_INF) { *where = i; free(is_src); return 3; }
-        for (j = rp[i]; j < rp[i + 1]; j++) {
-            if (col[j] == p && wt[j] >= 0 && dist[p] + wt[j] == dist[i]) { ok = 1; break; }
+        if (first < n) {
+            *where = first;
+            free(is_src);
+            return dist[first] == ORC_INF ? 5 : 3;
         }
-        if (!ok) { *where = i; free(is_src); return 3; }
     }
     /* C4: iterative walk with three colours (0 new, 1 on stack, 2 done). */
     state = (char *)calloc((size_t)(n > 0 ? n : 1), 1);
