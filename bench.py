@@ -63,8 +63,9 @@ def parse():
     p.add_argument("--no-timing", action="store_true", help="no per-launch events (roofline unmeasured)")
     p.add_argument("--autotune", action="store_true",
                    help="plain configs: tune the launch knobs on the warm-up batches first (dsssp_autotune)")
-    p.add_argument("--copy-probe", action="store_true",
-                   help="only copy 1 GiB device-to-device five times (calibrates HBM byte counters), then exit")
+    p.add_argument("--probe", action="store_true",
+                   help="only run the library's probe launches of known footprint (a 1 GiB copy, 8-byte gathers "
+                        "over 8 GiB and over 64 MiB) for calibrating HBM byte counters, then exit")
     for knob in ("grid", "heavy_degree", "chunk", "round_batch", "use_graph", "edges_per_thread", "tile_entries",
                  "local_tiles", "delta", "sweep"):
         p.add_argument("--" + knob.replace("_", "-"), type=int, default=None, help="tuning knob (dsssp_tuning)")
@@ -76,24 +77,56 @@ def pct(x, q):
     return float(np.percentile(np.asarray(x), q)) if len(x) else None
 
 
-def disjoint_batches(ds, np, n, a, b, w, k, ins, count, seed, wlo=1, whi=1000, first_deletable=0):
-    """count disjoint batches of exactly k records each (round(k*ins) insertions)."""
-    allc = ds.gen_changes(n, a, b, w, k * count, ins, wlo, whi, seed=seed, first_deletable=first_deletable)
-    ins_rec = allc[allc[:, 3] == 1]
-    del_rec = allc[allc[:, 3] == 0]
+def evolving_batches(ds, np, n, a, b, w, k, ins, count, seed, wlo=1, whi=1000, first_deletable=0):
+    """count batches of exactly k records (round(k*ins) insertions) for one graph
+    that they change in turn.  Insertions are distinct pairs absent from the
+    graph (never deleted later, so absent when inserted); the deletions of batch
+    i are sampled uniformly, without replacement, from the edges present before
+    it: the deletable initial edges [first_deletable, m) and the insertions of
+    batches 0..i-1, less everything deleted by batches 0..i-1."""
     ki = int(round(k * ins))
     kd = k - ki
+    m = len(a)
+    ins_all = (ds.gen_changes(n, a, b, w, ki * count, 1.0, wlo, whi, seed=seed) if ki
+               else np.zeros((0, 4), np.int32))
+    A = np.concatenate([a, ins_all[:, 0]])
+    B = np.concatenate([b, ins_all[:, 1]])
+    W = np.concatenate([w, ins_all[:, 2]])
+    dead = np.zeros(len(A), bool)
+    dead[:first_deletable] = True
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
-        c = np.concatenate([ins_rec[i * ki:(i + 1) * ki], del_rec[i * kd:(i + 1) * kd]])
+        present = m + ki * i  # index space of the edges present before batch i
+        pick = np.zeros(0, np.int64)
+        while len(pick) < kd:
+            cand = np.unique(rng.integers(0, present, 2 * (kd - len(pick)) + 64))
+            cand = cand[~dead[cand]]
+            cand = rng.permutation(cand)[:kd - len(pick)]
+            dead[cand] = True
+            pick = np.concatenate([pick, cand])
+        flip = rng.random(kd) < 0.5
+        dels = np.stack([np.where(flip, B[pick], A[pick]), np.where(flip, A[pick], B[pick]), W[pick],
+                         np.zeros(kd, np.int32)], 1).astype(np.int32)
+        c = np.concatenate([ins_all[i * ki:(i + 1) * ki], dels])
         rng.shuffle(c)
         out.append(np.ascontiguousarray(c))
     return out
 
 
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_check(ds, np, g, n, srcs, par, dist, k, label, run_cpu=True):
-    """Exactness on the final graph + the CPU baseline (one Dijkstra per source)."""
+    """Exactness on the final graph + the CPU baseline (one Dijkstra per source),
+    with the textbook lazy-deletion Dijkstra timed beside it for context."""
     import oracle
     ea, eb, ew = g.export()
     csr = oracle.CSR.from_edges(n, ea, eb, ew)
@@ -104,15 +137,20 @@ def cpu_check(ds, np, g, n, srcs, par, dist, k, label, run_cpu=True):
     else:
         _, cd = oracle.dijkstra(csr, [int(s) for s in srcs])
     t_cpu = time.perf_counter() - tc
-    mism = int(np.sum(cd != dist))
+    tc = time.perf_counter()
+    _, ld = oracle.dijkstra(csr, [int(s) for s in srcs])
+    t_lazy = time.perf_counter() - tc
+    mism = int(np.sum(cd != dist)) + int(np.sum(ld != dist))
     code, _ = oracle.certificate(csr, dist, par, [int(s) for s in srcs])
     canon = bool(np.array_equal(par, oracle.canonical_parents(csr, cd, [int(s) for s in srcs], rule=0)))
     check = {"dijkstra_mismatches": mism, "certificate": int(code), "parents_canonical": canon}
     cpu = {"value": k / t_cpu, "unit": "changed edges/s", "cores": 1, "kind": "port",
            "sample": f"one from-scratch sequential Dijkstra (oracle/oracle.c, restating "
                      f"seqSSSPwithDist_undir.cpp:66-128) on the final {label} graph ({len(csr.col) // 2} edges) "
-                     f"= the reference's CPU cost of one k={k} batch; {t_cpu:.2f} s"}
-    log(f"cpu dijkstra {t_cpu:.2f}s, mismatches {mism}, certificate {code}, canonical {canon}")
+                     f"= the reference's CPU cost of one k={k} batch; {t_cpu:.2f} s",
+           "cpu_model": cpu_model(), "nproc": os.cpu_count(), "seconds": t_cpu,
+           "lazy_dijkstra_seconds": t_lazy}
+    log(f"cpu dijkstra {t_cpu:.2f}s (lazy {t_lazy:.2f}s), mismatches {mism}, certificate {code}, canonical {canon}")
     return check, cpu
 
 
@@ -178,7 +216,7 @@ def line(args, value, wall, steps, cfg, stats, check, cpu, unit="changed edges/s
                               and check.get("dijkstra_mismatches", 0) == 0 and check.get("certificate", 0) == 0),
         "check": check, "roofline": roofline(stats, traffic), "cpu_baseline": cpu,
         "phases_ms": {ph: sum(s[f"t_{ph}_ms"] for s in stats) / max(1, len(stats))
-                      for ph in ("insert", "classify", "invalidate", "seed", "relax")},
+                      for ph in ("upkeep", "insert", "classify", "invalidate", "seed", "relax")},
         "rounds": {"invalidate": sum(s["inv_rounds"] for s in stats) / max(1, len(stats)),
                    "relax": sum(s["relax_rounds"] for s in stats) / max(1, len(stats))},
     }
@@ -187,8 +225,10 @@ def line(args, value, wall, steps, cfg, stats, check, cpu, unit="changed edges/s
     print(json.dumps(out), flush=True)
 
 
-def timed_steps(ds, g, t, res, warmup, timing=True):
-    for i in range(warmup):
+def timed_steps(ds, g, t, res, warmup, timing=True, warm=True):
+    """Applies res[warmup:] (after res[:warmup] untimed when warm is set)
+    between two device synchronisations; returns the stats and the wall time."""
+    for i in range(warmup if warm else 0):
         ds.update(g, t, res[i])
     ds.set_tuning(kernel_timing=1 if timing else 0)
     stats = []
@@ -200,7 +240,7 @@ def timed_steps(ds, g, t, res, warmup, timing=True):
     return stats, wall
 
 
-def run_plain(args, ds, np, n, a, b, w, src, batches, label, cfg, traffic=False, extra_batches=None):
+def run_plain(args, ds, np, n, a, b, w, src, batches, label, cfg, traffic=False, reruns=False):
     g = ds.Graph(n, a, b, w)
     t = ds.Tree(g, src)
     res = [ds.Batch(g, c) for c in batches]
@@ -210,23 +250,47 @@ def run_plain(args, ds, np, n, a, b, w, src, batches, label, cfg, traffic=False,
         tun, tms = ds.autotune(g, t, res[:args.warmup])
         log(f"{label}: autotuned in {time.perf_counter() - t0:.1f}s: {tun}, {tms:.3f} ms per warm-up batch")
         extra["autotuned"] = tun
-    stats, wall = timed_steps(ds, g, t, res, args.warmup, not args.no_timing)
-    for r in res:
-        r.close()
+    for r in res[:args.warmup]:
+        ds.update(g, t, r)
+    snap = None
+    if reruns:  # the state before the timed steps, to run the same batches again
+        snap = (g.clone(),)
+        snap = (snap[0], ds.Tree(snap[0], src))
+        snap[1].copy_from(t)
+    stats, wall = timed_steps(ds, g, t, res, args.warmup, not args.no_timing, warm=False)
     k = len(batches[-1])
     value = k * args.steps / wall
     log(f"{label}: {args.steps} steps in {wall:.3f}s -> {value:.3e} changed edges/s, "
         f"p50 {pct([s['t_update_ms'] for s in stats], 50):.3f} ms")
-    if extra_batches:
-        # The same workload continued without per-launch timing events, whose
-        # dispatch-side recording adds a few microseconds to every launch.
-        res = [ds.Batch(g, c) for c in extra_batches]
-        st2, wall2 = timed_steps(ds, g, t, res, 0, False)
-        for r in res:
-            r.close()
-        extra["value_without_launch_events"] = k * len(extra_batches) / wall2
+    if snap:
+        # The same batches from the same state without per-launch timing
+        # events (whose dispatch-side recording adds a few microseconds to
+        # every launch), then from host memory through dsssp_update, which adds
+        # the PCIe copy of each batch (t_h2d_ms, outside t_update_ms).
+        final = (g.clone(),)
+        final = (final[0], ds.Tree(final[0], src))
+        final[1].copy_from(t)
+        g.copy_from(snap[0])
+        t.copy_from(snap[1])
+        st2, wall2 = timed_steps(ds, g, t, res, args.warmup, False, warm=False)
+        extra["value_without_launch_events"] = k * args.steps / wall2
         extra["p50_ms_without_launch_events"] = pct([s["t_update_ms"] for s in st2], 50)
         log(f"{label}: without launch events {extra['value_without_launch_events']:.3e} changed edges/s")
+        g.copy_from(snap[0])
+        t.copy_from(snap[1])
+        st3, wall3 = timed_steps(ds, g, t, batches, args.warmup, False, warm=False)
+        extra["value_pcie_inclusive"] = k * args.steps / wall3
+        extra["t_h2d_ms"] = sum(s["t_h2d_ms"] for s in st3) / len(st3)
+        log(f"{label}: from host memory {extra['value_pcie_inclusive']:.3e} changed edges/s, "
+            f"H2D {extra['t_h2d_ms']:.3f} ms per batch")
+        eq = bool(np.array_equal(np.stack(t.read()), np.stack(final[1].read())))
+        extra["reruns_equal"] = eq
+        g.copy_from(final[0])
+        t.copy_from(final[1])
+        for o in (final[1], final[0], snap[1], snap[0]):
+            o.close()
+    for r in res:
+        r.close()
     # One full relaxation sweep over the converged tree: every live entry of
     # every reachable vertex, rows in vertex order.  Nothing may change (a
     # device-side fixed-point check); its launch is the kernel's streaming case.
@@ -256,7 +320,7 @@ def run_plain(args, ds, np, n, a, b, w, src, batches, label, cfg, traffic=False,
 def config1(args, ds, np):
     n, m, k = 100_000, 1_000_000, 10_000
     a, b, w = ds.gen_tree_plus_random(n, m, 1, 100, seed=args.seed)
-    batches = disjoint_batches(ds, np, n, a, b, w, k, 0.5, args.warmup + args.steps, args.seed + 1, 1, 100,
+    batches = evolving_batches(ds, np, n, a, b, w, k, 0.5, args.warmup + args.steps, args.seed + 1, 1, 100,
                                first_deletable=n - 1)
     run_plain(args, ds, np, n, a, b, w, 0, batches, "config1",
               {"workload": "random tree+pairs n=1e5 m=1e6 k=1e4 ins=0.5", "n": n, "m": m, "k": k, "src": 0})
@@ -270,16 +334,15 @@ def config2(args, ds, np):
     src = int(np.argmax(deg))  # ties -> smallest id
     log(f"rmat scale={args.scale} n={n} m={len(a)} max_deg={deg.max()} src={src} ({time.time() - t0:.1f}s)")
     t0 = time.time()
-    nb = args.warmup + 2 * args.steps if not args.no_timing else args.warmup + args.steps
-    allb = disjoint_batches(ds, np, n, a, b, w, args.k, args.ins, nb, args.seed + 1)
-    batches, extra_batches = allb[:args.warmup + args.steps], allb[args.warmup + args.steps:]
+    nb = args.warmup + args.steps
+    batches = evolving_batches(ds, np, n, a, b, w, args.k, args.ins, nb, args.seed + 1)
     log(f"{nb} batches of k={args.k} generated ({time.time() - t0:.1f}s)")
     cfg = {"workload": f"rmat-s{args.scale}-ef{args.edge_factor} batch k={args.k} ins={args.ins:g}",
            "scale": args.scale, "n": n, "m": len(a), "k": args.k, "ins_frac": args.ins, "src": src,
            "weights": "U[1,1000]"}
     default = args.scale == 24 and args.edge_factor == 16 and args.k == 1_000_000 and args.ins == 0.5
     run_plain(args, ds, np, n, a, b, w, src, batches, f"rmat-s{args.scale}", cfg, traffic=default,
-              extra_batches=extra_batches)
+              reruns=not args.no_timing)
 
 
 def config3(args, ds, np):
@@ -327,7 +390,7 @@ def config4a(args, ds, np):
     n = 1 << scale
     rng = np.random.default_rng(args.seed)
     srcs = rng.integers(0, n, K).astype(np.int32)
-    batches = disjoint_batches(ds, np, n, a, b, w, k, 0.5, args.warmup + args.steps, args.seed + 1)
+    batches = evolving_batches(ds, np, n, a, b, w, k, 0.5, args.warmup + args.steps, args.seed + 1)
     g = ds.Graph(n, a, b, w)
     f = ds.Tree(g, sources=srcs)
     res = [ds.Batch(g, c) for c in batches]
@@ -506,18 +569,16 @@ def main():
         sys.exit(2)
     import numpy as np
 
-    if args.copy_probe:
-        import torch
-        x = torch.empty(1 << 27, dtype=torch.int64, device="cuda")
-        y = torch.empty_like(x)
-        x.fill_(1)
-        for _ in range(5):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        log("copied 1 GiB five times")
-        return
     import dynamicssspv2_amd as ds
     ds.set_device(0)
+    if args.probe:
+        # kind 0: 1 GiB read + 1 GiB written; kind 1: 2^26 reads of 8 bytes at
+        # hashed positions of 8 GiB (beyond every cache) and of 64 MiB (held in
+        # the last-level cache)
+        for kind, nbytes, count in ((0, 1 << 30, 0), (1, 8 << 30, 1 << 26), (1, 64 << 20, 1 << 26)):
+            ms = ds.probe(kind, nbytes, count, reps=5)
+            log(f"probe kind {kind} bytes {nbytes} count {count}: {ms:.3f} ms per launch")
+        return
     knobs = {k: getattr(args, k) for k in ("grid", "heavy_degree", "chunk", "round_batch", "use_graph",
                                            "edges_per_thread", "tile_entries", "local_tiles", "delta")
              if getattr(args, k) is not None}

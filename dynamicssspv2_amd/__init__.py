@@ -49,7 +49,8 @@ class Stats(ctypes.Structure):
         [(f, ctypes.c_double) for f in (
             "t_h2d_ms", "t_update_ms", "t_insert_ms", "t_classify_ms", "t_invalidate_ms", "t_seed_ms",
             "t_relax_ms", "t_relax_kernel_ms", "alg_bytes", "alg_bytes_relax")] + \
-        [("relax_sweeps", ctypes.c_int64), ("t_sweep_ms", ctypes.c_double), ("alg_bytes_sweep", ctypes.c_double)]
+        [("relax_sweeps", ctypes.c_int64), ("t_sweep_ms", ctypes.c_double), ("alg_bytes_sweep", ctypes.c_double),
+         ("t_upkeep_ms", ctypes.c_double)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -83,7 +84,7 @@ DELTA_AUTO = -2  # Tuning.delta: ordered relaxation chosen per tree (include/dss
 class Tuning(ctypes.Structure):
     _fields_ = [(f, ctypes.c_int32) for f in ("block", "grid", "heavy_degree", "chunk", "round_batch",
                                               "kernel_timing", "use_graph", "edges_per_thread",
-                                              "tile_entries", "local_tiles", "delta", "sweep")]
+                                              "tile_entries", "local_tiles", "delta", "sweep", "give_up_ms")]
 
 
 def build(force: bool = True) -> None:
@@ -116,6 +117,7 @@ def lib():
         "dsssp_get_tuning": ([ctypes.POINTER(Tuning)], ctypes.c_int),
         "dsssp_set_tuning": ([ctypes.POINTER(Tuning)], ctypes.c_int),
         "dsssp_free": ([p], None),
+        "dsssp_probe": ([i32, u64, u64, i32, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "dsssp_read_graph": ([ctypes.c_char_p, i64, P64, pp, pp, pp], ctypes.c_int),
         "dsssp_read_changes": ([ctypes.c_char_p, i64, P64, pp], ctypes.c_int),
         "dsssp_read_sssp": ([ctypes.c_char_p, i64, p, p], ctypes.c_int),
@@ -290,6 +292,15 @@ def set_tuning(**kw) -> None:
     cur.update(kw)
     t = Tuning(**cur)
     _check(lib().dsssp_set_tuning(ctypes.byref(t)))
+
+
+def probe(kind: int, nbytes: int, count: int = 0, reps: int = 5) -> float:
+    """Launches of known footprint for counter calibration (kind 0: copy of
+    nbytes; kind 1: count 8-byte reads at hashed positions of nbytes); returns
+    the device ms per launch."""
+    ms = ctypes.c_double()
+    _check(lib().dsssp_probe(kind, nbytes, count, reps, ctypes.byref(ms)))
+    return ms.value
 
 
 def set_device(device: int) -> None:

@@ -100,6 +100,7 @@ struct Counters {
     uint32_t aux_hcnt;  // heavy chunks queued by a single-pass stage
     uint32_t err;
     unsigned long long stats[NSTAT];
+    unsigned long long popc;  // set bits counted by the last bitmap merge
 };
 
 // Device-side control block of one graph (delta pool allocation).

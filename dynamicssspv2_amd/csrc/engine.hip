@@ -43,7 +43,7 @@ using namespace dsssp;
 
 namespace {
 
-dsssp_tuning g_tuning = {FRONTIER_BLOCK, 1024, 256, 256, 4, 0, 0, 4, 1024, 2, DSSSP_DELTA_AUTO, -1};
+dsssp_tuning g_tuning = {FRONTIER_BLOCK, 1024, 256, 256, 4, 0, 0, 4, 1024, 2, DSSSP_DELTA_AUTO, -1, 120000};
 std::mutex g_tuning_mu;
 thread_local int t_device = 0;
 
@@ -146,7 +146,8 @@ struct dsssp_graph {
     GraphCtl *gc_host = nullptr;  // pinned
     bool zero_w = false;          // some weight was ever 0: parents need the tight-BFS repair
     uint32_t auto_delta = 1;      // step of ordered relaxation: twice the mean input weight
-    hipEvent_t ev[8] = {};
+    // ev[0..6]: update phases; ev[7]: upkeep start; ev[8..9]: host batch copy
+    hipEvent_t ev[10] = {};
     std::vector<hipEvent_t> kev;  // per-launch timing events (kernel_timing)
     // Trees and batches attached to this graph.  dsssp_graph_destroy while any
     // remain only marks the graph; the last detach releases it, so objects may
@@ -288,13 +289,13 @@ void zero_ring(dsssp_tree *t)
 // One pass of a single-pass mode (seed, parent pick) into bitmap 0 / flag 0.
 // Rows longer than the heavy threshold are queued as chunks on list 1 (idle
 // here) by the first launch and processed by a second, chunk-only launch.
-void single_pass(dsssp_tree *t, int mode, FArgs a, int min_grid = 0)
+void single_pass(dsssp_tree *t, int mode, FArgs a, int min_grid = 0, uint64_t *out_bm = nullptr)
 {
     hipStream_t s = t->g->stream;
     const int grid = std::max(frontier_grid(t->nw), min_grid);
     launch_fill_u32(&t->c->aux_hcnt, 0u, 1, s);
     a.ring = 2;  // output ring slot 0, read by round 0 of the loop that follows
-    a.out_bm = t->fb[0];
+    a.out_bm = out_bm ? out_bm : t->fb[0];
     a.out_act = &t->c->act[0];
     a.out_hq = t->hq[1];
     a.out_hcnt = &t->c->aux_hcnt;
@@ -326,9 +327,11 @@ FArgs sweep_args(dsssp_tree *t, int &grid)
 }
 
 // Full edge passes at the start of a tree built from scratch (the roots are
-// marked in bitmap 0; the passes add the slots they lower), while a pass
-// lowers at least 1/SCRATCH_PASS_SHARE of the slots.  Relaxation rounds
-// started from bitmap 0 then reach the same fixed point as without them.
+// marked in bitmap 0), while a pass lowers at least 1/SCRATCH_PASS_SHARE of
+// the slots.  A pass marks the slots it lowers in bitmap 1, which is then
+// counted (distinct slots, not improvements) and merged into bitmap 0, so
+// relaxation rounds started from bitmap 0 reach the same fixed point as
+// without the passes.
 int scratch_passes(dsssp_tree *t, int &passes)
 {
     passes = 0;
@@ -336,17 +339,13 @@ int scratch_passes(dsssp_tree *t, int &passes)
     hipStream_t s = t->g->stream;
     int grid = 0;
     const FArgs a = sweep_args(t, grid);
-    unsigned long long prev = 0;
     for (int k = 0; k < SCRATCH_PASSES; ++k) {
-        single_pass(t, M_SWEEP, a, grid);
+        single_pass(t, M_SWEEP, a, grid, t->fb[1]);
         ++passes;
-        launch_reduce_stats(t->c, t->bstats, s);
-        HIP_TRY(hipMemcpyAsync(&t->c_host->stats[ST_RELAX_MARKS], &t->c->stats[ST_RELAX_MARKS],
-                               sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+        launch_merge_count(t->fb[0], t->fb[1], t->nw, &t->c->popc, s);
+        HIP_TRY(hipMemcpyAsync(&t->c_host->popc, &t->c->popc, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        const unsigned long long total = t->c_host->stats[ST_RELAX_MARKS];
-        if ((total - prev) * SCRATCH_PASS_SHARE < t->nslots) break;
-        prev = total;
+        if (t->c_host->popc * SCRATCH_PASS_SHARE < t->nslots) break;
     }
     return DSSSP_OK;
 }
@@ -416,10 +415,15 @@ int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms, bool
         }
         return a;
     };
-    // Each threshold move follows a round that visited nothing and settles at
-    // least the smallest marked slot, so ordered relaxation needs at most about
-    // twice the rounds of the plain one.
-    const int64_t max_rounds = (base.delta ? 3 : 1) * (int64_t)t->nslots + 64;
+    // Every round settles at least one more hop of every shortest path (of
+    // every tree at once), so a loop needs at most n rounds; each threshold move
+    // of ordered relaxation follows a round that visited nothing and settles at
+    // least the smallest marked slot, so it needs at most about twice that.
+    const int64_t max_rounds = (base.delta ? 3 : 1) * g->n + 64;
+    // A loop that runs longer than the give-up time is abandoned (every kernel
+    // has returned by the time the host looks); the caller scrubs the tree.
+    const auto t_start = std::chrono::steady_clock::now();
+    const double give_up_ms = tu.give_up_ms > 0 ? (double)tu.give_up_ms : 120000.0;
     int64_t i = 0;
     for (;;) {
         if (timed) {
@@ -453,9 +457,12 @@ int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms, bool
             }
         }
         if (t->c_host->act[ci] == 0 && t->c_host->hcnt[ci] == 0) break;
-        // Every round settles at least one more hop of every shortest path, so
-        // more rounds than max_rounds can only come from an internal error.
+        // More rounds than max_rounds can only come from an internal error.
         if (i > max_rounds) return fail(DSSSP_ERR_STATE, "frontier did not empty after %lld rounds", (long long)i);
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+        if (ms > give_up_ms)
+            return fail(DSSSP_ERR_STATE, "frontier did not empty within %.0f ms (%lld rounds); gave up", give_up_ms,
+                        (long long)i);
         group = i == first && first < max_group ? rb : std::min(group * 2, max_group);
     }
     t->last_rounds[mode] = i == first ? first - 1 : i;
@@ -701,14 +708,15 @@ void tombstone(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b)
     launch_tomb(b->rec, b->k, g->view(), t->bstats, g->stream);
 }
 
-int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stats, double h2d_ms)
+int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stats, bool h2d_timed)
 {
     DevGuard dg(g->device);
     hipStream_t s = g->stream;
+    // Upkeep of the resident graph (pool room, edge map load, item lists) is
+    // part of the update's time: ev[7] opens the window.
+    HIP_TRY(hipEventRecord(g->ev[7], s));
     if (b->n_ins > 0) {
         if (int e = ensure_pool(g, (uint64_t)b->n_ins)) return e;
-    }
-    if (b->n_ins > 0) {
         if (int e = ensure_emap(g, 2 * (uint64_t)b->n_ins)) return e;
     }
     if (int e = ensure_hq(t, g->E + g->delta_entries + 2 * (uint64_t)b->n_ins)) return e;
@@ -790,13 +798,14 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
         S.relax_enqueued = (int64_t)c.stats[ST_RELAX_MARKS];
         S.other_edges = (int64_t)c.stats[ST_OTHER_EDGES];
         S.probes = (int64_t)c.stats[ST_PROBES];
-        S.t_h2d_ms = h2d_ms;
+        S.t_h2d_ms = h2d_timed ? elapsed(g->ev[8], g->ev[9]) : 0.0;
+        S.t_upkeep_ms = elapsed(g->ev[7], g->ev[0]);
         S.t_insert_ms = elapsed(g->ev[0], g->ev[1]);
         S.t_classify_ms = elapsed(g->ev[1], g->ev[2]);
         S.t_invalidate_ms = elapsed(g->ev[2], g->ev[3]);
         S.t_seed_ms = elapsed(g->ev[3], g->ev[4]);
         S.t_relax_ms = elapsed(g->ev[4], g->ev[5]) + elapsed(g->ev[5], g->ev[6]);
-        S.t_update_ms = elapsed(g->ev[0], g->ev[6]);
+        S.t_update_ms = elapsed(g->ev[7], g->ev[6]);
         S.t_relax_kernel_ms = relax_kernel_ms;
         // Algorithmic bytes of relaxation: per frontier slot 16 B row bounds +
         // 8 B delta descriptor + 8 B own state; per scanned entry 8 B entry +
@@ -880,11 +889,12 @@ extern "C" int dsssp_set_tuning(const dsssp_tuning *t)
     if (t->local_tiles) nt.local_tiles = t->local_tiles;
     if (t->delta) nt.delta = t->delta;
     if (t->sweep) nt.sweep = t->sweep;
+    if (t->give_up_ms) nt.give_up_ms = t->give_up_ms;
     nt.kernel_timing = t->kernel_timing;
     nt.use_graph = t->use_graph;
     if (nt.grid < 1 || nt.grid > 65536 || nt.heavy_degree < 64 || nt.chunk < 64 || nt.chunk % 64 ||
         nt.round_batch < 1 || nt.round_batch > 64 || nt.tile_entries < 64 || nt.local_tiles < -1 ||
-        nt.local_tiles > 1024 || nt.delta < DSSSP_DELTA_AUTO || nt.sweep < -1 || nt.sweep > 1000 ||
+        nt.local_tiles > 1024 || nt.delta < DSSSP_DELTA_AUTO || nt.sweep < -1 || nt.sweep > 1000 || nt.give_up_ms < 1 ||
         (nt.edges_per_thread != 1 && nt.edges_per_thread != 2 && nt.edges_per_thread != 4))
         return fail(DSSSP_ERR_ARG, "tuning out of range");
     g_tuning = nt;
@@ -1502,6 +1512,41 @@ extern "C" int dsssp_tree_relax_full(dsssp_tree *t, dsssp_stats *stats)
     return DSSSP_OK;
 }
 
+extern "C" int dsssp_probe(int32_t kind, uint64_t bytes, uint64_t count, int32_t reps, double *ms)
+{
+    clear_error();
+    if ((kind != 0 && kind != 1) || bytes < 8 || reps < 1 || !ms) return fail(DSSSP_ERR_ARG, "bad argument");
+    if (int s = check_device(t_device)) return s;
+    DevGuard dg(t_device);
+    const uint64_t n = bytes / 8;
+    const uint64_t out_words = kind == 0 ? n : 8192ull * 256;
+    uint64_t *in = nullptr, *out = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int st = dev_alloc((void **)&in, 8 * n, "probe input");
+    if (!st) st = dev_alloc((void **)&out, 8 * out_words, "probe output");
+    if (!st && (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+                hipEventCreate(&e1) != hipSuccess))
+        st = fail(DSSSP_ERR_HIP, "probe setup failed");
+    if (!st) {
+        launch_fill_u64(in, 1ull, n, s);
+        launch_probe(kind, in, n, count, out, s);  // warm-up
+        (void)hipEventRecord(e0, s);
+        for (int32_t r = 0; r < reps; ++r) launch_probe(kind, in, n, count, out, s);
+        (void)hipEventRecord(e1, s);
+        if (hipStreamSynchronize(s) != hipSuccess)
+            st = fail(DSSSP_ERR_HIP, "probe failed: %s", hipGetErrorString(hipGetLastError()));
+        else
+            *ms = elapsed(e0, e1) / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    dev_free(in);
+    dev_free(out);
+    return st;
+}
+
 extern "C" int dsssp_batch_upload(dsssp_graph *g, const dsssp_change *chg, int64_t k, dsssp_batch **out)
 {
     clear_error();
@@ -1557,7 +1602,7 @@ extern "C" int dsssp_update_batch(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b,
         if (stats) *stats = dsssp_stats{};
         return DSSSP_OK;
     }
-    return update_impl(g, t, b, stats, 0.0);
+    return update_impl(g, t, b, stats, false);
 }
 
 // The graph's reusable device batch, with room for k records.
@@ -1596,17 +1641,17 @@ static int ensure_stage(dsssp_graph *g, int64_t k)
     return DSSSP_OK;
 }
 
-// Applies the k records whose upload to the device batch is enqueued.
+// Applies the k records whose upload to the device batch is enqueued between
+// ev[8] and ev[9].
 static int apply_uploaded(dsssp_graph *g, dsssp_tree *t, int64_t k, int64_t ni, int64_t nd, bool zw,
-                          dsssp_stats *stats, std::chrono::steady_clock::time_point t0)
+                          dsssp_stats *stats)
 {
     dsssp_batch *b = g->upd;
     b->k = k;
     b->n_ins = ni;
     b->n_del = nd;
     b->zero_w = zw;
-    const double h2d = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    return update_impl(g, t, b, stats, h2d);
+    return update_impl(g, t, b, stats, true);
 }
 
 extern "C" int dsssp_update(dsssp_graph *g, dsssp_tree *t, const dsssp_change *chg, int64_t k, dsssp_stats *stats)
@@ -1614,7 +1659,6 @@ extern "C" int dsssp_update(dsssp_graph *g, dsssp_tree *t, const dsssp_change *c
     clear_error();
     if (!g || !t || k < 0 || (k > 0 && !chg)) return fail(DSSSP_ERR_ARG, "bad argument");
     if (t->g != g) return fail(DSSSP_ERR_ARG, "tree belongs to another graph");
-    const auto t0 = std::chrono::steady_clock::now();
     int64_t ni, nd;
     bool zw;
     if (int s = validate_changes(g->n, chg, k, ni, nd, zw)) return s;
@@ -1626,8 +1670,10 @@ extern "C" int dsssp_update(dsssp_graph *g, dsssp_tree *t, const dsssp_change *c
     if (int s = ensure_batch(g, k)) return s;
     if (int s = ensure_stage(g, k)) return s;
     memcpy(g->stage, chg, sizeof(dsssp_change) * (size_t)k);
+    HIP_TRY(hipEventRecord(g->ev[8], g->stream));
     HIP_TRY(hipMemcpyAsync(g->upd->rec, g->stage, sizeof(int4) * (size_t)k, hipMemcpyHostToDevice, g->stream));
-    return apply_uploaded(g, t, k, ni, nd, zw, stats, t0);
+    HIP_TRY(hipEventRecord(g->ev[9], g->stream));
+    return apply_uploaded(g, t, k, ni, nd, zw, stats);
 }
 
 namespace dsssp {
@@ -1654,7 +1700,6 @@ int update_pinned(dsssp_graph *g, dsssp_tree *t, const dsssp_change *const *buf,
     clear_error();
     if (!g || !t || nbuf < 0 || (nbuf > 0 && (!buf || !len))) return fail(DSSSP_ERR_ARG, "bad argument");
     if (t->g != g) return fail(DSSSP_ERR_ARG, "tree belongs to another graph");
-    const auto t0 = std::chrono::steady_clock::now();
     int64_t k = 0;
     for (int32_t i = 0; i < nbuf; ++i) k += len[i];
     if (k == 0) {
@@ -1665,13 +1710,15 @@ int update_pinned(dsssp_graph *g, dsssp_tree *t, const dsssp_change *const *buf,
     if (int s = ensure_batch(g, k)) return s;
     dsssp_batch *b = g->upd;
     int64_t pos = 0;
+    HIP_TRY(hipEventRecord(g->ev[8], g->stream));
     for (int32_t i = 0; i < nbuf; ++i) {
         if (len[i])
             HIP_TRY(hipMemcpyAsync(b->rec + pos, buf[i], sizeof(int4) * (size_t)len[i], hipMemcpyHostToDevice,
                                    g->stream));
         pos += len[i];
     }
-    return apply_uploaded(g, t, k, ni, nd, zw, stats, t0);
+    HIP_TRY(hipEventRecord(g->ev[9], g->stream));
+    return apply_uploaded(g, t, k, ni, nd, zw, stats);
 }
 
 }  // namespace dsssp

@@ -90,6 +90,12 @@ void launch_frontier(int mode, const FArgs &a, int grid, hipStream_t s, hipEvent
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 void launch_reduce_stats(Counters *c, unsigned long long *bstats, hipStream_t s);
+// Probe launches of known footprint: kind 0 copies in[0, n) to out; kind 1 reads
+// count words at hashed positions of in[0, n) (8192 x 256 threads, out holds one
+// word per thread).
+void launch_probe(int kind, const uint64_t *in, uint64_t n, uint64_t count, uint64_t *out, hipStream_t s);
+// dst[i] |= src[i] and src[i] = 0 for i < nwords; *count = set bits src held.
+void launch_merge_count(uint64_t *dst, uint64_t *src, uint64_t nwords, unsigned long long *count, hipStream_t s);
 // out[v] = st[v << kshift | tree] for v < n (one tree of a forest, contiguous).
 void launch_gather_tree(const uint64_t *st, uint64_t n, uint32_t kshift, uint32_t tree, uint64_t *out,
                         hipStream_t s);

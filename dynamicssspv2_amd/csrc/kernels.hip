@@ -1303,6 +1303,46 @@ __global__ __launch_bounds__(RB) void k_fill_u32(uint32_t *p, uint32_t v, uint64
     for (uint64_t i = (uint64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * RB) p[i] = v;
 }
 
+// Probe kernels of known footprint (counter calibration).  k_probe_copy: out =
+// in over n words.  k_probe_gather: count 8-byte reads at hashed positions of
+// in[0, n), summed per lane and written once per lane (so no read is elided).
+__global__ __launch_bounds__(RB) void k_probe_copy(const uint64_t *in, uint64_t *out, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * RB) out[i] = in[i];
+}
+
+__global__ __launch_bounds__(RB) void k_probe_gather(const uint64_t *in, uint64_t n, uint64_t count, uint64_t *out)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * RB + threadIdx.x, nt = (uint64_t)gridDim.x * RB;
+    uint64_t acc = 0;
+    for (uint64_t i = t; i < count; i += nt) {
+        uint64_t h = (i + 1) * 0x9E3779B97F4A7C15ull;
+        h ^= h >> 31;
+        h *= 0xBF58476D1CE4E5B9ull;
+        h ^= h >> 29;
+        acc += in[h % n];
+    }
+    out[t] = acc;
+}
+
+// dst |= src word by word, src cleared, *count = the bits src held (one
+// atomicAdd per wave).
+__global__ __launch_bounds__(RB) void k_merge_count(uint64_t *dst, uint64_t *src, uint64_t nwords,
+                                                    unsigned long long *count)
+{
+    uint32_t c = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * RB + threadIdx.x; i < nwords; i += (uint64_t)gridDim.x * RB) {
+        const uint64_t v = src[i];
+        if (v) {
+            dst[i] |= v;
+            src[i] = 0;
+            c += (uint32_t)__popcll(v);
+        }
+    }
+    c = wave_sum(c);
+    if (lane_id() == 0 && c) atomicAdd(count, (unsigned long long)c);
+}
+
 // out[v] = st[v << kshift | tree]: one tree's states, contiguous.
 __global__ __launch_bounds__(RB) void k_gather_tree(const uint64_t *st, uint64_t n, uint32_t kshift, uint32_t tree,
                                                     uint64_t *out)
@@ -1428,6 +1468,22 @@ void launch_digest(const uint64_t *p, uint64_t n, unsigned long long *out, hipSt
     if (n == 0) return;
     const unsigned g = (unsigned)std::min<uint64_t>((n + RB - 1) / RB, 4096);
     hipLaunchKernelGGL(k_digest, dim3(g), dim3(RB), 0, s, p, n, out);
+}
+
+void launch_probe(int kind, const uint64_t *in, uint64_t n, uint64_t count, uint64_t *out, hipStream_t s)
+{
+    if (kind == 0)
+        hipLaunchKernelGGL(k_probe_copy, dim3(8192), dim3(RB), 0, s, in, out, n);
+    else
+        hipLaunchKernelGGL(k_probe_gather, dim3(8192), dim3(RB), 0, s, in, n, count, out);
+}
+
+void launch_merge_count(uint64_t *dst, uint64_t *src, uint64_t nwords, unsigned long long *count, hipStream_t s)
+{
+    launch_fill_u64((uint64_t *)count, 0ull, 1, s);
+    if (nwords == 0) return;
+    const unsigned g = (unsigned)std::min<uint64_t>((nwords + RB - 1) / RB, 4096);
+    hipLaunchKernelGGL(k_merge_count, dim3(g), dim3(RB), 0, s, dst, src, nwords, count);
 }
 
 void launch_gather_tree(const uint64_t *st, uint64_t n, uint32_t kshift, uint32_t tree, uint64_t *out,

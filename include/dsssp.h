@@ -71,10 +71,12 @@ typedef struct dsssp_graph dsssp_graph;  /* device-resident mutable CSR         
 typedef struct dsssp_tree dsssp_tree;    /* device-resident SSSP tree of a graph  */
 typedef struct dsssp_batch dsssp_batch;  /* change batch resident in HBM          */
 
-/* Per-update statistics.  Times are device time measured with HIP events on
- * the graph's stream.  t_update_ms spans "batch resident in HBM" to
- * "converged": insertion into the resident CSR, deletion classification and
- * tombstoning, subtree invalidation, seeding and relaxation. */
+/* Per-update statistics.  Times are measured with HIP events on the graph's
+ * stream.  t_update_ms spans "batch resident in HBM" to "converged": upkeep of
+ * the resident graph, insertion into the resident CSR, deletion classification
+ * and tombstoning, subtree invalidation, seeding and relaxation.  t_h2d_ms is
+ * the host-to-device copy of a batch given as host records (dsssp_update),
+ * outside t_update_ms. */
 typedef struct {
     int64_t n_insert, n_delete;     /* records with op 1 / op 0                */
     int64_t n_invalidated;          /* vertices reset by subtree invalidation  */
@@ -96,6 +98,9 @@ typedef struct {
     int64_t relax_sweeps;           /* relaxation rounds run as full sweeps    */
     double t_sweep_ms;              /* dsssp_tree_relax_full: its edge pass    */
     double alg_bytes_sweep;         /* and that pass's algorithmic HBM bytes   */
+    double t_upkeep_ms;             /* part of t_update_ms: room in the delta
+                                       pool, edge map load and item lists made
+                                       before the batch's first kernel         */
 } dsssp_stats;
 
 /* Tuning knobs (zero fields keep the current value). */
@@ -128,6 +133,9 @@ typedef struct {
                                  this share relaxes every slot, rows read in vertex
                                  order, instead of the marked ones.  -1: never.
                                  Only without distance ordering and below 64 trees */
+    int32_t give_up_ms;       /* a loop of rounds still running after this long is
+                                 abandoned: the call scrubs the tree and returns
+                                 DSSSP_ERR_STATE (default 120000)                     */
 } dsssp_tuning;
 
 #define DSSSP_DELTA_AUTO (-2)
@@ -141,6 +149,12 @@ int dsssp_get_tuning(dsssp_tuning *t);
 int dsssp_set_tuning(const dsssp_tuning *t);
 /* Frees memory the library returned (file readers, generators). */
 void dsssp_free(void *p);
+/* Launches of known HBM footprint, for calibrating hardware byte counters
+ * against the update's access patterns (no reference counterpart): kind 0
+ * copies a buffer of `bytes` (reads and writes `bytes`); kind 1 reads `count`
+ * 8-byte words at hashed positions of a buffer of `bytes`.  Each is launched
+ * `reps` times after one warm-up; *ms = device time per launch. */
+int dsssp_probe(int32_t kind, uint64_t bytes, uint64_t count, int32_t reps, double *ms);
 
 /* ---- text formats (host only) --------------------------------------------
  * Replace read_graphEdges / readin_changes / read_SSSP
