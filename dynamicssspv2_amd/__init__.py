@@ -148,6 +148,7 @@ def lib():
         "dsssp_batch_upload": ([p, p, i64, pp], ctypes.c_int),
         "dsssp_batch_destroy": ([p], ctypes.c_int),
         "dsssp_update_batch": ([p, p, p, ctypes.POINTER(Stats)], ctypes.c_int),
+        "dsssp_update_noalloc": ([p, p, p, i64, ctypes.POINTER(Stats)], ctypes.c_int),
         "dsssp_autotune": ([p, p, pp, i32, i32, ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_double)],
                            ctypes.c_int),
         "dsssp_server_create": ([ctypes.POINTER(ServerConfig), pp], ctypes.c_int),
@@ -502,6 +503,14 @@ def update(graph: Graph, tree: Tree, changes) -> dict:
     else:
         rec = as_changes(changes)
         _check(lib().dsssp_update(graph._h, tree._h, _ptr(rec), len(rec), ctypes.byref(st)))
+    return st.as_dict()
+
+
+def update_noalloc(graph: Graph, tree: Tree, device_records: int, k: int) -> dict:
+    """Allocation-free update (dsssp_update_noalloc): device_records is the
+    device address of k records in the dsssp_change layout (int32 a, b, w, op)."""
+    st = Stats()
+    _check(lib().dsssp_update_noalloc(graph._h, tree._h, ctypes.c_void_p(device_records), k, ctypes.byref(st)))
     return st.as_dict()
 
 

@@ -385,7 +385,8 @@ int captured_rounds(dsssp_tree *t, int mode, int64_t first, int group, int grid,
 // Rounds run back to back on the stream (replayed from captured hipGraphs when
 // use_graph is set); the host reads one flag pair per group of rounds.  A round
 // after convergence reads an empty bitmap and is a no-op.
-int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms, bool full_first = false)
+int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms, bool full_first = false,
+               bool no_alloc = false)
 {
     dsssp_graph *g = t->g;
     hipStream_t s = g->stream;
@@ -399,7 +400,9 @@ int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms, bool
     const int first = (int)std::max<int64_t>(rb, std::min<int64_t>(t->last_rounds[mode], max_group));
     int group = first;
     const int grid = frontier_grid(t->nw);
-    const bool timed = tu.kernel_timing && kernel_ms;
+    // (no per-launch events or captured graphs for the allocation-free update)
+    const bool timed = tu.kernel_timing && kernel_ms && !no_alloc;
+    const bool graphs = tu.use_graph && !no_alloc;
     FArgs base = base_args(t);
     if (mode == M_RELAX) base.delta = relax_delta(t);
     // Push-vs-full-sweep (not with distance ordering, whose rounds visit only
@@ -434,7 +437,7 @@ int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms, bool
             }
             for (int j = 0; j < group; ++j)
                 launch_frontier(mode, args_of(i + j), grid, s, g->kev[2 * j], g->kev[2 * j + 1]);
-        } else if (tu.use_graph && !(full_first && i == 0)) {
+        } else if (graphs && !(full_first && i == 0)) {
             hipGraphExec_t exec = nullptr;
             if (int e = captured_rounds(t, mode, i, group, grid, base, &exec)) return e;
             HIP_TRY(hipGraphLaunch(exec, s));
@@ -483,7 +486,7 @@ int ensure_pick_buffers(dsssp_tree *t)
 // Recomputes every parent: smallest-id tight neighbour (use_lvl = 0), or the
 // smallest-id tight neighbour one BFS level closer to a root in the tight graph
 // (use_lvl = 1), which stays acyclic with zero weights.
-int repick_parents(dsssp_tree *t, bool use_lvl, int64_t &rounds)
+int repick_parents(dsssp_tree *t, bool use_lvl, int64_t &rounds, bool no_alloc = false)
 {
     dsssp_graph *g = t->g;
     if (int s = ensure_pick_buffers(t)) return s;
@@ -491,7 +494,7 @@ int repick_parents(dsssp_tree *t, bool use_lvl, int64_t &rounds)
     if (use_lvl) {
         zero_ring(t);
         launch_lvl_init(ns, t->kshift, t->st, t->lvl, t->best, t->fb[0], &t->c->act[0], g->stream);
-        if (int s = run_rounds(t, M_TBFS, rounds, nullptr)) return s;
+        if (int s = run_rounds(t, M_TBFS, rounds, nullptr, false, no_alloc)) return s;
     } else {
         launch_fill_u32(t->best, 0xFFFFFFFFu, ns, g->stream);
     }
@@ -708,18 +711,45 @@ void tombstone(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b)
     launch_tomb(b->rec, b->k, g->view(), t->bstats, g->stream);
 }
 
-int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stats, bool h2d_timed)
+// Whether the update of a batch with n_ins insertions fits the graph's and
+// tree's current buffers (the allocation-free update refuses otherwise).
+int fits_without_upkeep(dsssp_graph *g, dsssp_tree *t, uint64_t n_ins, bool zero_w)
+{
+    if (n_ins > 0 && 3 * g->pool_top + 10 * n_ins > g->pool_cap)
+        return fail(DSSSP_ERR_STATE, "allocation-free update: the delta pool would have to grow for %llu insertions "
+                                     "(apply one update with dsssp_update_batch first)", (unsigned long long)n_ins);
+    if (n_ins > 0 && (g->emap_used + 2 * n_ins) * 10 > g->emap_slots * 7)
+        return fail(DSSSP_ERR_STATE, "allocation-free update: the edge map would have to be rebuilt for %llu "
+                                     "insertions", (unsigned long long)n_ins);
+    const dsssp_tuning tu = tuning();
+    const uint64_t total = g->E + g->delta_entries + 2 * n_ins;
+    const uint64_t per_tree = total / (uint64_t)std::max(64, tu.chunk) + total / (uint64_t)std::max(64, tu.heavy_degree) +
+                              std::min<uint64_t>((uint64_t)g->n, 64 * total / (uint64_t)std::max(64, tu.tile_entries)) + 1;
+    if ((per_tree << t->kshift) + 1024 > t->hcap)
+        return fail(DSSSP_ERR_STATE, "allocation-free update: the tree's item lists would have to grow");
+    if ((g->zero_w || zero_w) && (!t->lvl || !t->best))
+        return fail(DSSSP_ERR_STATE, "allocation-free update: zero weights need the parent-repair buffers "
+                                     "(apply one update with dsssp_update_batch first)");
+    return DSSSP_OK;
+}
+
+int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stats, bool h2d_timed,
+                bool no_alloc = false)
 {
     DevGuard dg(g->device);
     hipStream_t s = g->stream;
     // Upkeep of the resident graph (pool room, edge map load, item lists) is
     // part of the update's time: ev[7] opens the window.
     HIP_TRY(hipEventRecord(g->ev[7], s));
-    if (b->n_ins > 0) {
-        if (int e = ensure_pool(g, (uint64_t)b->n_ins)) return e;
-        if (int e = ensure_emap(g, 2 * (uint64_t)b->n_ins)) return e;
+    if (no_alloc) {
+        if (int e = fits_without_upkeep(g, t, (uint64_t)b->n_ins, b->zero_w)) return e;
+    } else {
+        if (b->n_ins > 0) {
+            if (int e = ensure_pool(g, (uint64_t)b->n_ins)) return e;
+            if (int e = ensure_emap(g, 2 * (uint64_t)b->n_ins)) return e;
+        }
+        if (int e = ensure_hq(t, g->E + g->delta_entries + 2 * (uint64_t)b->n_ins)) return e;
     }
-    if (int e = ensure_hq(t, g->E + g->delta_entries + 2 * (uint64_t)b->n_ins)) return e;
     int64_t inv_rounds = 0, relax_rounds = 0, repair_rounds = 0;
     double relax_kernel_ms = 0;
 
@@ -745,7 +775,7 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
         tombstone(g, t, b);
     }
     HIP_TRY(hipEventRecord(g->ev[2], s));
-    if (b->n_del > 0) code = run_rounds(t, M_INVAL, inv_rounds, nullptr);
+    if (b->n_del > 0) code = run_rounds(t, M_INVAL, inv_rounds, nullptr, false, no_alloc);
     HIP_TRY(hipEventRecord(g->ev[3], s));
     // 4. seed invalidated slots from valid neighbours, offer inserted edges
     if (!code) {
@@ -762,10 +792,10 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
     }
     HIP_TRY(hipEventRecord(g->ev[4], s));
     // 5. relaxation
-    if (!code) code = run_rounds(t, M_RELAX, relax_rounds, &relax_kernel_ms);
+    if (!code) code = run_rounds(t, M_RELAX, relax_rounds, &relax_kernel_ms, false, no_alloc);
     HIP_TRY(hipEventRecord(g->ev[5], s));
     // 6. zero-weight parent repair
-    if (!code && g->zero_w) code = repick_parents(t, true, repair_rounds);
+    if (!code && g->zero_w) code = repick_parents(t, true, repair_rounds, no_alloc);
     HIP_TRY(hipEventRecord(g->ev[6], s));
     launch_reduce_stats(t->c, t->bstats, s);
     HIP_TRY(hipMemcpyAsync(t->c_host, t->c, sizeof(Counters), hipMemcpyDeviceToHost, s));
@@ -857,10 +887,13 @@ int validate_changes(int64_t n, const dsssp_change *chg, int64_t k, int64_t &ni,
 extern "C" const char *dsssp_last_error(void) { return last_error(); }
 extern "C" const char *dsssp_version(void) { return "dsssp-mi355x 0.2 (gfx950)"; }
 
+static int scratch_of_device(int dev);
+
 extern "C" int dsssp_set_device(int device)
 {
     clear_error();
     if (int s = check_device(device)) return s;
+    if (int s = scratch_of_device(device)) return s;  // the allocation-free update's scratch address
     t_device = device;
     return DSSSP_OK;
 }
@@ -1674,6 +1707,98 @@ extern "C" int dsssp_update(dsssp_graph *g, dsssp_tree *t, const dsssp_change *c
     HIP_TRY(hipMemcpyAsync(g->upd->rec, g->stage, sizeof(int4) * (size_t)k, hipMemcpyHostToDevice, g->stream));
     HIP_TRY(hipEventRecord(g->ev[9], g->stream));
     return apply_uploaded(g, t, k, ni, nd, zw, stats);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free update.  Its only device memory beyond the caller's records
+// and the graph's and tree's own buffers is `scratch`: per record copy, the
+// position it took in its row's delta block, then four validation counters.
+// Each device holds its own copy for the life of the process; every call on a
+// device shares it and leaves the last call's contents, so calls that overlap
+// on one device race.  Host code uses the per-device address obtained from
+// hipGetSymbolAddress, never the array's name.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr int64_t NOALLOC_MAX_RECORDS = 1 << 21;
+constexpr size_t NOALLOC_SLOT_BYTES = 2 * (size_t)NOALLOC_MAX_RECORDS * sizeof(uint32_t);
+constexpr size_t NOALLOC_SCRATCH_BYTES = NOALLOC_SLOT_BYTES + 256;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+void *g_scratch_at[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on device dev (looked up once per device,
+// with that device current).
+int scratch_of(int dev, unsigned char **out)
+{
+    *out = nullptr;
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(DSSSP_ERR_NODEV, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_at[dev]) {
+        DevGuard dg(dev);
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(DSSSP_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev,
+                        hipGetErrorString(e));
+        g_scratch_at[dev] = p;
+    }
+    *out = (unsigned char *)g_scratch_at[dev];
+    return DSSSP_OK;
+}
+
+}  // namespace
+
+static int scratch_of_device(int dev)
+{
+    unsigned char *p = nullptr;
+    return scratch_of(dev, &p);
+}
+
+extern "C" int dsssp_update_noalloc(dsssp_graph *g, dsssp_tree *t, const dsssp_change *d_chg, int64_t k,
+                                    dsssp_stats *stats)
+{
+    clear_error();
+    if (!g || !t || k < 0 || (k > 0 && !d_chg)) return fail(DSSSP_ERR_ARG, "bad argument");
+    if (t->g != g) return fail(DSSSP_ERR_ARG, "tree belongs to another graph");
+    if (k > NOALLOC_MAX_RECORDS)
+        return fail(DSSSP_ERR_ARG, "allocation-free update: %lld records exceed its maximum of %lld", (long long)k,
+                    (long long)NOALLOC_MAX_RECORDS);
+    if (k == 0) {
+        if (stats) *stats = dsssp_stats{};
+        return DSSSP_OK;
+    }
+    unsigned char *sc = nullptr;
+    if (int s = scratch_of(g->device, &sc)) return s;
+    DevGuard dg(g->device);
+    hipStream_t s = g->stream;
+    unsigned long long *cnt = (unsigned long long *)(sc + NOALLOC_SLOT_BYTES);
+    launch_validate((const int4 *)d_chg, k, g->n, cnt, s);
+    unsigned long long h[4] = {0, 0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (h[3] != ~0ull) {
+        dsssp_change c{};
+        HIP_TRY(hipMemcpyAsync(&c, d_chg + h[3], sizeof c, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        return fail(c.w < 0 ? DSSSP_ERR_WEIGHT : DSSSP_ERR_RANGE,
+                    "change %llu (%d %d %d %d): vertex outside [0,%lld) or negative weight", h[3], c.a, c.b, c.w,
+                    c.op, (long long)g->n);
+    }
+    dsssp_batch b;
+    b.g = g;
+    b.k = k;
+    b.n_ins = (int64_t)h[0];
+    b.n_del = (int64_t)h[1];
+    b.zero_w = h[2] != 0;
+    b.rec = (int4 *)d_chg;
+    b.tslot = (uint32_t *)sc;
+    b.cap = k;
+    return update_impl(g, t, &b, stats, false, true);
 }
 
 namespace dsssp {

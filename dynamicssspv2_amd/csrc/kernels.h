@@ -90,6 +90,9 @@ void launch_frontier(int mode, const FArgs &a, int grid, hipStream_t s, hipEvent
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 void launch_reduce_stats(Counters *c, unsigned long long *bstats, hipStream_t s);
+// Checks k records in device memory: cnt[0..2] = insertions, deletions,
+// zero-weight insertions; cnt[3] = smallest index of an invalid record (~0: none).
+void launch_validate(const int4 *rec, int64_t k, int64_t n, unsigned long long *cnt, hipStream_t s);
 // Probe launches of known footprint: kind 0 copies in[0, n) to out; kind 1 reads
 // count words at hashed positions of in[0, n) (8192 x 256 threads, out holds one
 // word per thread).

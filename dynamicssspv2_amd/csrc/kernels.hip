@@ -1303,6 +1303,36 @@ __global__ __launch_bounds__(RB) void k_fill_u32(uint32_t *p, uint32_t v, uint64
     for (uint64_t i = (uint64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * RB) p[i] = v;
 }
 
+// Device-side check of k change records (the host check of dsssp_update,
+// for records already in device memory): cnt[0] insertions, cnt[1] deletions,
+// cnt[2] zero-weight insertions, cnt[3] the smallest index of a record with
+// op 0/1 and a vertex outside [0, n) or a negative weight (~0: none).  Records
+// with another op are ignored.
+__global__ __launch_bounds__(RB) void k_validate(const int4 *rec, int64_t k, int64_t n, unsigned long long *cnt)
+{
+    const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
+    uint32_t ins = 0, del = 0, zw = 0;
+    if (i < k) {
+        const int4 r = rec[i];
+        if (r.w == 0 || r.w == 1) {
+            if (r.x < 0 || r.x >= n || r.y < 0 || r.y >= n || r.z < 0)
+                atomicMin(&cnt[3], (unsigned long long)i);
+            else if (r.w == 1)
+                ins = 1, zw = r.z == 0;
+            else
+                del = 1;
+        }
+    }
+    ins = wave_sum(ins);
+    del = wave_sum(del);
+    zw = wave_sum(zw);
+    if (lane_id() == 0) {
+        if (ins) atomicAdd(&cnt[0], (unsigned long long)ins);
+        if (del) atomicAdd(&cnt[1], (unsigned long long)del);
+        if (zw) atomicAdd(&cnt[2], (unsigned long long)zw);
+    }
+}
+
 // Probe kernels of known footprint (counter calibration).  k_probe_copy: out =
 // in over n words.  k_probe_gather: count 8-byte reads at hashed positions of
 // in[0, n), summed per lane and written once per lane (so no read is elided).
@@ -1468,6 +1498,13 @@ void launch_digest(const uint64_t *p, uint64_t n, unsigned long long *out, hipSt
     if (n == 0) return;
     const unsigned g = (unsigned)std::min<uint64_t>((n + RB - 1) / RB, 4096);
     hipLaunchKernelGGL(k_digest, dim3(g), dim3(RB), 0, s, p, n, out);
+}
+
+void launch_validate(const int4 *rec, int64_t k, int64_t n, unsigned long long *cnt, hipStream_t s)
+{
+    launch_fill_u64((uint64_t *)cnt, 0ull, 3, s);
+    launch_fill_u64((uint64_t *)cnt + 3, ~0ull, 1, s);
+    if (k > 0) hipLaunchKernelGGL(k_validate, dim3(blocks_for(k)), dim3(RB), 0, s, rec, k, n, cnt);
 }
 
 void launch_probe(int kind, const uint64_t *in, uint64_t n, uint64_t count, uint64_t *out, hipStream_t s)

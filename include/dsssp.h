@@ -284,6 +284,21 @@ int dsssp_update(dsssp_graph *g, dsssp_tree *t, const dsssp_change *chg, int64_t
 int dsssp_batch_upload(dsssp_graph *g, const dsssp_change *chg, int64_t k, dsssp_batch **out);
 int dsssp_batch_destroy(dsssp_batch *b);
 int dsssp_update_batch(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stats);
+/* Allocation-free variant of dsssp_update_batch.  No reference counterpart:
+ * it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  d_chg points to k records
+ * (dsssp_change layout) already in device memory of the graph's device; they
+ * are validated on the device.  The call allocates nothing: its scratch (the
+ * delta-block position of each record copy, validation counters) is a static
+ * device array of 16 MiB + 256 B per device, shared by every call on that
+ * device -- calls that overlap on one device race.  Limits: k <= 2097152
+ * records (DSSSP_ERR_ARG beyond); the batch must fit the graph's delta pool,
+ * edge map and the tree's item lists as they are (DSSSP_ERR_STATE when they
+ * would have to grow, or when a graph with zero weights has not yet allocated
+ * its parent-repair buffers: apply one update with dsssp_update_batch first).
+ * No per-launch timing events or captured round graphs are used. */
+int dsssp_update_noalloc(dsssp_graph *g, dsssp_tree *t, const dsssp_change *d_chg, int64_t k,
+                         dsssp_stats *stats);
 /* Autotuner (no reference counterpart; the reference has fixed launch
  * shapes, SSSPmainForUndirected.cu:200-203,229-236,271-277): coordinate descent
  * over grid, edges_per_thread, heavy_degree, tile_entries, local_tiles,
