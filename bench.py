@@ -55,8 +55,10 @@ def parse():
     p.add_argument("--trees", type=int, default=64)
     p.add_argument("--graphs", type=int, default=32)
     p.add_argument("--instances", type=int, default=16)
-    p.add_argument("--requests", type=int, default=400)
-    p.add_argument("--rate", type=float, default=0.0, help="served: arrivals per second (0: all at once)")
+    p.add_argument("--requests", type=int, default=400, help="served: requests per run (Poisson, then saturated)")
+    p.add_argument("--rate", type=float, default=5000.0,
+                   help="served: Poisson arrivals per second of the first half of the requests (below capacity)")
+    p.add_argument("--serve-scale", type=int, default=0, help="served: every instance at this R-MAT scale")
     p.add_argument("--lanes", type=int, default=0, help="served: concurrent lanes (0: the library's default)")
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU Dijkstra baseline / exactness check")
@@ -66,8 +68,8 @@ def parse():
     p.add_argument("--probe", action="store_true",
                    help="only run the library's probe launches of known footprint (a 1 GiB copy, 8-byte gathers "
                         "over 8 GiB and over 64 MiB) for calibrating HBM byte counters, then exit")
-    for knob in ("grid", "heavy_degree", "chunk", "round_batch", "use_graph", "edges_per_thread", "tile_entries",
-                 "local_tiles", "delta", "sweep"):
+    for knob in ("block", "grid", "heavy_degree", "chunk", "round_batch", "use_graph", "edges_per_thread",
+                 "tile_entries", "local_tiles", "delta", "sweep"):
         p.add_argument("--" + knob.replace("_", "-"), type=int, default=None, help="tuning knob (dsssp_tuning)")
     return p.parse_args()
 
@@ -394,31 +396,42 @@ def config4a(args, ds, np):
     g = ds.Graph(n, a, b, w)
     f = ds.Tree(g, sources=srcs)
     res = [ds.Batch(g, c) for c in batches]
+    tuned = None
+    if args.autotune and args.warmup > 0:
+        t0 = time.perf_counter()
+        tuned, tms = ds.autotune(g, f, res[:args.warmup])
+        log(f"forest: autotuned in {time.perf_counter() - t0:.1f}s: {tuned}, {tms:.2f} ms per warm-up batch")
     stats, wall = timed_steps(ds, g, f, res, args.warmup, not args.no_timing)
     for r in res:
         r.close()
-    # plain comparison: the same batches applied to one tree at a time (first 4 sources, extrapolated)
-    plain_ms = []
-    for j in range(min(4, K)):
-        gp = ds.Graph(n, a, b, w)
+    # plain comparison: every tree alone (one tree per graph, a device copy of
+    # the initial graph each), the same batches, device time per update
+    fr = [f.read(j) for j in range(K)]
+    g0 = ds.Graph(n, a, b, w)
+    plain_ms = 0.0
+    same = True
+    for j in range(K):
+        gp = g0.clone()
         tp = ds.Tree(gp, int(srcs[j]))
         for i, c in enumerate(batches):
             st = ds.update(gp, tp, c)
             if i >= args.warmup:
-                plain_ms.append(st["t_update_ms"])
-        if j == 0:
-            pp, pd = tp.read()
-            fp, fd = f.read(0)
-            same = bool(np.array_equal(pp, fp) and np.array_equal(pd, fd))
+                plain_ms += st["t_update_ms"]
+        pp, pd = tp.read()
+        same &= bool(np.array_equal(pp, fr[j][0]) and np.array_equal(pd, fr[j][1]))
+        tp.close()
         gp.close()
-    plain_per_tree = sum(plain_ms) / len(plain_ms)
+    g0.close()
+    plain_per_tree = plain_ms / (K * args.steps)
     tree_updates = k * K * args.steps
     value = tree_updates / wall
     log(f"forest K={K} s{scale}: {value:.3e} tree-edge updates/s; plain {plain_per_tree:.2f} ms/tree/batch")
     line(args, value, wall, args.steps,
          {"workload": f"rmat-s{scale} K={K} trees, shared batch k={k} ins=0.5", "n": n, "m": len(a), "k": k,
-          "trees": K}, stats, {"forest_equals_plain_tree0": same}, None, unit="changed edges x trees /s",
-         extra={"speedup_vs_plain": (plain_per_tree * K * args.steps / 1e3) / wall})
+          "trees": K}, stats, {"every_tree_equals_plain": same}, None, unit="changed edges x trees /s",
+         extra={"speedup_vs_plain": (plain_ms / 1e3) / wall, "plain_ms_per_step": plain_ms / args.steps,
+                "autotuned": tuned,
+                "use_graph": ds.get_tuning()["use_graph"]})
 
 
 def config4b(args, ds, np):
@@ -442,6 +455,13 @@ def config4b(args, ds, np):
     t0 = ds.Tree(g0, roots=roots)
     g = g0.clone()
     t = ds.Tree(g, roots=roots)  # a tree on the clone; its state is restored per step
+    tuned = None
+    if args.autotune:
+        bt = ds.Batch(g, C)
+        tq = time.perf_counter()
+        tuned, tms = ds.autotune(g, t, [bt])
+        bt.close()
+        log(f"32 graphs: autotuned in {time.perf_counter() - tq:.1f}s: {tuned}, {tms:.2f} ms per batch")
     stats = []
     for i in range(args.warmup + args.steps):
         g.copy_from(g0)
@@ -474,14 +494,64 @@ def config4b(args, ds, np):
     line(args, value, dev, args.steps, {"workload": f"{args.graphs} random graphs n~logU[2^14,2^21] 1% batches",
                                           "N": N, "M": len(A), "k_total": len(C)}, stats,
          {"each_equals_plain": same}, None,
-         extra={"speedup_vs_plain": plain / (dev / args.steps * 1e3)})
+         extra={"speedup_vs_plain": plain / (dev / args.steps * 1e3), "autotuned": tuned,
+                "use_graph": ds.get_tuning()["use_graph"]})
+
+
+def served_requests(ds, np, rng, insts, count):
+    """count requests: k log-uniform in [1, 1e5], insertion ratio U[0, 1],
+    uniform target instance.  Per instance, one generator call gives distinct
+    deletions of its initial edges and distinct absent pairs to insert, split
+    over its requests in order."""
+    plan = []
+    for _ in range(count):
+        i = int(rng.integers(0, len(insts)))
+        k = int(math.exp(rng.uniform(0, math.log(100_000))))
+        ki = int(round(k * float(rng.uniform(0, 1))))
+        plan.append((i, k - ki, ki))
+    reqs = [None] * count
+    for i, (n, a, b, w, _) in enumerate(insts):
+        mine = [j for j, p in enumerate(plan) if p[0] == i]
+        kd, ki = sum(plan[j][1] for j in mine), sum(plan[j][2] for j in mine)
+        if kd + ki == 0:
+            continue
+        allc = ds.gen_changes(n, a, b, w, kd + ki, ki / (kd + ki), 1, 1000, seed=9000 + i)
+        dels, ins = allc[allc[:, 3] == 0], allc[allc[:, 3] == 1]
+        od = oi = 0
+        for j in mine:
+            c = np.concatenate([dels[od:od + plan[j][1]], ins[oi:oi + plan[j][2]]])
+            od += plan[j][1]
+            oi += plan[j][2]
+            rng.shuffle(c)
+            reqs[j] = (i, np.ascontiguousarray(c))
+    return [r for r in reqs if r is not None and len(r[1])]
+
+
+def serve_run(srv, np, reqs, rate, rng):
+    """Submits reqs (Poisson arrivals at `rate` per second, or all at once when
+    rate is 0) and waits for all; returns (wall seconds, request stats)."""
+    gaps = rng.exponential(1.0 / rate, len(reqs)) if rate > 0 else np.zeros(len(reqs))
+    tickets = []
+    t0 = time.perf_counter()
+    tnext = t0
+    for (i, c), gap in zip(reqs, gaps):
+        tnext += gap
+        d = tnext - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        tickets.append(srv.submit(i, c))
+    rs = [srv.wait(tk) for tk in tickets]
+    return time.perf_counter() - t0, rs
 
 
 def config5(args, ds, np):
     rng = np.random.default_rng(args.seed)
-    scales = [20 + (i * 5) // args.instances for i in range(args.instances)]  # 20..24 spread evenly
-    if args.scale < 24:
-        scales = [min(s, args.scale) for s in scales]
+    if args.serve_scale:
+        scales = [args.serve_scale] * args.instances
+    else:
+        scales = [20 + (i * 5) // args.instances for i in range(args.instances)]  # 20..24 spread evenly
+        if args.scale < 24:
+            scales = [min(s, args.scale) for s in scales]
     budget = 160 << 30
     srv = ds.Server(mem_budget_bytes=budget, lanes=args.lanes)
     insts = []
@@ -500,63 +570,53 @@ def config5(args, ds, np):
     t0 = time.time()
     srv.start()
     log(f"server started ({time.time() - t0:.1f}s)")
-    reqs = []
-    for r in range(args.requests):
-        i = int(rng.integers(0, len(insts)))
-        n, a, b, w, _ = insts[i]
-        k = int(math.exp(rng.uniform(0, math.log(100_000))))
-        c = ds.gen_changes(n, a, b, w, k, float(rng.uniform(0, 1)), 1, 1000, seed=args.seed + 1000 + r)
-        reqs.append((i, c))
-    log("requests generated")
-    # served: Poisson arrivals (or all at once when rate == 0)
-    gaps = rng.exponential(1.0 / args.rate, len(reqs)) if args.rate > 0 else np.zeros(len(reqs))
-    tickets = []
-    t0 = time.perf_counter()
-    tnext = t0
-    for (i, c), gap in zip(reqs, gaps):
-        tnext += gap
-        d = tnext - time.perf_counter()
-        if d > 0:
-            time.sleep(d)
-        tickets.append(srv.submit(i, c))
-    t_submit = time.perf_counter() - t0
-    rs = [srv.wait(tk) for tk in tickets]
-    wall = time.perf_counter() - t0
-    dev_ms = sum(r["update_ms"] / max(1, r["merged"]) for r in rs)  # each launch once
-    log(f"submitted in {t_submit * 1e3:.1f} ms, all applied after {wall * 1e3:.1f} ms, "
-        f"device time of the merged updates {dev_ms:.1f} ms")
-    total = sum(len(c) for _, c in reqs)
-    served = total / wall
-    lat = [r["latency_ms"] for r in rs]
-    # plain: the same requests one launch each, on per-instance graphs
-    plain_ms = 0.0
+    reqs = served_requests(ds, np, rng, insts, 2 * args.requests)
+    poisson, burst = reqs[:len(reqs) // 2], reqs[len(reqs) // 2:]
+    log(f"{len(reqs)} requests generated")
+    # 1. a Poisson stream below capacity; 2. the saturation run (all at once)
+    wall_p, rs_p = serve_run(srv, np, poisson, args.rate, rng)
+    wall_s, rs_s = serve_run(srv, np, burst, 0.0, rng)
+    rec_p, rec_s = sum(len(c) for _, c in poisson), sum(len(c) for _, c in burst)
+    lat_p, lat_s = [r["latency_ms"] for r in rs_p], [r["latency_ms"] for r in rs_s]
+    served = rec_s / wall_s
+    log(f"poisson {args.rate}/s: {rec_p / wall_p:.3e} changed edges/s, p50 {pct(lat_p, 50):.2f} p99 {pct(lat_p, 99):.2f} ms"
+        f"; saturated: {served:.3e}, p50 {pct(lat_s, 50):.2f} p99 {pct(lat_s, 99):.2f} ms")
+    # plain: the same requests, in order, one update each on per-instance graphs
+    plain_ms = {"poisson": 0.0, "burst": 0.0}
     same = True
-    per = {}
     for i in range(len(insts)):
         n, a, b, w, src = insts[i]
         gp = ds.Graph(n, a, b, w)
         tp = ds.Tree(gp, src)
-        for j, c in reqs:
-            if j == i:
-                plain_ms += ds.update(gp, tp, c)["t_update_ms"]
+        for part, rr in (("poisson", poisson), ("burst", burst)):
+            for j, c in rr:
+                if j == i:
+                    plain_ms[part] += ds.update(gp, tp, c)["t_update_ms"]
         pp, pd = tp.read()
         sp, sd = srv.read(i)
         same &= bool(np.array_equal(pp, sp) and np.array_equal(pd, sd))
+        tp.close()
         gp.close()
-    plain = total / (plain_ms / 1e3)
+    plain = rec_s / (plain_ms["burst"] / 1e3)
     info = srv.info()
     srv.close()
-    log(f"served {served:.3e} vs plain {plain:.3e} changed edges/s; p50 {pct(lat, 50):.2f} p99 {pct(lat, 99):.2f} ms")
-    out = {"metric": METRIC, "value": served, "unit": "changed edges/s", "n_gpus": 1, "steps": len(reqs),
-           "warmup": 0, "ms_per_step": wall / len(reqs) * 1e3, "higher_is_better": True, "scaling": "weak",
+    log(f"served {served:.3e} vs plain {plain:.3e} changed edges/s (x{served / plain:.2f}); equal to plain: {same}")
+    out = {"metric": METRIC, "value": served, "unit": "changed edges/s", "n_gpus": 1, "steps": len(burst),
+           "warmup": 0, "ms_per_step": wall_s / len(burst) * 1e3, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "u32", "data": "synthetic",
            "config": {"workload": f"served {len(insts)} rmat instances s{min(scales)}-s{max(scales)}, "
-                                  f"{len(reqs)} requests k~logU[1,1e5]", "rate_per_s": args.rate,
-                      "lanes": args.lanes,
-                      "scales": scales},
-           "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "plain_changed_edges_per_s": plain,
-           "speedup_vs_plain": served / plain, "dist_bitexact": same, "check": {"each_equals_plain": same},
-           "server": info, "mean_merged": float(np.mean([r["merged"] for r in rs])),
+                                  f"requests k~logU[1,1e5] ins~U[0,1]", "lanes": args.lanes, "scales": scales,
+                      "budget_bytes": budget},
+           "p50_ms": pct(lat_s, 50), "p99_ms": pct(lat_s, 99),
+           "poisson": {"rate_per_s": args.rate, "requests": len(poisson), "records": rec_p,
+                       "changed_edges_per_s": rec_p / wall_p, "p50_ms": pct(lat_p, 50), "p99_ms": pct(lat_p, 99),
+                       "mean_merged": float(np.mean([r["merged"] for r in rs_p])),
+                       "plain_changed_edges_per_s": rec_p / (plain_ms["poisson"] / 1e3)},
+           "saturated": {"requests": len(burst), "records": rec_s, "changed_edges_per_s": served,
+                         "p50_ms": pct(lat_s, 50), "p99_ms": pct(lat_s, 99),
+                         "mean_merged": float(np.mean([r["merged"] for r in rs_s]))},
+           "plain_changed_edges_per_s": plain, "speedup_vs_plain": served / plain,
+           "dist_bitexact": same, "check": {"each_equals_plain": same}, "server": info,
            "roofline": None, "cpu_baseline": None}
     print(json.dumps(out), flush=True)
 
@@ -579,8 +639,8 @@ def main():
             ms = ds.probe(kind, nbytes, count, reps=5)
             log(f"probe kind {kind} bytes {nbytes} count {count}: {ms:.3f} ms per launch")
         return
-    knobs = {k: getattr(args, k) for k in ("grid", "heavy_degree", "chunk", "round_batch", "use_graph",
-                                           "edges_per_thread", "tile_entries", "local_tiles", "delta")
+    knobs = {k: getattr(args, k) for k in ("block", "grid", "heavy_degree", "chunk", "round_batch", "use_graph",
+                                           "edges_per_thread", "tile_entries", "local_tiles", "delta", "sweep")
              if getattr(args, k) is not None}
     if knobs:
         ds.set_tuning(**knobs)

@@ -56,6 +56,15 @@ class Stats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class GraphUpkeep(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("base_entries", ctypes.c_int64), ("delta_entries", ctypes.c_int64),
+                ("dead_entries", ctypes.c_int64), ("pool_top", ctypes.c_uint64), ("pool_cap", ctypes.c_uint64),
+                ("emap_slots", ctypes.c_uint64), ("repacks", ctypes.c_int64)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("mem_budget_bytes", ctypes.c_uint64), ("max_merge", ctypes.c_int32), ("lanes", ctypes.c_int32),
                 ("max_records", ctypes.c_int64)]
@@ -133,6 +142,8 @@ def lib():
         "dsssp_graph_destroy": ([p], ctypes.c_int),
         "dsssp_graph_info": ([p, P64, P64, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
         "dsssp_graph_export": ([p, P64, p, p, p], ctypes.c_int),
+        "dsssp_graph_stats": ([p, ctypes.POINTER(GraphUpkeep)], ctypes.c_int),
+        "dsssp_graph_repack": ([p], ctypes.c_int),
         "dsssp_graph_clone": ([p, pp], ctypes.c_int),
         "dsssp_graph_copy": ([p, p], ctypes.c_int),
         "dsssp_tree_create": ([p, i32, p, p, u32, pp], ctypes.c_int),
@@ -153,6 +164,7 @@ def lib():
                            ctypes.c_int),
         "dsssp_server_create": ([ctypes.POINTER(ServerConfig), pp], ctypes.c_int),
         "dsssp_server_add_instance": ([p, i64, i64, p, p, p, i32, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+        "dsssp_server_remove_instance": ([p, i32], ctypes.c_int),
         "dsssp_server_start": ([p], ctypes.c_int),
         "dsssp_server_submit": ([p, i32, p, i64, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
         "dsssp_server_wait": ([p, ctypes.c_uint64, ctypes.POINTER(RequestStats)], ctypes.c_int),
@@ -338,6 +350,17 @@ class Graph:
         _check(lib().dsssp_graph_info(self._h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(by)))
         return {"n": n.value, "m_live": m.value, "device_bytes": by.value}
 
+    def upkeep(self) -> dict:
+        """Row entries scanned (base + delta), tombstones since the last repack,
+        pool and edge-map sizes, repacks so far."""
+        u = GraphUpkeep()
+        _check(lib().dsssp_graph_stats(self._h, ctypes.byref(u)))
+        return u.as_dict()
+
+    def repack(self) -> None:
+        """Drops every tombstoned entry and compacts the delta pool (on the device)."""
+        _check(lib().dsssp_graph_repack(self._h))
+
     def export(self):
         """Live undirected edges (a <= b, each once) as int32 arrays."""
         m = ctypes.c_int64()
@@ -441,14 +464,19 @@ class Server:
         h = ctypes.c_void_p()
         _check(lib().dsssp_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
-        self.sizes = []
+        self.sizes = {}
 
     def add_instance(self, n: int, a, b, w, src: int) -> int:
+        """Admits an instance (before start, or while the server runs)."""
         a, b, w = _i32(a), _i32(b), _i32(w)
         i = ctypes.c_int32()
         _check(lib().dsssp_server_add_instance(self._h, n, len(a), _ptr(a), _ptr(b), _ptr(w), src, ctypes.byref(i)))
-        self.sizes.append(n)
+        self.sizes[i.value] = n
         return i.value
+
+    def remove_instance(self, instance: int) -> None:
+        """Retires an instance once its submitted requests are applied."""
+        _check(lib().dsssp_server_remove_instance(self._h, instance))
 
     def start(self) -> None:
         _check(lib().dsssp_server_start(self._h))

@@ -103,12 +103,17 @@ struct Counters {
     unsigned long long popc;  // set bits counted by the last bitmap merge
 };
 
-// Device-side control block of one graph (delta pool allocation).
+// Device-side control block of one graph (delta pool allocation, upkeep).
 struct GraphCtl {
     unsigned long long pool_top;   // next free pool entry
     unsigned long long batch_new;  // pool entries placed by the current batch
     uint32_t err;
     uint32_t pad;
+    unsigned long long dead;       // entries tombstoned since the last repack
+    unsigned long long live[2];    // repack: live base / delta entries counted
+    unsigned long long total;      // repack: pool entries of the new delta blocks
+    uint32_t merge_first;          // merge check: first request that cannot join the merge (~0: none)
+    uint32_t pad2;
 };
 
 }  // namespace dsssp

@@ -105,9 +105,9 @@ constexpr uint32_t INIT_DCAP = 4;  // initial delta block capacity of every row
 constexpr int64_t LOCAL_MAX = 32;  // largest continuation budget of a round (tiles)
 // The full edge pass: rows longer than SWEEP_PIECE entries go to the item
 // launch in pieces of that size; both launches have (at least) as many
-// workgroups as the device holds at once at 6 waves per SIMD (6 per CU).
+// workgroups as the device holds at once at 6 waves per SIMD (24 waves per CU).
 constexpr uint32_t SWEEP_PIECE = 512;
-constexpr int SWEEP_WG_PER_CU = 6;
+constexpr int SWEEP_WAVES_PER_CU = 24;
 // A tree built from scratch starts with full edge passes while a pass still
 // lowers at least 1/SCRATCH_PASS_SHARE of the slots (low-diameter graphs),
 // at most SCRATCH_PASSES, then relaxation rounds finish it.
@@ -135,6 +135,16 @@ struct dsssp_graph {
     uint64_t pool_cap = 0;
     uint64_t pool_top = 0;       // host copy, valid between updates
     uint64_t delta_entries = 0;  // sum of delta row lengths
+    uint64_t base_entries = 0;   // sum of base row lengths (base rows shrink at a repack)
+    uint64_t dead_entries = 0;   // tombstoned entries still in rows (since the last repack)
+    int64_t repacks = 0;
+    bool emap_stale = false;     // a repack moved entries and the map could not be rebuilt yet
+    // Merge check of served requests: hash table and request starts.
+    unsigned long long *mkeys = nullptr;
+    uint32_t *mreq = nullptr;
+    uint64_t mslots = 0;
+    uint64_t *mfirst = nullptr;
+    int32_t mfirst_cap = 0;
     uint64_t *tbm = nullptr;  // rows whose delta block grows in this batch
     uint64_t *emap = nullptr;  // edge map (fingerprint << 32 | position in row)
     uint64_t emap_slots = 0;
@@ -234,6 +244,7 @@ FArgs base_args(dsssp_tree *t)
     a.budget = (uint32_t)tu.tile_entries;
     a.local = tu.local_tiles > 0 ? (uint32_t)tu.local_tiles : 0u;
     a.delta = 0;  // set for the relaxation rounds and the seeding before them
+    a.block = (uint32_t)tu.block;
     return a;
 }
 
@@ -241,9 +252,10 @@ FArgs base_args(dsssp_tree *t)
 // grid, or enough workgroups to read the whole bitmap in one pass (a wave
 // reads 64 words per pass), so that a nearly empty frontier of a large graph
 // costs one round trip to memory rather than one per pass.
-int frontier_grid(uint64_t nwords)
+int frontier_grid(uint64_t nwords, uint32_t block)
 {
-    const uint64_t one_pass = (nwords + 64 * (FRONTIER_BLOCK / 64) - 1) / (64 * (FRONTIER_BLOCK / 64));
+    const uint64_t words_per_wg = 64 * (uint64_t)(block / 64);
+    const uint64_t one_pass = (nwords + words_per_wg - 1) / words_per_wg;
     return (int)std::max<uint64_t>((uint64_t)std::max(1, tuning().grid), std::min<uint64_t>(one_pass, 4096));
 }
 
@@ -292,7 +304,7 @@ void zero_ring(dsssp_tree *t)
 void single_pass(dsssp_tree *t, int mode, FArgs a, int min_grid = 0, uint64_t *out_bm = nullptr)
 {
     hipStream_t s = t->g->stream;
-    const int grid = std::max(frontier_grid(t->nw), min_grid);
+    const int grid = std::max(frontier_grid(t->nw, a.block), min_grid);
     launch_fill_u32(&t->c->aux_hcnt, 0u, 1, s);
     a.ring = 2;  // output ring slot 0, read by round 0 of the loop that follows
     a.out_bm = out_bm ? out_bm : t->fb[0];
@@ -322,7 +334,7 @@ FArgs sweep_args(dsssp_tree *t, int &grid)
     a.heavy = a.chunk = SWEEP_PIECE;
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, t->g->device) != hipSuccess) cus = 0;
-    grid = cus * SWEEP_WG_PER_CU;
+    grid = cus * SWEEP_WAVES_PER_CU / (int)(a.block / 64);
     return a;
 }
 
@@ -399,11 +411,11 @@ int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms, bool
     const int rb = std::max(1, tu.round_batch);
     const int first = (int)std::max<int64_t>(rb, std::min<int64_t>(t->last_rounds[mode], max_group));
     int group = first;
-    const int grid = frontier_grid(t->nw);
+    FArgs base = base_args(t);
+    const int grid = frontier_grid(t->nw, base.block);
     // (no per-launch events or captured graphs for the allocation-free update)
     const bool timed = tu.kernel_timing && kernel_ms && !no_alloc;
     const bool graphs = tu.use_graph && !no_alloc;
-    FArgs base = base_args(t);
     if (mode == M_RELAX) base.delta = relax_delta(t);
     // Push-vs-full-sweep (not with distance ordering, whose rounds visit only
     // slots below a threshold, nor on the tree-vector path of 64-tree forests).
@@ -534,52 +546,89 @@ int ensure_hq(dsssp_tree *t, uint64_t total_entries)
     return DSSSP_OK;
 }
 
-// Makes room in the delta pool for a batch of n_ins insertions.  A vertex whose
-// block overflows gets a block of max(2 cap, need, 4) entries, so one batch
-// allocates at most 2 * (live capacity) + 10 * n_ins entries, and the live
-// capacity is at most the pool top.  When that bound does not fit, the pool is
-// compacted (live blocks only, capacities kept) and grown.
-int ensure_pool(dsssp_graph *g, uint64_t n_ins)
+int build_emap(dsssp_graph *g, uint64_t entries, uint64_t extra);
+int ensure_emap(dsssp_graph *g, uint64_t add);
+
+// Repack of the resident CSR on the device, in the graph's stream: every row's
+// tombstoned entries are dropped (live base entries moved to the front of the
+// row, in order; live delta entries copied to a fresh block of a new pool) and
+// the edge map is rebuilt.  Room is left in the new pool for n_ins more
+// insertions under the growth bound of the insertion kernels (a batch allocates
+// at most 2 x live capacity + 10 x n_ins entries).
+int repack(dsssp_graph *g, uint64_t n_ins)
 {
+    hipStream_t s = g->stream;
+    const uint64_t n = (uint64_t)g->n;
     const uint64_t add = 10 * n_ins;
-    if (3 * g->pool_top + add <= g->pool_cap) return DSSSP_OK;
-    // Compacted on the host: rare (geometric growth), and keeps the kernels free
-    // of pool bookkeeping.
-    std::vector<VRec> vr((size_t)g->n);
-    std::vector<uint64_t> pool((size_t)g->pool_top), out;
-    HIP_TRY(hipMemcpyAsync(vr.data(), g->vr, sizeof(VRec) * (size_t)g->n, hipMemcpyDeviceToHost, g->stream));
-    if (g->pool_top)
-        HIP_TRY(hipMemcpyAsync(pool.data(), g->pool, sizeof(uint64_t) * (size_t)g->pool_top, hipMemcpyDeviceToHost,
-                               g->stream));
-    HIP_TRY(hipStreamSynchronize(g->stream));
-    uint64_t live = 0;
-    for (int64_t v = 0; v < g->n; ++v) live += vr[v].dcap;
-    uint64_t cap = std::max<uint64_t>(g->pool_cap, 2 * (3 * live + add) + (1u << 16));
-    if (cap > 0xFFFFFFFFull) cap = 0xFFFFFFFFull;
-    if (3 * live + add > cap) return fail(DSSSP_ERR_NOMEM, "delta pool would exceed 2^32 entries");
-    out.reserve((size_t)live);
-    for (int64_t v = 0; v < g->n; ++v) {
-        if (!vr[v].dcap) continue;
-        const uint64_t d = vr[v].dbl;
-        const uint64_t beg = d >> 32, len = (uint32_t)d;
-        const uint64_t nb = out.size();
-        out.insert(out.end(), pool.begin() + (int64_t)beg, pool.begin() + (int64_t)(beg + len));
-        out.resize(nb + vr[v].dcap, 0ull);
-        vr[v].dbl = (nb << 32) | len;
+    // every new block holds max(INIT_DCAP, live delta entries of its row)
+    const uint64_t ub = INIT_DCAP * n + g->delta_entries;
+    const uint64_t ncap = std::min<uint64_t>(std::max<uint64_t>(g->pool_cap, 3 * ub + add + (1u << 16)), 0xFFFFFFFFull);
+    const uint64_t nb = (n + 255) / 256;
+    uint32_t *cap = nullptr;
+    uint64_t *bsum = nullptr, *bbase = nullptr, *npool = nullptr;
+    int st = dev_alloc((void **)&npool, sizeof(uint64_t) * ncap, "delta pool");
+    if (!st) st = dev_alloc((void **)&cap, sizeof(uint32_t) * n, "repack capacities");
+    if (!st) st = dev_alloc((void **)&bsum, sizeof(uint64_t) * nb, "repack sums");
+    if (!st) st = dev_alloc((void **)&bbase, sizeof(uint64_t) * nb, "repack offsets");
+    if (st) {
+        dev_free(npool);
+        dev_free(cap);
+        dev_free(bsum);
+        dev_free(bbase);
+        return st;
     }
-    uint64_t *np = nullptr;
-    if (int s = dev_alloc((void **)&np, sizeof(uint64_t) * cap, "delta pool")) return s;
-    if (!out.empty())
-        HIP_TRY(hipMemcpyAsync(np, out.data(), sizeof(uint64_t) * out.size(), hipMemcpyHostToDevice, g->stream));
-    HIP_TRY(hipMemcpyAsync(g->vr, vr.data(), sizeof(VRec) * (size_t)g->n, hipMemcpyHostToDevice, g->stream));
-    HIP_TRY(hipStreamSynchronize(g->stream));
+    // From here on entries move: nothing below can leave the graph half done
+    // except the edge map, which is then rebuilt before the next use.
+    g->emap_stale = true;
+    launch_fill_u64((uint64_t *)&g->gc->live[0], 0ull, 3, s);  // live[0], live[1], total
+    launch_repack_rows(g->vr, g->adj, g->pool, n, INIT_DCAP, cap, g->gc, s);
+    launch_repack_scan(cap, n, bsum, bbase, &g->gc->total, s);
+    launch_repack_delta(g->vr, g->pool, npool, n, cap, bbase, s);
+    launch_fill_u64((uint64_t *)&g->gc->dead, 0ull, 1, s);
+    const hipError_t e1 = hipMemcpyAsync(g->gc_host, g->gc, sizeof(GraphCtl), hipMemcpyDeviceToHost, s);
+    const hipError_t e2 = hipStreamSynchronize(s);
+    dev_free(cap);
+    dev_free(bsum);
+    dev_free(bbase);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        dev_free(npool);
+        return fail(DSSSP_ERR_HIP, "repack failed: %s", hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+    }
+    const uint64_t total = g->gc_host->total;
     dev_free(g->pool);
-    g->pool = np;
-    g->pool_cap = cap;
-    g->pool_top = out.size();
-    g->gc_host->pool_top = g->pool_top;
-    HIP_TRY(hipMemcpy(&g->gc->pool_top, &g->pool_top, sizeof(uint64_t), hipMemcpyHostToDevice));
+    g->pool = npool;
+    g->pool_cap = ncap;
+    g->pool_top = total;
+    g->gc_host->pool_top = total;
+    launch_fill_u64((uint64_t *)&g->gc->pool_top, total, 1, s);
+    g->base_entries = g->gc_host->live[0];
+    g->delta_entries = g->gc_host->live[1];
+    g->dead_entries = 0;
+    ++g->repacks;
+    if (3 * total + add > ncap) return fail(DSSSP_ERR_NOMEM, "delta pool would exceed 2^32 entries");
+    const uint64_t live = g->base_entries + g->delta_entries;
+    if (int e = build_emap(g, live, live / 2 + 2 * n_ins)) return e;
+    g->emap_stale = false;
     return DSSSP_OK;
+}
+
+// Upkeep before a batch with n_ins insertions: a repack when the delta pool's
+// growth bound for the batch does not fit, or when tombstones are more than
+// 1/8 of the entries that row scans read; then room in the edge map.
+int ensure_graph(dsssp_graph *g, uint64_t n_ins)
+{
+    const bool pool = n_ins > 0 && 3 * g->pool_top + 10 * n_ins > g->pool_cap;
+    const bool dead = g->dead_entries * 8 > g->base_entries + g->delta_entries;
+    if (pool || dead || g->emap_stale) {
+        if (g->emap_stale && !pool && !dead) {
+            const uint64_t live = g->base_entries + g->delta_entries;
+            if (int e = build_emap(g, live, live / 2 + 2 * n_ins)) return e;
+            g->emap_stale = false;
+        } else if (int e = repack(g, n_ins)) {
+            return e;
+        }
+    }
+    return n_ins > 0 ? ensure_emap(g, 2 * n_ins) : DSSSP_OK;
 }
 
 float elapsed(hipEvent_t a, hipEvent_t b)
@@ -676,7 +725,8 @@ int build_emap(dsssp_graph *g, uint64_t entries, uint64_t extra)
     a.ept = 4;
     a.out_hq = list;
     a.out_hcnt = &c->aux_hcnt;
-    const int grid = frontier_grid(a.nwords);
+    a.block = FRONTIER_BLOCK;
+    const int grid = frontier_grid(a.nwords, a.block);
     launch_frontier(M_EMAP, a, grid, g->stream);
     FArgs b = a;
     b.tiles = 0;
@@ -701,20 +751,23 @@ int build_emap(dsssp_graph *g, uint64_t entries, uint64_t extra)
 int ensure_emap(dsssp_graph *g, uint64_t add)
 {
     if ((g->emap_used + add) * 10 <= g->emap_slots * 7) return DSSSP_OK;
-    const uint64_t entries = g->E + g->delta_entries;  // live entries are at most these
+    const uint64_t entries = g->base_entries + g->delta_entries;  // live entries are at most these
     return build_emap(g, entries, entries / 2 + add);
 }
 
 // Tombstones every copy of every deleted (a, b, w) in both rows.
 void tombstone(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b)
 {
-    launch_tomb(b->rec, b->k, g->view(), t->bstats, g->stream);
+    launch_tomb(b->rec, b->k, g->view(), t->bstats, g->gc, g->stream);
 }
 
 // Whether the update of a batch with n_ins insertions fits the graph's and
 // tree's current buffers (the allocation-free update refuses otherwise).
 int fits_without_upkeep(dsssp_graph *g, dsssp_tree *t, uint64_t n_ins, bool zero_w)
 {
+    if (g->emap_stale)
+        return fail(DSSSP_ERR_STATE, "allocation-free update: the edge map must be rebuilt first "
+                                     "(apply one update with dsssp_update_batch)");
     if (n_ins > 0 && 3 * g->pool_top + 10 * n_ins > g->pool_cap)
         return fail(DSSSP_ERR_STATE, "allocation-free update: the delta pool would have to grow for %llu insertions "
                                      "(apply one update with dsssp_update_batch first)", (unsigned long long)n_ins);
@@ -744,10 +797,7 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
     if (no_alloc) {
         if (int e = fits_without_upkeep(g, t, (uint64_t)b->n_ins, b->zero_w)) return e;
     } else {
-        if (b->n_ins > 0) {
-            if (int e = ensure_pool(g, (uint64_t)b->n_ins)) return e;
-            if (int e = ensure_emap(g, 2 * (uint64_t)b->n_ins)) return e;
-        }
+        if (int e = ensure_graph(g, (uint64_t)b->n_ins)) return e;
         if (int e = ensure_hq(t, g->E + g->delta_entries + 2 * (uint64_t)b->n_ins)) return e;
     }
     int64_t inv_rounds = 0, relax_rounds = 0, repair_rounds = 0;
@@ -806,6 +856,7 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
         g->pool_top = g->gc_host->pool_top;
         g->delta_entries += 2 * (uint64_t)b->n_ins;
     }
+    g->dead_entries = g->gc_host->dead;
     if (!code) code = confirm_overflow(t);
     if (!code) code = finish_errors(t, b->n_ins > 0 ? g->gc_host->err : 0u);
     if (code) {
@@ -912,7 +963,7 @@ extern "C" int dsssp_set_tuning(const dsssp_tuning *t)
     if (!t) return fail(DSSSP_ERR_ARG, "null argument");
     std::lock_guard<std::mutex> lk(g_tuning_mu);
     dsssp_tuning nt = g_tuning;
-    if (t->block && t->block != FRONTIER_BLOCK) return fail(DSSSP_ERR_ARG, "block must be %d", FRONTIER_BLOCK);
+    if (t->block) nt.block = t->block;
     if (t->grid) nt.grid = t->grid;
     if (t->heavy_degree) nt.heavy_degree = t->heavy_degree;
     if (t->chunk) nt.chunk = t->chunk;
@@ -928,6 +979,7 @@ extern "C" int dsssp_set_tuning(const dsssp_tuning *t)
     if (nt.grid < 1 || nt.grid > 65536 || nt.heavy_degree < 64 || nt.chunk < 64 || nt.chunk % 64 ||
         nt.round_batch < 1 || nt.round_batch > 64 || nt.tile_entries < 64 || nt.local_tiles < -1 ||
         nt.local_tiles > 1024 || nt.delta < DSSSP_DELTA_AUTO || nt.sweep < -1 || nt.sweep > 1000 || nt.give_up_ms < 1 ||
+        (nt.block != 128 && nt.block != 256 && nt.block != 512) ||
         (nt.edges_per_thread != 1 && nt.edges_per_thread != 2 && nt.edges_per_thread != 4))
         return fail(DSSSP_ERR_ARG, "tuning out of range");
     g_tuning = nt;
@@ -972,6 +1024,9 @@ static void graph_release(dsssp_graph *g)
     dev_free(g->bsum);
     dev_free(g->bbase);
     dev_free(g->gc);
+    dev_free(g->mkeys);
+    dev_free(g->mreq);
+    dev_free(g->mfirst);
     if (g->gc_host) (void)hipHostFree(g->gc_host);
     for (auto &e : g->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1057,6 +1112,7 @@ extern "C" int dsssp_graph_create(int64_t n, int64_t m, const int32_t *a, const 
         return s;
     }
     g->pool_top = INIT_DCAP * (uint64_t)n;
+    g->base_entries = g->E;
     g->gc_host->pool_top = g->pool_top;
     int s = hipMemcpy(&g->gc->pool_top, &g->pool_top, sizeof(uint64_t), hipMemcpyHostToDevice) == hipSuccess
                 ? DSSSP_OK
@@ -1102,6 +1158,30 @@ extern "C" int dsssp_graph_info(const dsssp_graph *g, int64_t *n, int64_t *m_liv
         *device_bytes = 32 * (uint64_t)g->n + 8 * g->E + 12 * (uint64_t)g->nw +
                         16 * (uint64_t)((g->nw + 255) / 256) + 8 * g->pool_cap + 8 * g->emap_slots;
     return DSSSP_OK;
+}
+
+extern "C" int dsssp_graph_stats(const dsssp_graph *g, dsssp_graph_upkeep *out)
+{
+    clear_error();
+    if (!g || !out) return fail(DSSSP_ERR_ARG, "null argument");
+    out->n = g->n;
+    out->base_entries = (int64_t)g->base_entries;
+    out->delta_entries = (int64_t)g->delta_entries;
+    out->dead_entries = (int64_t)g->dead_entries;
+    out->pool_top = g->pool_top;
+    out->pool_cap = g->pool_cap;
+    out->emap_slots = g->emap_slots;
+    out->repacks = g->repacks;
+    return DSSSP_OK;
+}
+
+extern "C" int dsssp_graph_repack(dsssp_graph *g)
+{
+    clear_error();
+    if (!g) return fail(DSSSP_ERR_ARG, "null argument");
+    DevGuard dg(g->device);
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return repack(g, 0);
 }
 
 extern "C" int dsssp_graph_export(dsssp_graph *g, int64_t *m, int32_t *a, int32_t *b, int32_t *w)
@@ -1192,6 +1272,10 @@ extern "C" int dsssp_graph_copy(dsssp_graph *dst, dsssp_graph *src)
     HIP_TRY(hipStreamSynchronize(s));
     dst->pool_top = src->pool_top;
     dst->delta_entries = src->delta_entries;
+    dst->base_entries = src->base_entries;
+    dst->dead_entries = src->dead_entries;
+    dst->repacks = src->repacks;
+    dst->emap_stale = src->emap_stale;
     dst->emap_used = src->emap_used;
     dst->zero_w = src->zero_w;
     dst->auto_delta = src->auto_delta;
@@ -1819,31 +1903,75 @@ void pinned_free(void *p)
     if (p) (void)hipHostFree(p);
 }
 
-int update_pinned(dsssp_graph *g, dsssp_tree *t, const dsssp_change *const *buf, const int64_t *len, int32_t nbuf,
-                  int64_t ni, int64_t nd, bool zw, dsssp_stats *stats)
+int update_pinned(dsssp_graph *g, dsssp_tree *t, const dsssp_change *const *buf, const int64_t *len,
+                  const int64_t *ni, const int64_t *nd, const uint8_t *zw, int32_t nbuf, bool check,
+                  int32_t *applied, dsssp_stats *stats)
 {
     clear_error();
-    if (!g || !t || nbuf < 0 || (nbuf > 0 && (!buf || !len))) return fail(DSSSP_ERR_ARG, "bad argument");
+    *applied = 0;
+    if (!g || !t || nbuf < 1 || !buf || !len) return fail(DSSSP_ERR_ARG, "bad argument");
     if (t->g != g) return fail(DSSSP_ERR_ARG, "tree belongs to another graph");
-    int64_t k = 0;
-    for (int32_t i = 0; i < nbuf; ++i) k += len[i];
-    if (k == 0) {
+    std::vector<uint64_t> first((size_t)nbuf + 1, 0);
+    uint64_t dels = 0;
+    for (int32_t i = 0; i < nbuf; ++i) {
+        first[i + 1] = first[i] + (uint64_t)len[i];
+        dels += (uint64_t)nd[i];
+    }
+    const int64_t k = (int64_t)first[nbuf];
+    DevGuard dg(g->device);
+    if (k > 0) {
+        if (int s = ensure_batch(g, k)) return s;
+    }
+    dsssp_batch *b = g->upd;
+    HIP_TRY(hipEventRecord(g->ev[8], g->stream));
+    for (int32_t i = 0; i < nbuf; ++i)
+        if (len[i])
+            HIP_TRY(hipMemcpyAsync(b->rec + first[i], buf[i], sizeof(int4) * (size_t)len[i], hipMemcpyHostToDevice,
+                                   g->stream));
+    HIP_TRY(hipEventRecord(g->ev[9], g->stream));
+    int32_t take = nbuf;
+    if (check && nbuf > 1 && dels > 0) {
+        uint64_t slots = 1024;
+        while (slots < 2 * dels) slots *= 2;
+        if (g->mslots < slots) {
+            HIP_TRY(hipStreamSynchronize(g->stream));
+            dev_free(g->mkeys);
+            dev_free(g->mreq);
+            g->mslots = 0;
+            if (int s = dev_alloc((void **)&g->mkeys, 8 * slots, "merge check keys")) return s;
+            if (int s = dev_alloc((void **)&g->mreq, 4 * slots, "merge check requests")) return s;
+            g->mslots = slots;
+        }
+        if (g->mfirst_cap < nbuf) {
+            HIP_TRY(hipStreamSynchronize(g->stream));
+            dev_free(g->mfirst);
+            g->mfirst_cap = 0;
+            const int32_t cap = std::max(64, 2 * nbuf);
+            if (int s = dev_alloc((void **)&g->mfirst, 8 * (size_t)cap, "merge check starts")) return s;
+            g->mfirst_cap = cap;
+        }
+        HIP_TRY(hipMemcpyAsync(g->mfirst, first.data(), 8 * (size_t)nbuf, hipMemcpyHostToDevice, g->stream));
+        launch_merge_check(b->rec, k, g->mfirst, (uint32_t)nbuf, g->mkeys, g->mreq, g->mslots, &g->gc->merge_first,
+                           g->stream);
+        HIP_TRY(hipMemcpyAsync(&g->gc_host->merge_first, &g->gc->merge_first, sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        const uint32_t f = g->gc_host->merge_first;
+        if (f < (uint32_t)nbuf) take = std::max<int32_t>(1, (int32_t)f);
+    }
+    int64_t ki = 0, kd = 0;
+    bool kz = false;
+    for (int32_t i = 0; i < take; ++i) {
+        ki += ni[i];
+        kd += nd[i];
+        kz = kz || zw[i];
+    }
+    *applied = take;
+    if (first[take] == 0) {
         if (stats) *stats = dsssp_stats{};
         return DSSSP_OK;
     }
-    DevGuard dg(g->device);
-    if (int s = ensure_batch(g, k)) return s;
-    dsssp_batch *b = g->upd;
-    int64_t pos = 0;
-    HIP_TRY(hipEventRecord(g->ev[8], g->stream));
-    for (int32_t i = 0; i < nbuf; ++i) {
-        if (len[i])
-            HIP_TRY(hipMemcpyAsync(b->rec + pos, buf[i], sizeof(int4) * (size_t)len[i], hipMemcpyHostToDevice,
-                                   g->stream));
-        pos += len[i];
-    }
-    HIP_TRY(hipEventRecord(g->ev[9], g->stream));
-    return apply_uploaded(g, t, k, ni, nd, zw, stats);
+    return apply_uploaded(g, t, (int64_t)first[take], ki, kd, kz, stats);
 }
 
 }  // namespace dsssp
@@ -1931,6 +2059,7 @@ extern "C" int dsssp_autotune(dsssp_graph *g, dsssp_tree *t, dsssp_batch *const 
         std::vector<int32_t> values;
     };
     const std::vector<Knob> knobs = {
+        {&dsssp_tuning::block, {128, 256, 512}},
         {&dsssp_tuning::grid, {512, 1024, 2048}},
         {&dsssp_tuning::edges_per_thread, {2, 4}},
         {&dsssp_tuning::heavy_degree, {128, 256, 512}},

@@ -12,12 +12,17 @@
 
 namespace dsssp {
 
-// Served form: applies the concatenation of nbuf pinned host record buffers to
-// (g, t), already in g's vertex ids and validated, with ni insertions, nd
-// deletions and zw set when an insertion has weight 0 (engine.hip).  The
+// Served form: applies nbuf requests -- pinned host record buffers, already in
+// g's vertex ids and validated, with ni[i] insertions, nd[i] deletions and
+// zw[i] set when an insertion has weight 0 -- to (g, t) as one batch
+// (engine.hip).  With check set, the merged records are first checked on the
+// device: only the requests before the first one that inserts a key an earlier
+// request deletes are applied (one batch would then leave another graph than
+// the requests one by one); *applied = the requests applied, always >= 1.  The
 // buffers may be reused once it returns.
-int update_pinned(dsssp_graph *g, dsssp_tree *t, const dsssp_change *const *buf, const int64_t *len, int32_t nbuf,
-                  int64_t ni, int64_t nd, bool zw, dsssp_stats *stats);
+int update_pinned(dsssp_graph *g, dsssp_tree *t, const dsssp_change *const *buf, const int64_t *len,
+                  const int64_t *ni, const int64_t *nd, const uint8_t *zw, int32_t nbuf, bool check,
+                  int32_t *applied, dsssp_stats *stats);
 // Page-locked host memory (hipHostMalloc / hipHostFree).
 int pinned_alloc(void **p, size_t bytes);
 void pinned_free(void *p);

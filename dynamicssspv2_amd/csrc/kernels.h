@@ -79,9 +79,11 @@ struct FArgs {
     // ones; it still consumes the input bitmap.  0: never.  Rounds count their
     // marks into ring slot ring + 1 only when sweep is set.
     uint32_t sweep;
+    uint32_t block;  // threads per workgroup: 128, 256 or 512 (the launch's grid is in workgroups of this size)
+    uint32_t pad_;
 };
 
-constexpr int FRONTIER_BLOCK = 256;
+constexpr int FRONTIER_BLOCK = 256;  // default workgroup size of frontier launches
 
 // With start/stop events, they are recorded by the dispatch itself
 // (hipExtLaunchKernel), bracketing exactly this kernel.
@@ -122,8 +124,27 @@ void launch_ins_pending(const int4 *rec, int64_t k, GView g, uint32_t *tslot, co
 // the tombstoning of every copy of each deleted entry.
 void launch_classify(const int4 *rec, int64_t k, uint32_t kshift, uint64_t *st, uint64_t *out_bm, uint64_t *inv_bm,
                      uint32_t *out_act, hipStream_t s);
-// Tombstones every copy of each deleted entry, found through the edge map.
-void launch_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats, hipStream_t s);
+// Tombstones every copy of each deleted entry, found through the edge map; the
+// copies tombstoned are added to gc->dead.
+void launch_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats, GraphCtl *gc, hipStream_t s);
+// Merge check of k records made of nreq requests (request q starts at record
+// first_rec[q]; first_rec has nreq entries): *first = the smallest q whose
+// insertions include a key (a, b, w) that a request before q deletes, ~0 if
+// none.  keys/req: a hash table of `slots` entries (a power of two, at least
+// twice the deletions), cleared here.
+void launch_merge_check(const int4 *rec, int64_t k, const uint64_t *first_rec, uint32_t nreq,
+                        unsigned long long *keys, uint32_t *req, uint64_t slots, uint32_t *first, hipStream_t s);
+// Repack of the resident CSR: live base entries to the front of each row
+// (blen, blo rewritten; cap[v] = capacity of the row's new delta block; live
+// totals added to gc->live), the exclusive scan of cap (bbase per RB rows, the
+// sum in *total), then the live delta entries copied to npool (dbl, dcap
+// rewritten).
+void launch_repack_rows(VRec *vr, uint64_t *adj, const uint64_t *pool, uint64_t n, uint32_t min_cap, uint32_t *cap,
+                        GraphCtl *gc, hipStream_t s);
+void launch_repack_scan(const uint32_t *cap, uint64_t n, uint64_t *bsum, uint64_t *bbase, unsigned long long *total,
+                        hipStream_t s);
+void launch_repack_delta(VRec *vr, const uint64_t *pool, uint64_t *npool, uint64_t n, const uint32_t *cap,
+                         const uint64_t *bbase, hipStream_t s);
 // With far set, the distances it marks go to ring slot 0 of c->fenc.
 void launch_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint32_t *tslot, GView g, uint64_t *st,
                       uint64_t *out_bm, uint32_t *out_act, Counters *c, bool far, hipStream_t s);

@@ -42,9 +42,7 @@ namespace dsssp {
 
 namespace {
 
-constexpr int FB = FRONTIER_BLOCK;
-constexpr int FW = FB / 64;  // waves per workgroup
-constexpr int RB = 256;      // block size of per-record / per-vertex kernels
+constexpr int RB = 256;  // block size of per-record / per-vertex kernels
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -99,8 +97,8 @@ __device__ __forceinline__ void emap_insert(const GView &g, uint32_t row, uint32
 }
 
 // Tombstones every live copy of (row, col, w); returns map slots and delta
-// entries read.
-__device__ __forceinline__ uint32_t emap_kill(const GView &g, uint32_t row, uint32_t col, uint32_t w)
+// entries read, and adds the copies it tombstoned to `kills`.
+__device__ __forceinline__ uint32_t emap_kill(const GView &g, uint32_t row, uint32_t col, uint32_t w, uint32_t &kills)
 {
     const uint64_t h = emap_hash(row, col, w);
     const uint32_t fp = emap_fp(h);
@@ -118,13 +116,17 @@ __device__ __forceinline__ uint32_t emap_kill(const GView &g, uint32_t row, uint
         if ((uint32_t)(v >> 32) != fp || pos >= blen + dl) continue;
         uint64_t *ep = pos < blen ? &g.adj[b0 + pos] : &g.pool[(d >> 32) + (pos - blen)];
         const uint64_t e = *ep;
-        if (e_key(e) == key && !(e & E_DEAD)) *ep = e | E_DEAD;
+        // (a returning atomicOr, so that a copy that duplicate records kill
+        // together is counted once)
+        if (e_key(e) == key && !(e & E_DEAD))
+            kills += !(atomicOr((unsigned long long *)ep, E_DEAD) & E_DEAD);
     }
     const uint32_t ns = min(dl, EMAP_DSKIP);
     uint64_t *dp = &g.pool[d >> 32];
     for (uint32_t i = 0; i < ns; ++i) {
         const uint64_t e = dp[i];
-        if (e_key(e) == key && !(e & E_DEAD)) dp[i] = e | E_DEAD;
+        if (e_key(e) == key && !(e & E_DEAD))
+            kills += !(atomicOr((unsigned long long *)&dp[i], E_DEAD) & E_DEAD);
     }
     return probes + ns;
 }
@@ -310,16 +312,17 @@ __device__ __forceinline__ void visit_next(const FArgs &A, uint32_t tgt, uint32_
     }
 }
 
-struct TileSmem {
-    uint32_t off[FW][64];
-    uint32_t u[FW][64];
-    uint32_t aux0[FW][64];
-    uint32_t aux1[FW][64];
-    uint32_t blen[FW][64];
-    uint32_t dbeg[FW][64];
-    uint64_t b0[FW][64];
-    unsigned long long pull[FW][64];  // SEED / PICK: per-owner minimum candidate
-    uint32_t stk[FW][LSTK];           // push modes: slots to visit in this launch
+// One wave's LDS: the fields of its tile's up to 64 slots, and its stack.
+struct WaveSmem {
+    uint32_t off[64];
+    uint32_t u[64];
+    uint32_t aux0[64];
+    uint32_t aux1[64];
+    uint32_t blen[64];
+    uint32_t dbeg[64];
+    uint64_t b0[64];
+    unsigned long long pull[64];  // SEED / PICK: per-owner minimum candidate
+    uint32_t stk[LSTK];           // push modes: slots to visit in this launch
 };
 
 constexpr bool is_pull(int mode) { return mode == M_SEED || mode == M_PICK || mode == M_SWEEP; }
@@ -415,8 +418,8 @@ __device__ __forceinline__ uint32_t select_bit(uint64_t w, uint32_t k)
 // fresh tile (from the input bitmap) hands its heavy rows to items and covers all
 // its other entries; an item covers A.chunk entries of its range.
 template <int MODE, int EPT>
-__device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start, bool fresh, TileSmem &sm,
-                         uint32_t wv, WaveStats &S, uint32_t &top)
+__device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start, bool fresh, WaveSmem &sm,
+                         WaveStats &S, uint32_t &top)
 {
     const uint32_t lane = lane_id();
     uint32_t deg = 0, blen = 0, dbeg = 0, aux0 = 0, aux1 = 0;
@@ -500,14 +503,14 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
     const uint32_t total = __shfl(incl, 63);
     const uint32_t end = fresh ? total : min(start + A.chunk, total);
     if (end <= start) return;
-    sm.off[wv][lane] = incl - deg;
-    sm.u[wv][lane] = os;
-    sm.aux0[wv][lane] = aux0;
-    sm.aux1[wv][lane] = aux1;
-    sm.blen[wv][lane] = blen;
-    sm.dbeg[wv][lane] = dbeg;
-    sm.b0[wv][lane] = b0;
-    if constexpr (is_pull(MODE)) sm.pull[wv][lane] = ~0ull;
+    sm.off[lane] = incl - deg;
+    sm.u[lane] = os;
+    sm.aux0[lane] = aux0;
+    sm.aux1[lane] = aux1;
+    sm.blen[lane] = blen;
+    sm.dbeg[lane] = dbeg;
+    sm.b0[lane] = b0;
+    if constexpr (is_pull(MODE)) sm.pull[lane] = ~0ull;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -525,21 +528,21 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
                 uint32_t o = 0;
 #pragma unroll
                 for (uint32_t st = 32; st > 0; st >>= 1)
-                    if (sm.off[wv][o + st] <= e) o += st;
-                const uint32_t jj = e - sm.off[wv][o];
-                const uint32_t bl = sm.blen[wv][o];
+                    if (sm.off[o + st] <= e) o += st;
+                const uint32_t jj = e - sm.off[o];
+                const uint32_t bl = sm.blen[o];
                 oi[j] = o;
                 pos[j] = jj;
-                ows[j] = sm.u[wv][o];
-                a0[j] = sm.aux0[wv][o];
-                a1[j] = sm.aux1[wv][o];
-                ent[j] = jj < bl ? A.g.adj[sm.b0[wv][o] + jj] : A.g.pool[(uint64_t)sm.dbeg[wv][o] + (jj - bl)];
+                ows[j] = sm.u[o];
+                a0[j] = sm.aux0[o];
+                a1[j] = sm.aux1[o];
+                ent[j] = jj < bl ? A.g.adj[sm.b0[o] + jj] : A.g.pool[(uint64_t)sm.dbeg[o] + (jj - bl)];
             }
         }
         if constexpr (MODE == M_EMAP) {
 #pragma unroll
             for (int j = 0; j < EPT; ++j)
-                if (act[j] && !(ent[j] & E_DEAD) && emap_keeps(pos[j], sm.blen[wv][oi[j]]))
+                if (act[j] && !(ent[j] & E_DEAD) && emap_keeps(pos[j], sm.blen[oi[j]]))
                     emap_insert(A.g, ows[j], e_col(ent[j]), e_w(ent[j]), pos[j]);
             continue;
         }
@@ -552,10 +555,10 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
             uint32_t td = 0;
             const uint32_t tgt = apply<MODE>(A, ows[j], a0[j], a1[j], ent[j], pre[j], S, pull, td);
             if constexpr (is_pull(MODE)) {
-                if (pull != ~0ull) atomicMin(&sm.pull[wv][oi[j]], (unsigned long long)pull);
+                if (pull != ~0ull) atomicMin(&sm.pull[oi[j]], (unsigned long long)pull);
             }
             if constexpr (MODE == M_RELAX || MODE == M_INVAL || MODE == M_TBFS || MODE == M_SWEEP)
-                visit_next<MODE>(A, tgt, td, sm.stk[wv], top, S);
+                visit_next<MODE>(A, tgt, td, sm.stk, top, S);
         }
     }
     S.edges += end - start;
@@ -564,9 +567,9 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if constexpr (is_pull(MODE)) {
         // One commit per owner with entries in this range.
-        const uint32_t lo = sm.off[wv][lane];
+        const uint32_t lo = sm.off[lane];
         const bool mine = deg != 0 && lo < end && lo + deg > start;
-        const uint64_t best = sm.pull[wv][lane];
+        const uint64_t best = sm.pull[lane];
         if (mine && pull_commit<MODE>(A, os, best)) {
             mark(A.out_bm, os);
             S.any = true;
@@ -758,7 +761,7 @@ __device__ void do_forest(const FArgs &A, uint32_t v, uint64_t mask, uint32_t st
 // Visits the slots on the wave's stack, 64 at a time, for at most A.local
 // tiles; what is left is marked for the next round.
 template <int MODE, int EPT>
-__device__ void drain(const FArgs &A, TileSmem &sm, uint32_t wv, WaveStats &S, uint32_t &top)
+__device__ void drain(const FArgs &A, WaveSmem &sm, WaveStats &S, uint32_t &top)
 {
     if constexpr (MODE == M_RELAX || MODE == M_INVAL || MODE == M_TBFS) {
         const uint32_t lane = lane_id();
@@ -769,7 +772,7 @@ __device__ void drain(const FArgs &A, TileSmem &sm, uint32_t wv, WaveStats &S, u
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             if (budget == 0) {
                 for (uint32_t i = lane; i < top; i += 64) {
-                    const uint32_t slot = sm.stk[wv][i];
+                    const uint32_t slot = sm.stk[i];
                     mark(A.out_bm, slot);
                     if constexpr (MODE == M_RELAX) {
                         if (A.delta) note_mark(S, st_dist(A.st[slot]));
@@ -783,19 +786,20 @@ __device__ void drain(const FArgs &A, TileSmem &sm, uint32_t wv, WaveStats &S, u
             --budget;
             const uint32_t n = min(64u, top);
             const bool valid = lane < n;
-            const uint32_t os = valid ? sm.stk[wv][top - n + lane] : 0u;
+            const uint32_t os = valid ? sm.stk[top - n + lane] : 0u;
             top -= n;
-            do_slots<MODE, EPT>(A, os, valid, 0u, true, sm, wv, S, top);
+            do_slots<MODE, EPT>(A, os, valid, 0u, true, sm, S, top);
         }
     }
 }
 
 // At least 6 waves per SIMD: the register budget is capped so that occupancy
 // hides the latency of the random gathers.
-template <int MODE, int EPT, bool FOREST>
+template <int MODE, int EPT, bool FOREST, int FB>
 __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_frontier(FArgs A)
 {
-    __shared__ TileSmem sm;
+    constexpr int FW = FB / 64;  // waves per workgroup
+    __shared__ WaveSmem smem[FW];
     const uint32_t lane = lane_id();
     if (blockIdx.x == 0 && threadIdx.x < 64) {
         if (threadIdx.x == 0) {
@@ -841,6 +845,7 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_
     const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t gw = blockIdx.x * FW + wv;
     const uint32_t nw = gridDim.x * FW;
+    WaveSmem &sm = smem[wv];
     // The bitmap is read in blocks of 8 words (64 bytes); in one pass a wave
     // reads 8 blocks nw apart, so that a cluster of non-empty words (frontiers
     // cluster by vertex id) spreads over many waves.  The set slots of the 64
@@ -907,9 +912,9 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_
             const uint32_t owi = (uint32_t)__shfl((int)wi, (int)o);
             const bool valid = q < T;
             const uint32_t os = valid ? owi * 64 + select_bit(ow, q - oex) : 0u;
-            do_slots<MODE, EPT>(A, os, valid, 0u, true, sm, wv, S, top);
+            do_slots<MODE, EPT>(A, os, valid, 0u, true, sm, S, top);
         }
-        drain<MODE, EPT>(A, sm, wv, S, top);
+        drain<MODE, EPT>(A, sm, S, top);
     }
     for (uint32_t it = gw; it < H; it += nw) {
         const uint64_t mask = A.in_hq[2 * (uint64_t)it];
@@ -921,9 +926,9 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_
                 continue;
             }
         }
-        do_slots<MODE, EPT>(A, widx * 64 + lane, (mask >> lane) & 1ull, (uint32_t)ws, false, sm, wv, S, top);
+        do_slots<MODE, EPT>(A, widx * 64 + lane, (mask >> lane) & 1ull, (uint32_t)ws, false, sm, S, top);
     }
-    drain<MODE, EPT>(A, sm, wv, S, top);
+    drain<MODE, EPT>(A, sm, S, top);
     const uint32_t marks = wave_sum(S.marks);
     const uint32_t gath = MODE == M_SWEEP ? wave_sum(S.gath) : 0u;
     const bool any = __ballot(S.any) != 0;
@@ -1166,19 +1171,24 @@ __global__ __launch_bounds__(RB) void k_ins_pending(const int4 *rec, int64_t k, 
 
 // Deletion records: thread 2i+s tombstones the copies of record i in row a
 // (s = 0) or row b (s = 1); a self-loop's two copies both sit in row a.
-__global__ __launch_bounds__(RB) void k_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats)
+__global__ __launch_bounds__(RB) void k_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats,
+                                             GraphCtl *gc)
 {
     const int64_t t = (int64_t)blockIdx.x * RB + threadIdx.x;
     const int64_t ri = t >> 1;
-    uint32_t probes = 0;
+    uint32_t probes = 0, kills = 0;
     if (ri < k) {
         const int4 r = rec[ri];
         const bool side = t & 1;
         if (r.w == 0 && !(side && r.x == r.y))
-            probes = emap_kill(g, (uint32_t)(side ? r.y : r.x), (uint32_t)(side ? r.x : r.y), (uint32_t)r.z);
+            probes = emap_kill(g, (uint32_t)(side ? r.y : r.x), (uint32_t)(side ? r.x : r.y), (uint32_t)r.z, kills);
     }
     const uint32_t p = wave_sum(probes);
-    if (lane_id() == 0) add_stat(bstats, ST_PROBES, p);
+    const uint32_t kl = wave_sum(kills);
+    if (lane_id() == 0) {
+        add_stat(bstats, ST_PROBES, p);
+        if (kl) atomicAdd(&gc->dead, (unsigned long long)kl);
+    }
 }
 
 // deleteEdge's tree-edge test (gpuFunctions_undir.cuh:189-198), weight ignored:
@@ -1273,6 +1283,224 @@ __global__ __launch_bounds__(RB) void k_ins_offer(const int4 *rec, int64_t k, ui
 #pragma unroll
         for (int d = 32; d > 0; d >>= 1) fe = max(fe, (uint32_t)__shfl_xor((int)fe, d));
         if (fe && lane_id() == 0) raise_u32(&c->fenc[0][blockIdx.x & 63u], fe);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Merge check of a batch made of several requests applied as one.  One batch
+// applies its insertions, then removes every copy of each deleted (a, b, w),
+// so it leaves the graph that the requests applied one by one leave unless a
+// request inserts a key that an earlier request deletes.  The deletions enter
+// a hash table of key -> earliest request index (atomicMin); each insertion
+// looks its key up; *first = the smallest index of a request that inserts a
+// key an earlier request deletes (~0: none).  Keys are 64-bit hashes of
+// (min(a,b), max(a,b), w): a collision can only report a conflict that is not
+// there, which splits the merge and costs time, never exactness.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t change_key(const int4 &r)
+{
+    const uint64_t lo = (uint32_t)min(r.x, r.y), hi = (uint32_t)max(r.x, r.y);
+    uint64_t h = ((lo << 32) | hi) * 0x9E3779B97F4A7C15ull;
+    h ^= ((uint64_t)(uint32_t)r.z + 0x51ull) * 0xC2B2AE3D27D4EB4Full;
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 29;
+    return h ? h : 1ull;
+}
+
+// Request index of record i: the last request whose first record is <= i.
+__device__ __forceinline__ uint32_t request_of(const uint64_t *first_rec, uint32_t nreq, uint64_t i)
+{
+    uint32_t lo = 0, hi = nreq;  // first_rec[lo] <= i < first_rec[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (first_rec[mid] <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(RB) void k_conflict_del(const int4 *rec, int64_t k, const uint64_t *first_rec,
+                                                     uint32_t nreq, unsigned long long *keys, uint32_t *req,
+                                                     uint64_t mask)
+{
+    const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
+    if (i >= k) return;
+    const int4 r = rec[i];
+    if (r.w != 0) return;
+    const unsigned long long h = change_key(r);
+    const uint32_t q = request_of(first_rec, nreq, (uint64_t)i);
+    for (uint64_t sl = h & mask;; sl = (sl + 1) & mask) {
+        const unsigned long long prev = atomicCAS(&keys[sl], 0ull, h);
+        if (prev == 0ull || prev == h) {
+            atomicMin(&req[sl], q);
+            return;
+        }
+    }
+}
+
+__global__ __launch_bounds__(RB) void k_conflict_ins(const int4 *rec, int64_t k, const uint64_t *first_rec,
+                                                     uint32_t nreq, const unsigned long long *keys,
+                                                     const uint32_t *req, uint64_t mask, uint32_t *first)
+{
+    const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
+    if (i >= k) return;
+    const int4 r = rec[i];
+    if (r.w != 1) return;
+    const unsigned long long h = change_key(r);
+    for (uint64_t sl = h & mask;; sl = (sl + 1) & mask) {
+        const unsigned long long kk = keys[sl];
+        if (kk == 0ull) return;
+        if (kk == h) {
+            const uint32_t q = request_of(first_rec, nreq, (uint64_t)i);
+            if (req[sl] < q) atomicMin(first, q);
+            return;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Repack of the resident CSR (tombstone reclamation and pool compaction), in
+// three steps, one wave per row in the first and last:
+//   rows   a row's live base entries move to the front of its base segment in
+//          order (so it stays sorted), its base length and the count of them
+//          below the vertex's own id are rewritten, and the capacity of its new
+//          delta block (at least min_cap, at least its live delta entries) is
+//          recorded; live totals go to gc->live
+//   scan   new pool offsets: exclusive prefix sum of the capacities
+//   delta  the row's live delta entries are copied, in order, to its new block
+//          in the new pool
+// The edge map is rebuilt afterwards (entry positions changed).
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(RB) void k_repack_rows(VRec *vr, uint64_t *adj, const uint64_t *pool, uint64_t n,
+                                                    uint32_t min_cap, uint32_t *cap, GraphCtl *gc)
+{
+    const uint32_t lane = lane_id();
+    const uint64_t gw = ((uint64_t)blockIdx.x * RB + threadIdx.x) >> 6;
+    const uint64_t nw = ((uint64_t)gridDim.x * RB) >> 6;
+    unsigned long long lb = 0, ld = 0;  // wave-uniform totals
+    for (uint64_t v = gw; v < n; v += nw) {
+        const uint64_t boff = vr[v].boff;
+        const uint32_t blen = vr[v].blen;
+        const uint64_t dd = vr[v].dbl;
+        uint32_t w = 0, below = 0;
+        for (uint32_t i0 = 0; i0 < blen; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            uint64_t e = 0;
+            bool live = false;
+            if (i < blen) {
+                e = adj[boff + i];
+                live = !(e & E_DEAD);
+            }
+            // every lane's entry is loaded before any is stored (the ballot
+            // needs them all), and stores go at or below the chunk read
+            const unsigned long long m = __ballot(live);
+            if (live) {
+                adj[boff + w + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = e;
+                below += e_col(e) < (uint32_t)v;
+            }
+            w += (uint32_t)__popcll(m);
+        }
+        below = wave_sum(below);
+        const uint64_t dbeg = dd >> 32;
+        const uint32_t dl = (uint32_t)dd;
+        uint32_t live_d = 0;
+        for (uint32_t i0 = 0; i0 < dl; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            live_d += (uint32_t)__popcll(__ballot(i < dl && !(pool[dbeg + i] & E_DEAD)));
+        }
+        if (lane == 0) {
+            vr[v].blen = w;
+            vr[v].blo = below;
+            cap[v] = max(min_cap, live_d);
+        }
+        lb += w;
+        ld += live_d;
+    }
+    if (lane == 0 && (lb || ld)) {
+        atomicAdd(&gc->live[0], lb);
+        atomicAdd(&gc->live[1], ld);
+    }
+}
+
+// Scan, step 1: bsum[b] = sum of cap over the RB rows of workgroup b.
+__global__ __launch_bounds__(RB) void k_scan_blocks(const uint32_t *cap, uint64_t n, uint64_t *bsum)
+{
+    const uint64_t v = (uint64_t)blockIdx.x * RB + threadIdx.x;
+    uint64_t total;
+    (void)block_exclusive_scan(v < n ? cap[v] : 0u, &total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// Scan, step 2 (one workgroup): bbase = exclusive prefix sum of bsum[0, nb);
+// *total = the sum.
+__global__ __launch_bounds__(1024) void k_scan_top(const uint64_t *bsum, uint64_t nb, uint64_t *bbase,
+                                                   unsigned long long *total)
+{
+    __shared__ unsigned long long part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint64_t per = (nb + 1023) / 1024;
+    const uint64_t lo = min((uint64_t)t * per, nb), hi = min(lo + per, nb);
+    unsigned long long sum = 0;
+    for (uint64_t i = lo; i < hi; ++i) sum += bsum[i];
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {
+        const unsigned long long y = t >= d ? part[t - d] : 0ull;
+        __syncthreads();
+        part[t] += y;
+        __syncthreads();
+    }
+    unsigned long long run = part[t] - sum;
+    for (uint64_t i = lo; i < hi; ++i) {
+        bbase[i] = run;
+        run += bsum[i];
+    }
+    if (t == 1023) *total = part[1023];
+}
+
+// Scan, step 3 and the move: row v's new block starts at bbase[block] plus the
+// in-block prefix of cap; its live delta entries are copied there in order.
+__global__ __launch_bounds__(RB) void k_repack_delta(VRec *vr, const uint64_t *pool, uint64_t *npool, uint64_t n,
+                                                     const uint32_t *cap, const uint64_t *bbase)
+{
+    __shared__ uint64_t off[RB];
+    __shared__ uint32_t cp[RB];
+    const uint64_t v0 = (uint64_t)blockIdx.x * RB;
+    const uint64_t v = v0 + threadIdx.x;
+    const uint32_t c = v < n ? cap[v] : 0u;
+    uint64_t total;
+    off[threadIdx.x] = bbase[blockIdx.x] + block_exclusive_scan(c, &total);
+    cp[threadIdx.x] = c;
+    __syncthreads();
+    // each wave copies the rows of the workgroup in turn, one row at a time
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+    for (uint32_t j = wv; j < RB && v0 + j < n; j += RB / 64) {
+        const uint64_t row = v0 + j;
+        const uint64_t dd = vr[row].dbl;
+        const uint64_t dbeg = dd >> 32;
+        const uint32_t dl = (uint32_t)dd;
+        uint32_t w = 0;
+        for (uint32_t i0 = 0; i0 < dl; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            uint64_t e = 0;
+            bool live = false;
+            if (i < dl) {
+                e = pool[dbeg + i];
+                live = !(e & E_DEAD);
+            }
+            const unsigned long long m = __ballot(live);
+            if (live) npool[off[j] + w + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = e;
+            w += (uint32_t)__popcll(m);
+        }
+        if (lane == 0) {
+            vr[row].dbl = (off[j] << 32) | w;
+            vr[row].dcap = cp[j];
+        }
     }
 }
 
@@ -1428,13 +1656,24 @@ __global__ __launch_bounds__(RB) void k_apply_best(uint32_t nslots, uint32_t ksh
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + RB - 1) / RB); }
 
+template <int MODE, int EPT, bool FOREST, int FB>
+void launch_block(const FArgs &a, int grid, hipStream_t s, hipEvent_t start, hipEvent_t stop)
+{
+    if (start || stop)
+        hipExtLaunchKernelGGL((k_frontier<MODE, EPT, FOREST, FB>), dim3(grid), dim3(FB), 0, s, start, stop, 0, a);
+    else
+        hipLaunchKernelGGL((k_frontier<MODE, EPT, FOREST, FB>), dim3(grid), dim3(FB), 0, s, a);
+}
+
+// Workgroup size of the launch: a.block threads (128, 256 or 512).
 template <int MODE, int EPT, bool FOREST>
 void launch_kernel(const FArgs &a, int grid, hipStream_t s, hipEvent_t start, hipEvent_t stop)
 {
-    if (start || stop)
-        hipExtLaunchKernelGGL((k_frontier<MODE, EPT, FOREST>), dim3(grid), dim3(FB), 0, s, start, stop, 0, a);
-    else
-        hipLaunchKernelGGL((k_frontier<MODE, EPT, FOREST>), dim3(grid), dim3(FB), 0, s, a);
+    switch (a.block) {
+    case 128: launch_block<MODE, EPT, FOREST, 128>(a, grid, s, start, stop); break;
+    case 512: launch_block<MODE, EPT, FOREST, 512>(a, grid, s, start, stop); break;
+    default: launch_block<MODE, EPT, FOREST, 256>(a, grid, s, start, stop); break;
+    }
 }
 
 // Slot layouts with 64 trees per vertex use the variant with the tree-vector
@@ -1507,6 +1746,43 @@ void launch_validate(const int4 *rec, int64_t k, int64_t n, unsigned long long *
     if (k > 0) hipLaunchKernelGGL(k_validate, dim3(blocks_for(k)), dim3(RB), 0, s, rec, k, n, cnt);
 }
 
+void launch_merge_check(const int4 *rec, int64_t k, const uint64_t *first_rec, uint32_t nreq,
+                        unsigned long long *keys, uint32_t *req, uint64_t slots, uint32_t *first, hipStream_t s)
+{
+    launch_fill_u64((uint64_t *)keys, 0ull, slots, s);
+    launch_fill_u32(req, 0xFFFFFFFFu, slots, s);
+    launch_fill_u32(first, 0xFFFFFFFFu, 1, s);
+    if (k <= 0) return;
+    hipLaunchKernelGGL(k_conflict_del, dim3(blocks_for(k)), dim3(RB), 0, s, rec, k, first_rec, nreq, keys, req,
+                       slots - 1);
+    hipLaunchKernelGGL(k_conflict_ins, dim3(blocks_for(k)), dim3(RB), 0, s, rec, k, first_rec, nreq, keys, req,
+                       slots - 1, first);
+}
+
+void launch_repack_rows(VRec *vr, uint64_t *adj, const uint64_t *pool, uint64_t n, uint32_t min_cap, uint32_t *cap,
+                        GraphCtl *gc, hipStream_t s)
+{
+    if (n == 0) return;
+    const unsigned g = (unsigned)std::min<uint64_t>((n * 64 + RB - 1) / RB, 16384);
+    hipLaunchKernelGGL(k_repack_rows, dim3(g), dim3(RB), 0, s, vr, adj, pool, n, min_cap, cap, gc);
+}
+
+void launch_repack_scan(const uint32_t *cap, uint64_t n, uint64_t *bsum, uint64_t *bbase, unsigned long long *total,
+                        hipStream_t s)
+{
+    const uint64_t nb = (n + RB - 1) / RB;
+    if (nb == 0) return;
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(RB), 0, s, cap, n, bsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, bsum, nb, bbase, total);
+}
+
+void launch_repack_delta(VRec *vr, const uint64_t *pool, uint64_t *npool, uint64_t n, const uint32_t *cap,
+                         const uint64_t *bbase, hipStream_t s)
+{
+    const uint64_t nb = (n + RB - 1) / RB;
+    if (nb) hipLaunchKernelGGL(k_repack_delta, dim3((unsigned)nb), dim3(RB), 0, s, vr, pool, npool, n, cap, bbase);
+}
+
 void launch_probe(int kind, const uint64_t *in, uint64_t n, uint64_t count, uint64_t *out, hipStream_t s)
 {
     if (kind == 0)
@@ -1575,9 +1851,9 @@ void launch_classify(const int4 *rec, int64_t k, uint32_t kshift, uint64_t *st, 
                            inv_bm, out_act);
 }
 
-void launch_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats, hipStream_t s)
+void launch_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats, GraphCtl *gc, hipStream_t s)
 {
-    if (k > 0) hipLaunchKernelGGL(k_tomb, dim3(blocks_for(2 * k)), dim3(RB), 0, s, rec, k, g, bstats);
+    if (k > 0) hipLaunchKernelGGL(k_tomb, dim3(blocks_for(2 * k)), dim3(RB), 0, s, rec, k, g, bstats, gc);
 }
 
 void launch_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint32_t *tslot, GView g, uint64_t *st,

@@ -105,7 +105,8 @@ typedef struct {
 
 /* Tuning knobs (zero fields keep the current value). */
 typedef struct {
-    int32_t block;            /* threads per workgroup of frontier kernels (fixed 256) */
+    int32_t block;            /* threads per workgroup of frontier kernels: 128, 256
+                                 (default) or 512                                    */
     int32_t grid;             /* least workgroups per frontier launch (more when the
                                  bitmap needs them to be read in one pass)          */
     int32_t heavy_degree;     /* a tile (the set slots of one 64-bit frontier word)
@@ -216,6 +217,22 @@ int dsssp_graph_destroy(dsssp_graph *g);
 int dsssp_graph_info(const dsssp_graph *g, int64_t *n, int64_t *m_live, uint64_t *device_bytes);
 /* Live undirected edges (each once, a <= b).  With a == NULL only *m is set. */
 int dsssp_graph_export(dsssp_graph *g, int64_t *m, int32_t *a, int32_t *b, int32_t *w);
+/* Upkeep state of the resident CSR.  base_entries + delta_entries is what
+ * scans of every row read (live and tombstoned entries); dead_entries counts
+ * the tombstoned ones since the last repack.  A repack (on the device, inside
+ * the next update's time, t_upkeep_ms) drops every tombstoned entry and
+ * compacts the delta pool; it runs before a batch whose insertions would
+ * outgrow the pool, and when tombstones exceed 1/8 of the scanned entries.
+ * No reference counterpart: the reference rebuilds its adjacency on the host
+ * for every run (all_structure_undir.cuh:106-117, SSSPmainForUndirected.cu:66-72). */
+typedef struct {
+    int64_t n, base_entries, delta_entries, dead_entries;
+    uint64_t pool_top, pool_cap, emap_slots;
+    int64_t repacks;
+} dsssp_graph_upkeep;
+int dsssp_graph_stats(const dsssp_graph *g, dsssp_graph_upkeep *out);
+/* A repack now (synchronous); the graph's trees are unaffected. */
+int dsssp_graph_repack(dsssp_graph *g);
 /* Device-to-device copy of the whole graph state; dst must have been created
  * from a graph of the same n and base size (dsssp_graph_clone makes one). */
 int dsssp_graph_clone(dsssp_graph *src, dsssp_graph **out);
@@ -301,8 +318,8 @@ int dsssp_update_noalloc(dsssp_graph *g, dsssp_tree *t, const dsssp_change *d_ch
                          dsssp_stats *stats);
 /* Autotuner (no reference counterpart; the reference has fixed launch
  * shapes, SSSPmainForUndirected.cu:200-203,229-236,271-277): coordinate descent
- * over grid, edges_per_thread, heavy_degree, tile_entries, local_tiles,
- * round_batch and delta, each trial applying the nb resident batches to a
+ * over block, grid, edges_per_thread, heavy_degree, tile_entries, local_tiles,
+ * round_batch, delta and sweep, each trial applying the nb resident batches to a
  * snapshot of g and t (restored afterwards, so both end as they started) and
  * timed on the device; a trial whose tree differs from the first one's is an
  * error.  Sets the fastest tuning globally and returns it with its device time
@@ -313,12 +330,15 @@ int dsssp_autotune(dsssp_graph *g, dsssp_tree *t, dsssp_batch *const *batches, i
 /* ---- served form -------------------------------------------------------------
  * Resident instances (each a graph with its own source) kept in HBM under a
  * memory budget, and request loops that merge batches arriving for different
- * instances into shared launches (continuous batching).  The instances are
- * spread over `lanes`, each one device graph with its own stream and loop
- * thread, so the merged updates of different lanes overlap on the GPU.  Each
- * instance's result is bit-identical to updating it alone.  No reference
- * counterpart: the reference applies one batch to one graph per process
- * (SSSPmainForUndirected.cu:27-335). */
+ * instances -- and consecutive batches of one instance -- into shared launches
+ * (continuous batching).  The instances are spread over `lanes`, each one
+ * device graph with its own stream and loop thread, so the merged updates of
+ * different lanes overlap on the GPU.  Consecutive requests of one instance are
+ * merged only while no request inserts an edge (a, b, w) that an earlier
+ * request of the same merge deletes, the one case where one batch would leave
+ * a different graph.  Each instance's result is bit-identical to applying its
+ * requests alone, one update each.  No reference counterpart: the reference
+ * applies one batch to one graph per process (SSSPmainForUndirected.cu:27-335). */
 typedef struct dsssp_server dsssp_server;
 typedef struct {
     uint64_t mem_budget_bytes;  /* device bytes all admitted instances may use      */
@@ -342,9 +362,18 @@ typedef struct {
 } dsssp_server_totals;
 
 int dsssp_server_create(const dsssp_server_config *cfg, dsssp_server **out);
-/* Admits an instance before start; DSSSP_ERR_BUDGET if it does not fit. */
+/* Admits an instance; DSSSP_ERR_BUDGET if it does not fit the budget left.
+ * Before dsssp_server_start it is placed with the others at start; while the
+ * server runs it gets a lane of its own, built in the calling thread while the
+ * other lanes go on serving, and takes submissions when this call returns. */
 int dsssp_server_add_instance(dsssp_server *s, int64_t n, int64_t m, const int32_t *a, const int32_t *b,
                               const int32_t *w, int32_t src, int32_t *id);
+/* Stops submissions to an instance, waits until its submitted requests are
+ * applied, and retires it.  Its lane's device memory, and the lane's share of
+ * the budget, is released once every instance of that lane is retired (an
+ * instance admitted while the server runs has a lane of its own, so its share
+ * comes back at once).  Its id is not reused. */
+int dsssp_server_remove_instance(dsssp_server *s, int32_t id);
 /* Uploads every admitted instance, computes their trees, starts the loop. */
 int dsssp_server_start(dsssp_server *s);
 /* Queues a batch for one instance (ids local to the instance); returns at once. */

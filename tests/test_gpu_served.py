@@ -121,3 +121,124 @@ def test_served_concurrent_submitters():
         pp, pd = t.read()
         assert np.array_equal(pd, dist) and np.array_equal(pp, par)
     srv.close()
+
+
+def plain_tree(n, a, b, w, s, seq):
+    """The plain path: the requests applied one update each to a graph of its own."""
+    g = ds.Graph(n, a, b, w)
+    t = ds.Tree(g, s)
+    for c in seq:
+        ds.update(g, t, c)
+    out = t.read()
+    t.close()
+    g.close()
+    return out
+
+
+def test_served_merges_one_instance_exactly():
+    """Requests of one instance merge into shared updates except across a key
+    that an earlier request deletes and a later one inserts again; the result
+    equals the requests applied one by one."""
+    rng = np.random.default_rng(31)
+    a, b, w = ds.gen_rmat(12, 16, wlo=1, whi=50, seed=310)
+    n = 1 << 12
+    src = int(np.argmax(np.bincount(np.concatenate([a, b]), minlength=n)))
+    srv = ds.Server(mem_budget_bytes=1 << 32)
+    iid = srv.add_instance(n, a, b, w, src)
+    srv.start()
+    seq, ca, cb, cw = [], a, b, w
+    for j in range(40):
+        if j % 7 == 3 and seq:
+            # re-insert some keys the previous request deleted: must not merge with it
+            prev = seq[-1]
+            dels = prev[prev[:, 3] == 0][:5].copy()
+            dels[:, 3] = 1
+            c = np.concatenate([dels, random_batch(rng, n, ca, cb, cw, 50, 0.5, 1, 50)]).astype(np.int32)
+        else:
+            c = random_batch(rng, n, ca, cb, cw, int(rng.integers(1, 400)), rng.uniform(0, 1), 1, 50)
+        ca, cb, cw = oracle.apply_changes(n, ca, cb, cw, c)
+        seq.append(c)
+    say("served: 40 requests to one instance, all at once")
+    tickets = [srv.submit(iid, c) for c in seq]
+    stats = [srv.wait(t) for t in tickets]
+    assert max(s["merged"] for s in stats) > 1  # consecutive requests of the instance merged
+    par, dist = srv.read(iid)
+    pp, pd = plain_tree(n, a, b, w, src, seq)
+    assert np.array_equal(pd, dist) and np.array_equal(pp, par)
+    _, dref = oracle.dijkstra(oracle.CSR.from_edges(n, ca, cb, cw), [src])
+    assert np.array_equal(dist, dref)
+    srv.close()
+
+
+def test_served_admit_and_remove_while_serving():
+    """Instances join while requests of others are in flight and leave after
+    theirs; the budget admits a new instance once a late one is removed; every
+    served instance equals the plain path alone."""
+    rng = np.random.default_rng(41)
+
+    def inst(scale, seed):
+        a, b, w = ds.gen_rmat(scale, 16, wlo=1, whi=1000, seed=seed)
+        n = 1 << scale
+        return n, a, b, w, int(np.argmax(np.bincount(np.concatenate([a, b]), minlength=n)))
+
+    A, B, C, D = inst(12, 411), inst(11, 412), inst(13, 413), inst(13, 414)
+    # the budget: what A and B hold, plus room for one of C, D (measured on an
+    # unstarted server, which charges the same bytes at admission)
+    probe = ds.Server(mem_budget_bytes=1 << 40)
+    probe.add_instance(*A)
+    probe.add_instance(*B)
+    u = probe.info()["used_bytes"]
+    probe.add_instance(*C)
+    c_bytes = probe.info()["used_bytes"] - u
+    probe.close()
+    srv = ds.Server(mem_budget_bytes=u + c_bytes + c_bytes // 2)
+    ia, ib = srv.add_instance(*A), srv.add_instance(*B)
+    srv.start()
+    cur = {ia: A[1:4], ib: B[1:4]}
+    seqs = {ia: [], ib: []}
+
+    def next_batch(i, n, k):
+        ca, cb, cw = cur[i]
+        c = random_batch(rng, n, ca, cb, cw, k, rng.uniform(0, 1), 1, 1000)
+        cur[i] = oracle.apply_changes(n, ca, cb, cw, c)
+        seqs[i].append(c)
+        return c
+
+    tickets = []
+    for j in range(30):  # a burst to A and B, in flight while C is admitted
+        i = ia if j % 2 else ib
+        tickets.append(srv.submit(i, next_batch(i, (A if i == ia else B)[0], int(rng.integers(50, 2000)))))
+    say("served: admitting an instance while requests are in flight")
+    ic = srv.add_instance(*C)
+    cur[ic], seqs[ic] = C[1:4], []
+    for j in range(20):
+        i = (ia, ib, ic)[j % 3]
+        n = (A, B, C)[(ia, ib, ic).index(i)][0]
+        tickets.append(srv.submit(i, next_batch(i, n, int(rng.integers(50, 2000)))))
+    with pytest.raises(ds.DssspError) as e:  # no room for D while C is held
+        srv.add_instance(*D)
+    assert e.value.code == ds.ERR_BUDGET
+    for t in tickets:
+        srv.wait(t)
+    pc = srv.read(ic)
+    srv.remove_instance(ic)  # C had a lane of its own: its budget comes back
+    with pytest.raises(ds.DssspError):
+        srv.submit(ic, seqs[ic][0])
+    u_before_d = srv.info()["used_bytes"]
+    assert u_before_d == u
+    id_d = srv.add_instance(*D)
+    d_bytes = srv.info()["used_bytes"] - u_before_d
+    cur[id_d], seqs[id_d] = D[1:4], []
+    for j in range(10):
+        srv.wait(srv.submit(id_d, next_batch(id_d, D[0], int(rng.integers(50, 2000)))))
+    for i, I in ((ia, A), (ib, B), (id_d, D)):
+        par, dist = srv.read(i)
+        pp, pd = plain_tree(*I, seqs[i])
+        assert np.array_equal(pd, dist) and np.array_equal(pp, par), i
+    pp, pd = plain_tree(*C, seqs[ic])
+    assert np.array_equal(pd, pc[1]) and np.array_equal(pp, pc[0])
+    srv.remove_instance(ia)
+    assert srv.info()["used_bytes"] == u + d_bytes  # A's lane still holds B
+    srv.remove_instance(ib)  # the start-time lane of A and B is released with its last instance
+    assert srv.info()["used_bytes"] == d_bytes
+    srv.close()

@@ -506,3 +506,106 @@ def test_autotune_keeps_graph_tree_and_results():
             r.close()
     finally:
         ds.set_tuning(**old)
+
+
+@pytest.mark.parametrize("block", [128, 512])
+def test_block_sizes_byte_equal(block):
+    """The frontier kernels at 128 and 512 threads per workgroup give the bytes
+    of the default 256: trees from scratch, update sequences, a forest, and the
+    full edge pass."""
+    a, b, w = ds.gen_rmat(12, 16, seed=61)
+    n = 1 << 12
+    src = int(np.argmax(np.bincount(np.concatenate([a, b]), minlength=n)))
+    batches, ca, cb, cw = [], a, b, w
+    for i in range(3):
+        c = ds.gen_changes(n, ca, cb, cw, 1500, 0.5, 1, 1000, seed=62 + i)
+        ca, cb, cw = oracle.apply_changes(n, ca, cb, cw, c)
+        batches.append(c)
+    old = ds.get_tuning()
+
+    def run():
+        g = ds.Graph(n, a, b, w)
+        t = ds.Tree(g, src)
+        outs = [np.stack(t.read())]
+        for c in batches:
+            ds.update(g, t, c)
+            outs.append(np.stack(t.read()))
+        st = t.relax_full()
+        outs.append(np.stack(t.read()))
+        t.close()
+        g.close()
+        g = ds.Graph(n, a, b, w)  # the forest on a graph of its own
+        f = ds.Tree(g, sources=[src, 1, 2, 3, 4])
+        ds.update(g, f, batches[0])
+        outs += [np.stack(f.read(j)) for j in range(5)]
+        f.close()
+        g.close()
+        return outs, st["relax_improved"]
+
+    try:
+        ref, imp0 = run()
+        ds.set_tuning(block=block)
+        say(f"block {block}")
+        got, imp1 = run()
+    finally:
+        ds.set_tuning(**{k: old[k] for k in ("block",)})
+    assert imp0 == 0 and imp1 == 0
+    for x, y in zip(ref, got):
+        assert np.array_equal(x, y)
+    csr = oracle.CSR.from_edges(n, ca, cb, cw)
+    _, dref = oracle.dijkstra(csr, [src])
+    assert np.array_equal(got[3][1], dref)
+
+
+def test_resident_csr_upkeep_bounded():
+    """Forty-eight deletion-heavy batches: tombstoned entries are reclaimed on
+    the device (repacks inside the updates), so the entries that row scans read
+    stay within a fixed factor of the live ones instead of growing with every
+    batch; the pool is compacted on the device as insertions outgrow it; every
+    tree is exact and the resident CSR equals the oracle's graph."""
+    rng = np.random.default_rng(91)
+    n = 6000
+    a, b, w = random_connected(rng, n, 60_000, 1, 100)
+    src = 0
+    g = ds.Graph(n, a, b, w)
+    t = ds.Tree(g, src)
+    mirror = Mirror(n, a, b, w)
+    peak = 0
+    inserted = 0
+    try:
+        for step in range(48):
+            # 60 % deletions of current edges, 40 % insertions of random pairs
+            chg = random_batch(rng, n, mirror.a, mirror.b, mirror.w, 2500, 0.4, 1, 100)
+            inserted += 2 * int(np.sum(chg[:, 3] == 1))
+            st = ds.update(g, t, chg)
+            mirror.apply(chg)
+            u = g.upkeep()
+            live = 2 * len(mirror.a)
+            scanned = u["base_entries"] + u["delta_entries"]
+            peak = max(peak, scanned / max(1, live))
+            # repacked before a batch once tombstones pass 1/8 of the scanned
+            # entries, so after any batch: scanned <= live + 1/8 scanned + the
+            # batch's own deletions (2 copies each, parallel copies included)
+            assert u["dead_entries"] <= scanned
+            assert scanned - u["dead_entries"] == live, (step, u, live)
+            if step % 8 == 7:
+                check_tree(mirror, *t.read(), [src], f"upkeep step {step}")
+                check_graph(mirror, g)
+        u = g.upkeep()
+        scanned = u["base_entries"] + u["delta_entries"]
+        say(f"upkeep: {u['repacks']} repacks, peak scanned/live {peak:.2f}, scanned {scanned} "
+            f"vs {2 * len(a) + inserted} without reclamation")
+        assert u["repacks"] >= 3
+        assert peak < 1.5
+        assert scanned < 0.6 * (2 * len(a) + inserted)
+        g.repack()  # an explicit repack drops every tombstone
+        u = g.upkeep()
+        assert u["dead_entries"] == 0 and u["base_entries"] + u["delta_entries"] == 2 * len(mirror.a)
+        chg = random_batch(rng, n, mirror.a, mirror.b, mirror.w, 2000, 0.5, 1, 100)
+        ds.update(g, t, chg)
+        mirror.apply(chg)
+        check_tree(mirror, *t.read(), [src], "after explicit repack")
+        check_graph(mirror, g)
+    finally:
+        t.close()
+        g.close()
