@@ -81,6 +81,7 @@ enum Stat : int {
     ST_PROBES,           // entries read by deletion searches
     ST_SWEEPS,           // relaxation rounds run as full sweeps (every slot visited)
     ST_SWEEP_GATHERS,    // entries of an M_SWEEP pass that read their neighbour's state
+    ST_KILLS,            // entries tombstoned by deletions
     NSTAT
 };
 constexpr int STAT_SLOTS = 2048;
@@ -109,7 +110,6 @@ struct GraphCtl {
     unsigned long long batch_new;  // pool entries placed by the current batch
     uint32_t err;
     uint32_t pad;
-    unsigned long long dead;       // entries tombstoned since the last repack
     unsigned long long live[2];    // repack: live base / delta entries counted
     unsigned long long total;      // repack: pool entries of the new delta blocks
     uint32_t merge_first;          // merge check: first request that cannot join the merge (~0: none)

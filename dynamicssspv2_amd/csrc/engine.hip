@@ -136,7 +136,8 @@ struct dsssp_graph {
     uint64_t pool_top = 0;       // host copy, valid between updates
     uint64_t delta_entries = 0;  // sum of delta row lengths
     uint64_t base_entries = 0;   // sum of base row lengths (base rows shrink at a repack)
-    uint64_t dead_entries = 0;   // tombstoned entries still in rows (since the last repack)
+    uint64_t dead_entries = 0;   // tombstoned entries still in rows (since the last repack; duplicate
+                                 // deletion records in one batch may count a copy twice)
     int64_t repacks = 0;
     bool emap_stale = false;     // a repack moved entries and the map could not be rebuilt yet
     // Merge check of served requests: hash table and request starts.
@@ -584,7 +585,6 @@ int repack(dsssp_graph *g, uint64_t n_ins)
     launch_repack_rows(g->vr, g->adj, g->pool, n, INIT_DCAP, cap, g->gc, s);
     launch_repack_scan(cap, n, bsum, bbase, &g->gc->total, s);
     launch_repack_delta(g->vr, g->pool, npool, n, cap, bbase, s);
-    launch_fill_u64((uint64_t *)&g->gc->dead, 0ull, 1, s);
     const hipError_t e1 = hipMemcpyAsync(g->gc_host, g->gc, sizeof(GraphCtl), hipMemcpyDeviceToHost, s);
     const hipError_t e2 = hipStreamSynchronize(s);
     dev_free(cap);
@@ -758,7 +758,7 @@ int ensure_emap(dsssp_graph *g, uint64_t add)
 // Tombstones every copy of every deleted (a, b, w) in both rows.
 void tombstone(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b)
 {
-    launch_tomb(b->rec, b->k, g->view(), t->bstats, g->gc, g->stream);
+    launch_tomb(b->rec, b->k, g->view(), t->bstats, g->stream);
 }
 
 // Whether the update of a batch with n_ins insertions fits the graph's and
@@ -856,7 +856,7 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
         g->pool_top = g->gc_host->pool_top;
         g->delta_entries += 2 * (uint64_t)b->n_ins;
     }
-    g->dead_entries = g->gc_host->dead;
+    g->dead_entries += t->c_host->stats[ST_KILLS];
     if (!code) code = confirm_overflow(t);
     if (!code) code = finish_errors(t, b->n_ins > 0 ? g->gc_host->err : 0u);
     if (code) {

@@ -125,8 +125,8 @@ void launch_ins_pending(const int4 *rec, int64_t k, GView g, uint32_t *tslot, co
 void launch_classify(const int4 *rec, int64_t k, uint32_t kshift, uint64_t *st, uint64_t *out_bm, uint64_t *inv_bm,
                      uint32_t *out_act, hipStream_t s);
 // Tombstones every copy of each deleted entry, found through the edge map; the
-// copies tombstoned are added to gc->dead.
-void launch_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats, GraphCtl *gc, hipStream_t s);
+// copies tombstoned are counted in the statistics (ST_KILLS).
+void launch_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats, hipStream_t s);
 // Merge check of k records made of nreq requests (request q starts at record
 // first_rec[q]; first_rec has nreq entries): *first = the smallest q whose
 // insertions include a key (a, b, w) that a request before q deletes, ~0 if

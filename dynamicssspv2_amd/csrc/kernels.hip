@@ -116,17 +116,19 @@ __device__ __forceinline__ uint32_t emap_kill(const GView &g, uint32_t row, uint
         if ((uint32_t)(v >> 32) != fp || pos >= blen + dl) continue;
         uint64_t *ep = pos < blen ? &g.adj[b0 + pos] : &g.pool[(d >> 32) + (pos - blen)];
         const uint64_t e = *ep;
-        // (a returning atomicOr, so that a copy that duplicate records kill
-        // together is counted once)
-        if (e_key(e) == key && !(e & E_DEAD))
-            kills += !(atomicOr((unsigned long long *)ep, E_DEAD) & E_DEAD);
+        if (e_key(e) == key && !(e & E_DEAD)) {
+            *ep = e | E_DEAD;
+            ++kills;  // (duplicate records killing one copy together count it twice)
+        }
     }
     const uint32_t ns = min(dl, EMAP_DSKIP);
     uint64_t *dp = &g.pool[d >> 32];
     for (uint32_t i = 0; i < ns; ++i) {
         const uint64_t e = dp[i];
-        if (e_key(e) == key && !(e & E_DEAD))
-            kills += !(atomicOr((unsigned long long *)&dp[i], E_DEAD) & E_DEAD);
+        if (e_key(e) == key && !(e & E_DEAD)) {
+            dp[i] = e | E_DEAD;
+            ++kills;
+        }
     }
     return probes + ns;
 }
@@ -1057,6 +1059,22 @@ __device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t *t
     return before + incl - v;
 }
 
+// Sum of v over the workgroup's RB threads, valid in thread 0 (so that a
+// counter shared by the whole grid takes one atomic per workgroup).
+__device__ __forceinline__ unsigned long long block_sum(unsigned long long v)
+{
+    __shared__ unsigned long long part[RB / 64];
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
+    if (lane_id() == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    unsigned long long s = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < RB / 64; ++i) s += part[i];
+    __syncthreads();
+    return s;
+}
+
 // Capacity of a delta block that must hold `need` entries and holds `cap`:
 // doubling, so that a row is moved O(log length) times over its lifetime.
 __device__ __forceinline__ uint32_t grown_cap(uint32_t cap, uint32_t need)
@@ -1171,8 +1189,7 @@ __global__ __launch_bounds__(RB) void k_ins_pending(const int4 *rec, int64_t k, 
 
 // Deletion records: thread 2i+s tombstones the copies of record i in row a
 // (s = 0) or row b (s = 1); a self-loop's two copies both sit in row a.
-__global__ __launch_bounds__(RB) void k_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats,
-                                             GraphCtl *gc)
+__global__ __launch_bounds__(RB) void k_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats)
 {
     const int64_t t = (int64_t)blockIdx.x * RB + threadIdx.x;
     const int64_t ri = t >> 1;
@@ -1187,7 +1204,7 @@ __global__ __launch_bounds__(RB) void k_tomb(const int4 *rec, int64_t k, GView g
     const uint32_t kl = wave_sum(kills);
     if (lane_id() == 0) {
         add_stat(bstats, ST_PROBES, p);
-        if (kl) atomicAdd(&gc->dead, (unsigned long long)kl);
+        add_stat(bstats, ST_KILLS, kl);
     }
 }
 
@@ -1421,7 +1438,10 @@ __global__ __launch_bounds__(RB) void k_repack_rows(VRec *vr, uint64_t *adj, con
         lb += w;
         ld += live_d;
     }
-    if (lane == 0 && (lb || ld)) {
+    // lb, ld are wave-uniform: count them once per wave
+    lb = block_sum(lane == 0 ? lb : 0ull);
+    ld = block_sum(lane == 0 ? ld : 0ull);
+    if (threadIdx.x == 0 && (lb || ld)) {
         atomicAdd(&gc->live[0], lb);
         atomicAdd(&gc->live[1], ld);
     }
@@ -1538,26 +1558,25 @@ __global__ __launch_bounds__(RB) void k_fill_u32(uint32_t *p, uint32_t v, uint64
 // with another op are ignored.
 __global__ __launch_bounds__(RB) void k_validate(const int4 *rec, int64_t k, int64_t n, unsigned long long *cnt)
 {
-    const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
-    uint32_t ins = 0, del = 0, zw = 0;
-    if (i < k) {
+    unsigned long long ins = 0, del = 0, zw = 0;
+    for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < k; i += (int64_t)gridDim.x * RB) {
         const int4 r = rec[i];
         if (r.w == 0 || r.w == 1) {
             if (r.x < 0 || r.x >= n || r.y < 0 || r.y >= n || r.z < 0)
                 atomicMin(&cnt[3], (unsigned long long)i);
             else if (r.w == 1)
-                ins = 1, zw = r.z == 0;
+                ins += 1, zw += r.z == 0;
             else
-                del = 1;
+                del += 1;
         }
     }
-    ins = wave_sum(ins);
-    del = wave_sum(del);
-    zw = wave_sum(zw);
-    if (lane_id() == 0) {
-        if (ins) atomicAdd(&cnt[0], (unsigned long long)ins);
-        if (del) atomicAdd(&cnt[1], (unsigned long long)del);
-        if (zw) atomicAdd(&cnt[2], (unsigned long long)zw);
+    ins = block_sum(ins);
+    del = block_sum(del);
+    zw = block_sum(zw);
+    if (threadIdx.x == 0) {
+        if (ins) atomicAdd(&cnt[0], ins);
+        if (del) atomicAdd(&cnt[1], del);
+        if (zw) atomicAdd(&cnt[2], zw);
     }
 }
 
@@ -1597,8 +1616,8 @@ __global__ __launch_bounds__(RB) void k_merge_count(uint64_t *dst, uint64_t *src
             c += (uint32_t)__popcll(v);
         }
     }
-    c = wave_sum(c);
-    if (lane_id() == 0 && c) atomicAdd(count, (unsigned long long)c);
+    const unsigned long long tot = block_sum(c);
+    if (threadIdx.x == 0 && tot) atomicAdd(count, tot);
 }
 
 // out[v] = st[v << kshift | tree]: one tree's states, contiguous.
@@ -1743,7 +1762,7 @@ void launch_validate(const int4 *rec, int64_t k, int64_t n, unsigned long long *
 {
     launch_fill_u64((uint64_t *)cnt, 0ull, 3, s);
     launch_fill_u64((uint64_t *)cnt + 3, ~0ull, 1, s);
-    if (k > 0) hipLaunchKernelGGL(k_validate, dim3(blocks_for(k)), dim3(RB), 0, s, rec, k, n, cnt);
+    if (k > 0) hipLaunchKernelGGL(k_validate, dim3(std::min(blocks_for(k), 1024u)), dim3(RB), 0, s, rec, k, n, cnt);
 }
 
 void launch_merge_check(const int4 *rec, int64_t k, const uint64_t *first_rec, uint32_t nreq,
@@ -1763,7 +1782,7 @@ void launch_repack_rows(VRec *vr, uint64_t *adj, const uint64_t *pool, uint64_t 
                         GraphCtl *gc, hipStream_t s)
 {
     if (n == 0) return;
-    const unsigned g = (unsigned)std::min<uint64_t>((n * 64 + RB - 1) / RB, 16384);
+    const unsigned g = (unsigned)std::min<uint64_t>((n * 64 + RB - 1) / RB, 4096);
     hipLaunchKernelGGL(k_repack_rows, dim3(g), dim3(RB), 0, s, vr, adj, pool, n, min_cap, cap, gc);
 }
 
@@ -1795,7 +1814,7 @@ void launch_merge_count(uint64_t *dst, uint64_t *src, uint64_t nwords, unsigned 
 {
     launch_fill_u64((uint64_t *)count, 0ull, 1, s);
     if (nwords == 0) return;
-    const unsigned g = (unsigned)std::min<uint64_t>((nwords + RB - 1) / RB, 4096);
+    const unsigned g = (unsigned)std::min<uint64_t>((nwords + RB - 1) / RB, 1024);
     hipLaunchKernelGGL(k_merge_count, dim3(g), dim3(RB), 0, s, dst, src, nwords, count);
 }
 
@@ -1851,9 +1870,9 @@ void launch_classify(const int4 *rec, int64_t k, uint32_t kshift, uint64_t *st, 
                            inv_bm, out_act);
 }
 
-void launch_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats, GraphCtl *gc, hipStream_t s)
+void launch_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats, hipStream_t s)
 {
-    if (k > 0) hipLaunchKernelGGL(k_tomb, dim3(blocks_for(2 * k)), dim3(RB), 0, s, rec, k, g, bstats, gc);
+    if (k > 0) hipLaunchKernelGGL(k_tomb, dim3(blocks_for(2 * k)), dim3(RB), 0, s, rec, k, g, bstats);
 }
 
 void launch_ins_offer(const int4 *rec, int64_t k, uint32_t kshift, const uint32_t *tslot, GView g, uint64_t *st,

@@ -219,7 +219,8 @@ int dsssp_graph_info(const dsssp_graph *g, int64_t *n, int64_t *m_live, uint64_t
 int dsssp_graph_export(dsssp_graph *g, int64_t *m, int32_t *a, int32_t *b, int32_t *w);
 /* Upkeep state of the resident CSR.  base_entries + delta_entries is what
  * scans of every row read (live and tombstoned entries); dead_entries counts
- * the tombstoned ones since the last repack.  A repack (on the device, inside
+ * the tombstoned ones since the last repack (a copy that duplicate deletion
+ * records of one batch kill together counts twice).  A repack (on the device, inside
  * the next update's time, t_upkeep_ms) drops every tombstoned entry and
  * compacts the delta pool; it runs before a batch whose insertions would
  * outgrow the pool, and when tombstones exceed 1/8 of the scanned entries.
