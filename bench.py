@@ -171,7 +171,14 @@ def measured_traffic():
     return None
 
 
-def roofline(stats, traffic_applies=False):
+def gather_ceiling(ds):
+    """The device's rate of independent 8-byte reads at hashed positions of an
+    8 GiB buffer (beyond every cache): dsssp_probe kind 1, 2^26 reads."""
+    ms = ds.probe(1, 8 << 30, 1 << 26, reps=5)
+    return (1 << 26) / (ms / 1e3)
+
+
+def roofline(stats, traffic_applies=False, gather_rate=None):
     relax_bytes = sum(s["alg_bytes_relax"] for s in stats)
     relax_ms = sum(s["t_relax_kernel_ms"] for s in stats)
     launches = sum(s["relax_launches"] for s in stats)
@@ -185,6 +192,16 @@ def roofline(stats, traffic_applies=False):
     if traffic:
         out["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
         out["traffic_GBps"] = traffic * launches / (relax_ms / 1e3) / 1e9 if relax_ms > 0 else None
+    if gather_rate and relax_ms > 0:
+        # The kernel's accesses at unpredictable addresses: per frontier slot its
+        # vertex record and own state, per scanned entry the neighbour's state,
+        # per improvement the state atomicMin and the bitmap mark.
+        acc = sum(2 * s["frontier_vertices"] + s["relax_edges"] + 2 * s["relax_improved"] for s in stats)
+        rate = acc / (relax_ms / 1e3)
+        out["random_access"] = {"bound": "hbm random 8-byte accesses", "achieved": rate, "peak": gather_rate,
+                                "unit": "accesses/s", "frac": rate / gather_rate,
+                                "accesses_per_launch": acc / max(1, launches),
+                                "peak_source": "dsssp_probe kind 1: 2^26 reads at hashed positions of 8 GiB"}
     return out
 
 
@@ -204,7 +221,8 @@ def full_sweep(fs):
             "improved": fs["relax_improved"]}
 
 
-def line(args, value, wall, steps, cfg, stats, check, cpu, unit="changed edges/s", extra=None, traffic=False):
+def line(args, value, wall, steps, cfg, stats, check, cpu, unit="changed edges/s", extra=None, traffic=False,
+         gather_rate=None):
     import numpy as np
     lat = [s["t_update_ms"] for s in stats]
     alg = sum(s["alg_bytes"] for s in stats)
@@ -216,7 +234,7 @@ def line(args, value, wall, steps, cfg, stats, check, cpu, unit="changed edges/s
         "update_alg_GBps": alg / (sum(lat) / 1e3) / 1e9 if sum(lat) > 0 else None,
         "dist_bitexact": bool(check.get("gpu_recompute_equal", True) and check.get("full_sweep_fixed_point", True)
                               and check.get("dijkstra_mismatches", 0) == 0 and check.get("certificate", 0) == 0),
-        "check": check, "roofline": roofline(stats, traffic), "cpu_baseline": cpu,
+        "check": check, "roofline": roofline(stats, traffic, gather_rate), "cpu_baseline": cpu,
         "phases_ms": {ph: sum(s[f"t_{ph}_ms"] for s in stats) / max(1, len(stats))
                       for ph in ("upkeep", "insert", "classify", "invalidate", "seed", "relax")},
         "rounds": {"invalidate": sum(s["inv_rounds"] for s in stats) / max(1, len(stats)),
@@ -243,6 +261,8 @@ def timed_steps(ds, g, t, res, warmup, timing=True, warm=True):
 
 
 def run_plain(args, ds, np, n, a, b, w, src, batches, label, cfg, traffic=False, reruns=False):
+    gather_rate = gather_ceiling(ds)
+    log(f"random 8-byte gather ceiling {gather_rate:.3e} accesses/s")
     g = ds.Graph(n, a, b, w)
     t = ds.Tree(g, src)
     res = [ds.Batch(g, c) for c in batches]
@@ -316,7 +336,7 @@ def run_plain(args, ds, np, n, a, b, w, src, batches, label, cfg, traffic=False,
     if not args.no_cpu:
         c2, cpu = cpu_check(ds, np, g, n, [src], par, dist, k, label)
         check.update(c2)
-    line(args, value, wall, args.steps, cfg, stats, check, cpu, extra=extra, traffic=traffic)
+    line(args, value, wall, args.steps, cfg, stats, check, cpu, extra=extra, traffic=traffic, gather_rate=gather_rate)
 
 
 def config1(args, ds, np):
@@ -410,6 +430,8 @@ def config4a(args, ds, np):
     g0 = ds.Graph(n, a, b, w)
     plain_ms = 0.0
     same = True
+    forest_tuning = ds.get_tuning()
+    ds.set_tuning(use_graph=0)  # a fresh graph per tree: captures would only add instantiation time
     for j in range(K):
         gp = g0.clone()
         tp = ds.Tree(gp, int(srcs[j]))
@@ -422,6 +444,7 @@ def config4a(args, ds, np):
         tp.close()
         gp.close()
     g0.close()
+    ds.set_tuning(use_graph=forest_tuning["use_graph"])
     plain_per_tree = plain_ms / (K * args.steps)
     tree_updates = k * K * args.steps
     value = tree_updates / wall
@@ -474,10 +497,13 @@ def config4b(args, ds, np):
             stats.append(st)
     ds.set_tuning(kernel_timing=0)
     dev = sum(s["t_update_ms"] for s in stats) / 1e3
-    # plain: each graph separately, one launch sequence per graph
+    # plain: each graph separately, one launch sequence per graph (no captured
+    # round groups: on a fresh graph they would only add instantiation time)
     plain = 0.0
     par, dist = t.read()
     same = True
+    use_graph = ds.get_tuning()["use_graph"]
+    ds.set_tuning(use_graph=0)
     for n_i, a, b, w, c, s, o in parts:
         gp = ds.Graph(n_i, a, b, w)
         tp = ds.Tree(gp, s)
@@ -488,6 +514,7 @@ def config4b(args, ds, np):
         sp[sp >= 0] -= o
         same &= bool(np.array_equal(sp, pp) and np.array_equal(dist[o:o + n_i], pd))
         gp.close()
+    ds.set_tuning(use_graph=use_graph)
     value = len(C) * args.steps / dev
     log(f"{args.graphs} graphs in one launch: {value:.3e} changed edges/s; plain total {plain:.2f} ms vs "
         f"{dev / args.steps * 1e3:.2f} ms")

@@ -970,12 +970,14 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_
     }
 }
 
-// Sums the per-workgroup statistics into the counters and clears the slots.
+// Sums the per-workgroup statistics into the counters and clears the slots:
+// REDUCE_WG workgroups share the slots, one atomicAdd per statistic each.
+constexpr int REDUCE_WG = 8;
 __global__ __launch_bounds__(256) void k_reduce_stats(Counters *c, unsigned long long *bstats)
 {
     __shared__ unsigned long long part[256][NSTAT];
     unsigned long long acc[NSTAT] = {};
-    for (int i = threadIdx.x; i < STAT_SLOTS; i += 256) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < STAT_SLOTS; i += REDUCE_WG * 256) {
 #pragma unroll
         for (int k = 0; k < NSTAT; ++k) {
             acc[k] += bstats[i * NSTAT + k];
@@ -988,7 +990,7 @@ __global__ __launch_bounds__(256) void k_reduce_stats(Counters *c, unsigned long
     if (threadIdx.x < NSTAT) {
         unsigned long long s = 0;
         for (int i = 0; i < 256; ++i) s += part[i][threadIdx.x];
-        c->stats[threadIdx.x] += s;
+        if (s) atomicAdd(&c->stats[threadIdx.x], s);
     }
 }
 
@@ -1828,7 +1830,7 @@ void launch_gather_tree(const uint64_t *st, uint64_t n, uint32_t kshift, uint32_
 
 void launch_reduce_stats(Counters *c, unsigned long long *bstats, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_reduce_stats, dim3(1), dim3(256), 0, s, c, bstats);
+    hipLaunchKernelGGL(k_reduce_stats, dim3(REDUCE_WG), dim3(256), 0, s, c, bstats);
 }
 
 void launch_ins_place(const int4 *rec, int64_t k, GView g, uint32_t *tslot, uint64_t *tbm, hipStream_t s)

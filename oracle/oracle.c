@@ -9,6 +9,9 @@
  * Contents (each cites the reference file:line it restates):
  *   orc_build_csr          adjacency in file order     all_structure_undir.cuh:178-198,
  *                                                      SSSPmainForUndirected.cu:66-73
+ *   orc_build_csr_any      the same rows in an unspecified order, built by all
+ *                          threads (for the order-independent checks at 10^8+
+ *                          edges)
  *   orc_djk                the reference's sequential  seqSSSPwithDist_undir.cpp:66-128
  *                          Dijkstra (label-correcting
  *                          with the inQ flag)
@@ -24,13 +27,18 @@
  *   orc_certificate        O(n+m) exactness certificate for (dist, parent)
  *
  * How each function is pinned (tests/test_oracle.py):
+ *   orc_build_csr_any: row multisets equal to orc_build_csr's on random
+ *     multigraphs; Dijkstra, canonical parents and the certificate (planted
+ *     mistake included) give the same answers on both.
  *   orc_djk: reference fixture SSSP.txt (= op_ssspSeq on graph.txt, parents and
  *     distances), oracle/_ref/op_ssspSeq built from the reference sources on
  *     random graphs, and Bellman-Ford on random tiny graphs.
  *   orc_dijkstra_lazy, orc_bellman_ford: against each other and orc_djk; the
  *     closed-form path/star/grid distances.
- *   orc_apply_changes: the toy batch cE.txt (hand-derived result) and
- *     brute-force multiset arithmetic on random batches.
+ *   orc_apply_changes: the toy batch cE.txt (hand-derived result),
+ *     brute-force multiset arithmetic on random batches, and the exact output
+ *     order against a direct restatement on a 200,000-edge multigraph (its
+ *     filter runs on all threads and compacts in order).
  *   orc_ref_update: the toy known answer [0,4,6,6,4,9] and Dijkstra on the
  *     updated graph for random batches with distances below 999999.
  *   orc_canonical_parents: brute-force enumeration on tiny graphs.

@@ -611,3 +611,30 @@ def test_resident_csr_upkeep_bounded():
     finally:
         t.close()
         g.close()
+
+
+def test_round_loop_gives_up_cleanly():
+    """A loop of rounds still running past give_up_ms is abandoned between
+    round groups (every kernel has returned by then): the call fails with
+    DSSSP_ERR_STATE, and the library, the graph and later trees work as before.
+    Forced from the host with a 1 ms limit on a deep grid built from scratch."""
+    side = 400
+    a, b, w = ds.gen_grid(side, 0.01, 1, 10000, seed=33)
+    n = side * side
+    g = ds.Graph(n, a, b, w)
+    old = ds.get_tuning()
+    try:
+        ds.set_tuning(give_up_ms=1, delta=-1, local_tiles=-1, round_batch=1)
+        with pytest.raises(ds.DssspError) as e:
+            ds.Tree(g, 0)
+        assert e.value.code == ds.ERR_STATE
+    finally:
+        ds.set_tuning(**{k: old[k] for k in ("give_up_ms", "delta", "local_tiles", "round_batch")})
+    t = ds.Tree(g, 0)
+    chg = random_batch(np.random.default_rng(34), n, a, b, w, 500, 0.5, 1, 10000)
+    ds.update(g, t, chg)
+    mirror = Mirror(n, a, b, w)
+    mirror.apply(chg)
+    check_tree(mirror, *t.read(), [0], "after a give-up")
+    t.close()
+    g.close()

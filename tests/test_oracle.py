@@ -250,3 +250,54 @@ def test_certificate_rejects_planted_mistakes():
     # a zero-weight 2-cycle with consistent distances is caught by the walk
     csr0 = oracle.CSR.from_edges(3, [0, 1], [1, 2], [0, 0])
     assert oracle.certificate(csr0, [0, 0, 0], [0, 2, 1])[0] == 4
+
+
+def test_unordered_csr_builder_matches_ordered():
+    """The all-threads CSR builder (rows in an unspecified order) holds exactly
+    the file-order builder's rows as multisets, and every order-independent
+    check gives the same answers on it: Dijkstra distances, canonical parents,
+    the certificate (also on planted mistakes)."""
+    rng = np.random.default_rng(23)
+    for trial in range(4):
+        n = int(rng.integers(50, 3000))
+        a, b, w = random_multigraph(rng, n, int(rng.integers(n, 8 * n)), 0 if trial % 2 else 1, 50)
+        c1 = oracle.CSR.from_edges(n, a, b, w)
+        c2 = oracle.CSR.from_edges(n, a, b, w, ordered=False)
+        assert np.array_equal(c1.rp, c2.rp)
+        for v in range(n):
+            lo, hi = c1.rp[v], c1.rp[v + 1]
+            r1 = sorted(zip(c1.col[lo:hi].tolist(), c1.wt[lo:hi].tolist()))
+            r2 = sorted(zip(c2.col[lo:hi].tolist(), c2.wt[lo:hi].tolist()))
+            assert r1 == r2, (trial, v)
+        _, d1 = oracle.dijkstra(c1, [0])
+        _, d2 = oracle.dijkstra(c2, [0])
+        assert np.array_equal(d1, d2)
+        for rule in (0, 1):
+            p1 = oracle.canonical_parents(c1, d1, [0], rule=rule)
+            assert np.array_equal(p1, oracle.canonical_parents(c2, d1, [0], rule=rule))
+        p1 = oracle.canonical_parents(c1, d1, [0], rule=1)
+        assert oracle.certificate(c1, d1, p1) == oracle.certificate(c2, d1, p1) == (0, -1)
+        reach = np.flatnonzero((d1 != INF) & (np.arange(n) != 0))
+        if len(reach):
+            v = int(reach[-1])
+            d3 = d1.copy()
+            d3[v] += 1
+            assert oracle.certificate(c1, d3, p1) == oracle.certificate(c2, d3, p1) != (0, -1)
+    with pytest.raises(ValueError):
+        oracle.CSR.from_edges(4, [0, 5], [1, 2], [1, 1], ordered=False)
+
+
+def test_apply_changes_order_and_parallel_compaction():
+    """The edge filter runs on all threads and compacts in the original order:
+    survivors keep the graph's order, then the batch's insertions in batch
+    order (checked against a direct Python restatement on a large input)."""
+    rng = np.random.default_rng(29)
+    n = 400
+    a, b, w = random_multigraph(rng, n, 200_000, 1, 5)
+    chg = random_batch(rng, n, a, b, w, 3000, 0.4, 1, 5)
+    ua, ub, uw = oracle.apply_changes(n, a, b, w, chg)
+    dels = {(min(x, y), max(x, y), z) for x, y, z, op in chg.tolist() if op == 0}
+    keep = [(x, y, z) for x, y, z in zip(a.tolist(), b.tolist(), w.tolist())
+            if (min(x, y), max(x, y), z) not in dels]
+    keep += [(x, y, z) for x, y, z, op in chg.tolist() if op == 1 and (min(x, y), max(x, y), z) not in dels]
+    assert list(zip(ua.tolist(), ub.tolist(), uw.tolist())) == keep
