@@ -161,7 +161,8 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_frontier_relax_traffic_rmat_s24
 
 def measured_traffic():
     """HBM bytes per relaxation dispatch from the committed rocprofv3 PMC summary
-    (FETCH_SIZE and WRITE_SIZE runs of this workload, copy-calibrated)."""
+    (FETCH_SIZE and WRITE_SIZE runs of this workload, scaled by the factors that
+    profiles/probe_counter_calibration.md measured for random reads)."""
     try:
         for ln in open(TRAFFIC_FILE):
             if ln.startswith("machine-readable:"):
