@@ -358,6 +358,7 @@ void stop_lane(dsssp_server *s, Lane &L)
     }
     L.cv_work.notify_all();
     if (L.loop.joinable()) L.loop.join();
+    std::lock_guard<std::mutex> use(L.use);  // waits for a reader of the lane's tree
     if (L.t) dsssp_tree_destroy(L.t);
     if (L.g) dsssp_graph_destroy(L.g);
     L.t = nullptr;
@@ -692,6 +693,7 @@ extern "C" int dsssp_server_read(dsssp_server *s, int32_t id, int32_t *parent, i
     int rc;
     {
         std::lock_guard<std::mutex> use(L.use);
+        if (!L.t) return fail(DSSSP_ERR_STATE, "instance %d is not served", id);  // retired meanwhile
         rc = dsssp_tree_read(L.t, par.data(), d.data());
     }
     if (rc) return rc;
