@@ -69,7 +69,7 @@ def parse():
                    help="only run the library's probe launches of known footprint (a 1 GiB copy, 8-byte gathers "
                         "over 8 GiB and over 64 MiB) for calibrating HBM byte counters, then exit")
     for knob in ("block", "grid", "heavy_degree", "chunk", "round_batch", "use_graph", "edges_per_thread",
-                 "tile_entries", "local_tiles", "delta", "sweep"):
+                 "tile_entries", "local_tiles", "delta", "sweep", "rescue_degree", "rescue_levels"):
         p.add_argument("--" + knob.replace("_", "-"), type=int, default=None, help="tuning knob (dsssp_tuning)")
     return p.parse_args()
 
@@ -240,6 +240,10 @@ def line(args, value, wall, steps, cfg, stats, check, cpu, unit="changed edges/s
                       for ph in ("upkeep", "insert", "classify", "invalidate", "seed", "relax")},
         "rounds": {"invalidate": sum(s["inv_rounds"] for s in stats) / max(1, len(stats)),
                    "relax": sum(s["relax_rounds"] for s in stats) / max(1, len(stats))},
+        # per update: vertices invalidated (reset, then seeded), candidates kept by
+        # the rescue, entries scanned by relaxation and by invalidation + seeding
+        "work": {k: sum(s[k] for s in stats) / max(1, len(stats))
+                 for k in ("n_invalidated", "n_rescued", "relax_edges", "other_edges", "frontier_vertices")},
     }
     if extra:
         out.update(extra)
@@ -668,7 +672,8 @@ def main():
             log(f"probe kind {kind} bytes {nbytes} count {count}: {ms:.3f} ms per launch")
         return
     knobs = {k: getattr(args, k) for k in ("block", "grid", "heavy_degree", "chunk", "round_batch", "use_graph",
-                                           "edges_per_thread", "tile_entries", "local_tiles", "delta", "sweep")
+                                           "edges_per_thread", "tile_entries", "local_tiles", "delta", "sweep",
+                                           "rescue_degree", "rescue_levels")
              if getattr(args, k) is not None}
     if knobs:
         ds.set_tuning(**knobs)

@@ -50,7 +50,7 @@ class Stats(ctypes.Structure):
             "t_h2d_ms", "t_update_ms", "t_insert_ms", "t_classify_ms", "t_invalidate_ms", "t_seed_ms",
             "t_relax_ms", "t_relax_kernel_ms", "alg_bytes", "alg_bytes_relax")] + \
         [("relax_sweeps", ctypes.c_int64), ("t_sweep_ms", ctypes.c_double), ("alg_bytes_sweep", ctypes.c_double),
-         ("t_upkeep_ms", ctypes.c_double)]
+         ("t_upkeep_ms", ctypes.c_double), ("n_rescued", ctypes.c_int64)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -88,12 +88,14 @@ class ServerTotals(ctypes.Structure):
 
 
 DELTA_AUTO = -2  # Tuning.delta: ordered relaxation chosen per tree (include/dsssp.h)
+RESCUE_AUTO = -2  # Tuning.rescue_degree: invalidation rescue chosen per batch (include/dsssp.h)
 
 
 class Tuning(ctypes.Structure):
     _fields_ = [(f, ctypes.c_int32) for f in ("block", "grid", "heavy_degree", "chunk", "round_batch",
                                               "kernel_timing", "use_graph", "edges_per_thread",
-                                              "tile_entries", "local_tiles", "delta", "sweep", "give_up_ms")]
+                                              "tile_entries", "local_tiles", "delta", "sweep", "give_up_ms",
+                                              "rescue_degree", "rescue_levels")]
 
 
 def build(force: bool = True) -> None:

@@ -82,6 +82,7 @@ enum Stat : int {
     ST_SWEEPS,           // relaxation rounds run as full sweeps (every slot visited)
     ST_SWEEP_GATHERS,    // entries of an M_SWEEP pass that read their neighbour's state
     ST_KILLS,            // entries tombstoned by deletions
+    ST_RESCUED,          // invalidation candidates kept through another tight parent
     NSTAT
 };
 constexpr int STAT_SLOTS = 2048;
@@ -98,6 +99,9 @@ struct Counters {
     // Push-vs-full-sweep: slots a relaxation round marked in its output bitmap,
     // spread over 64 words (summed by the next round); round % 3.
     uint32_t nmark[3][64];
+    // Rescue of invalidation candidates: stage j of a loop reads cand[j & 1]
+    // (set when the step before it left a candidate) and clears the other.
+    uint32_t cand[2];
     uint32_t aux_hcnt;  // heavy chunks queued by a single-pass stage
     uint32_t err;
     unsigned long long stats[NSTAT];

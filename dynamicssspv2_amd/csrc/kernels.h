@@ -29,7 +29,12 @@ enum Mode : int {
     // part is a prefix): push (d[v]+w, v) to the neighbour, pull (d[u]+w, u)
     // into the owner.  Identity frontier; not with 64 trees per vertex.
     M_SWEEP = 7,
+    // INVAL that leaves rescue candidates (A.rescue_deg, A.cand_bm): a child
+    // with a long row is marked in cand_bm instead of being reset.  A mode of
+    // its own so that plain invalidation rounds keep their register count.
+    M_INVAL_RESCUE = 8,
 };
+constexpr bool is_inval(int mode) { return mode == M_INVAL || mode == M_INVAL_RESCUE; }
 
 // Arguments of one frontier launch.  The input frontier is a bitmap over
 // state slots (or every slot when identity is set, or in a full-sweep round)
@@ -52,6 +57,11 @@ struct FArgs {
     uint32_t *zero_hcnt;
     Counters *c;
     unsigned long long *bstats;  // STAT_SLOTS x NSTAT per-workgroup statistics
+    // INVAL with rescue_deg set: a child whose row has at least rescue_deg
+    // entries is not reset but marked in cand_bm (and *cand_flag set), to be
+    // kept by the rescue stage if another tight neighbour still holds it.
+    uint64_t *cand_bm;
+    uint32_t *cand_flag;
     uint32_t nslots;        // slots = n << kshift
     uint32_t nwords;        // words of a bitmap = ceil(nslots / 64)
     uint32_t hcap;          // capacity of each heavy-chunk list
@@ -80,7 +90,7 @@ struct FArgs {
     // marks into ring slot ring + 1 only when sweep is set.
     uint32_t sweep;
     uint32_t block;  // threads per workgroup: 128, 256 or 512 (the launch's grid is in workgroups of this size)
-    uint32_t pad_;
+    uint32_t rescue_deg;
 };
 
 constexpr int FRONTIER_BLOCK = 256;  // default workgroup size of frontier launches
@@ -122,8 +132,19 @@ void launch_ins_alloc(uint64_t *tbm, uint32_t nwords, const uint32_t *wsum, cons
 void launch_ins_pending(const int4 *rec, int64_t k, GView g, uint32_t *tslot, const GraphCtl *gc, hipStream_t s);
 // Deletions: the tree-edge test (marks the roots of invalidated subtrees) and
 // the tombstoning of every copy of each deleted entry.
+// With rescue_deg set, a victim whose row has at least rescue_deg entries is
+// marked in cand_bm (and *cand_flag set) instead of being reset.
 void launch_classify(const int4 *rec, int64_t k, uint32_t kshift, uint64_t *st, uint64_t *out_bm, uint64_t *inv_bm,
-                     uint32_t *out_act, hipStream_t s);
+                     uint32_t *out_act, const VRec *vr, uint32_t rescue_deg, uint64_t *cand_bm, uint32_t *cand_flag,
+                     hipStream_t s);
+// Rescue commit over the candidate bitmap (consumed): a candidate with a pick in
+// best (the smallest-id tight neighbour that was valid) keeps its distance with
+// that parent (best reset to ~0); the others are reset and marked in inv_bm and
+// out_bm (*out_act set).  Returns at once when *cand_flag is clear; clears
+// *clear_flag and *aux_hcnt for the next stage.
+void launch_rescue_commit(uint64_t *cand_bm, uint32_t nwords, uint64_t *st, uint32_t *best, uint64_t *inv_bm,
+                          uint64_t *out_bm, uint32_t *out_act, const uint32_t *cand_flag, uint32_t *clear_flag,
+                          uint32_t *aux_hcnt, unsigned long long *bstats, hipStream_t s);
 // Tombstones every copy of each deleted entry, found through the edge map; the
 // copies tombstoned are counted in the statistics (ST_KILLS).
 void launch_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats, hipStream_t s);

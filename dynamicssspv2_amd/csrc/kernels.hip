@@ -146,6 +146,7 @@ struct WaveStats {
     uint32_t fenc;
     uint32_t outm;  // wave-uniform: slots marked in the output bitmap (RELAX with sweep set)
     uint32_t gath;  // per lane: SWEEP entries that read their neighbour's state
+    bool cand;      // per lane: INVAL marked a rescue candidate
 };
 
 // Raises *p to v.  A relaxed load first skips the atomic when v is no record
@@ -183,7 +184,7 @@ __device__ __forceinline__ Pre gather(const FArgs &A, uint32_t os, uint64_t ent)
     if constexpr (MODE == M_SWEEP) {
         if (x > u) return p;  // relaxed from the row of x, the higher id
     }
-    if constexpr (MODE != M_INVAL) {
+    if constexpr (!is_inval(MODE)) {
         if (e_w(ent) & W_DEAD) return p;
     }
     const uint32_t xs = (x << A.kshift) | (os & ((1u << A.kshift) - 1u));
@@ -249,12 +250,23 @@ __device__ __forceinline__ uint32_t apply(const FArgs &A, uint32_t os, uint32_t 
         atomicMin((unsigned long long *)&A.st[xs], (unsigned long long)cand);
         S.marks++;
         return xs;
-    } else if constexpr (MODE == M_INVAL) {
+    } else if constexpr (is_inval(MODE)) {
         // x is a child of u in the pre-update tree: reset it and descend.
         // Tombstoned entries are scanned too, so a child hanging off a deleted
         // parallel copy is still found.  Only ST_INF is ever written in this
         // phase, so a plain store is race-free.
         if (p.sx == ST_INF || st_parent(p.sx) != u) return NO_SLOT;
+        if constexpr (MODE == M_INVAL_RESCUE) {
+            // A child with a long row keeps its state for now: the rescue stage
+            // after this round looks for another tight parent before its
+            // subtree is given up.
+            const VRec *r = &A.g.vr[x];
+            if (r->blen + (uint32_t)r->dbl >= A.rescue_deg) {
+                mark(A.cand_bm, xs);
+                S.cand = true;
+                return NO_SLOT;
+            }
+        }
         A.st[xs] = ST_INF;
         mark(A.inv_bm, xs);
         return xs;
@@ -475,7 +487,7 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
                 // continuation): the list can fill; the slot is then visited in
                 // the next round instead.  Pull modes visit each slot once.
                 void_items(A, base, tot);
-                if constexpr (MODE == M_RELAX || MODE == M_INVAL || MODE == M_TBFS) {
+                if constexpr (MODE == M_RELAX || is_inval(MODE) || MODE == M_TBFS) {
                     if (defer) {
                         mark(A.out_bm, os);
                         if constexpr (MODE == M_RELAX) {
@@ -559,7 +571,7 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
             if constexpr (is_pull(MODE)) {
                 if (pull != ~0ull) atomicMin(&sm.pull[oi[j]], (unsigned long long)pull);
             }
-            if constexpr (MODE == M_RELAX || MODE == M_INVAL || MODE == M_TBFS || MODE == M_SWEEP)
+            if constexpr (MODE == M_RELAX || is_inval(MODE) || MODE == M_TBFS || MODE == M_SWEEP)
                 visit_next<MODE>(A, tgt, td, sm.stk, top, S);
         }
     }
@@ -646,7 +658,7 @@ __device__ void do_forest(const FArgs &A, uint32_t v, uint64_t mask, uint32_t st
         base = __shfl(base, 0);
         if (base + nit > A.hcap) {
             void_items(A, base, nit);
-            if constexpr (MODE == M_RELAX || MODE == M_INVAL || MODE == M_TBFS) {
+            if constexpr (MODE == M_RELAX || is_inval(MODE) || MODE == M_TBFS) {
                 if (lane == 0) atomicOr((unsigned long long *)&A.out_bm[v], amask);
                 S.any = true;
                 if constexpr (MODE == M_RELAX) {
@@ -731,7 +743,7 @@ __device__ void do_forest(const FArgs &A, uint32_t v, uint64_t mask, uint32_t st
             } else if constexpr (MODE == M_OFCHECK) {
                 if (use && sx[j] == ST_INF) atomicOr(&A.c->err, ERR_DIST_OVERFLOW);
             }
-            if constexpr (MODE == M_RELAX || MODE == M_INVAL || MODE == M_TBFS) {
+            if constexpr (MODE == M_RELAX || is_inval(MODE) || MODE == M_TBFS) {
                 const unsigned long long m = __ballot(hit);
                 if (m) {
                     if (lane == 0) {
@@ -765,7 +777,7 @@ __device__ void do_forest(const FArgs &A, uint32_t v, uint64_t mask, uint32_t st
 template <int MODE, int EPT>
 __device__ void drain(const FArgs &A, WaveSmem &sm, WaveStats &S, uint32_t &top)
 {
-    if constexpr (MODE == M_RELAX || MODE == M_INVAL || MODE == M_TBFS) {
+    if constexpr (MODE == M_RELAX || is_inval(MODE) || MODE == M_TBFS) {
         const uint32_t lane = lane_id();
         uint32_t budget = A.local;
         while (top) {
@@ -817,7 +829,7 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_
     }
     // This round's threshold: the previous one, or, when the previous round
     // marked nothing below it, the smallest distance it marked plus delta.
-    WaveStats S{0, 0, 0, false, 0u, false, 0u, 0u, 0u};
+    WaveStats S{0, 0, 0, false, 0u, false, 0u, 0u, 0u, false};
     if constexpr (MODE == M_RELAX) {
         if (A.delta) {
             uint32_t fe = A.c->fenc[A.ring][lane];
@@ -946,6 +958,9 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_
                 if (fe) raise_u32(&A.c->fenc[o][blockIdx.x & 63u], fe);
             }
         }
+    }
+    if constexpr (MODE == M_INVAL_RESCUE) {
+        if (__ballot(S.cand) != 0 && lane == 0) *A.cand_flag = 1u;
     }
     if (lane == 0) {
         if (any && A.out_act) *A.out_act = 1u;
@@ -1214,12 +1229,14 @@ __global__ __launch_bounds__(RB) void k_tomb(const int4 *rec, int64_t k, GView g
 // the child endpoint of a deleted tree edge roots an invalidated subtree.
 // Thread i handles record i >> kshift for tree i & mask.
 __global__ __launch_bounds__(RB) void k_classify(const int4 *rec, int64_t k, uint32_t kshift, uint64_t *st,
-                                                 uint64_t *out_bm, uint64_t *inv_bm, uint32_t *out_act)
+                                                 uint64_t *out_bm, uint64_t *inv_bm, uint32_t *out_act,
+                                                 const VRec *vr, uint32_t rescue_deg, uint64_t *cand_bm,
+                                                 uint32_t *cand_flag)
 {
     const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
     const int64_t ri = i >> kshift;
     const uint32_t tree = (uint32_t)i & ((1u << kshift) - 1u);
-    bool hit = false;
+    bool hit = false, cand = false;
     if (ri < k) {
         const int4 r = rec[ri];
         const uint32_t a = (uint32_t)r.x, b = (uint32_t)r.y;
@@ -1234,14 +1251,67 @@ __global__ __launch_bounds__(RB) void k_classify(const int4 *rec, int64_t k, uin
                 if (va != ST_INF && st_parent(va) == b) victim = sa;
             }
             if (victim != 0xFFFFFFFFu) {
-                st[victim] = ST_INF;  // only ST_INF is written by this kernel
-                mark(out_bm, victim);
-                mark(inv_bm, victim);
-                hit = true;
+                const VRec *r = &vr[victim >> kshift];
+                if (rescue_deg && r->blen + (uint32_t)r->dbl >= rescue_deg) {
+                    mark(cand_bm, victim);  // kept for the rescue stage
+                    cand = true;
+                } else {
+                    st[victim] = ST_INF;  // only ST_INF is written by this kernel
+                    mark(out_bm, victim);
+                    mark(inv_bm, victim);
+                    hit = true;
+                }
             }
         }
     }
     if (__ballot(hit) != 0 && lane_id() == 0) *out_act = 1u;
+    if (__ballot(cand) != 0 && lane_id() == 0) *cand_flag = 1u;
+}
+
+// Rescue commit: every candidate slot of the bitmap either keeps its distance
+// under the parent the pick pass found (the smallest-id live neighbour y with
+// d[y] + w = d[slot] that was not reset) or is reset like any invalidated slot.
+// One thread per bitmap word.  A kept slot is exact with its new parent: that
+// parent's distance is strictly smaller (positive weights), so parents never
+// form a cycle, and if the parent is reset later its row scan finds the slot
+// again as its child.
+__global__ __launch_bounds__(RB) void k_rescue_commit(uint64_t *cand_bm, uint32_t nwords, uint64_t *st, uint32_t *best,
+                                                      uint64_t *inv_bm, uint64_t *out_bm, uint32_t *out_act,
+                                                      const uint32_t *cand_flag, uint32_t *clear_flag,
+                                                      uint32_t *aux_hcnt, unsigned long long *bstats)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *clear_flag = 0u;
+        *aux_hcnt = 0u;
+    }
+    if (*cand_flag == 0) return;
+    const uint64_t wi = (uint64_t)blockIdx.x * RB + threadIdx.x;
+    uint64_t word = 0;
+    if (wi < nwords) {
+        word = cand_bm[wi];
+        if (word) cand_bm[wi] = 0;
+    }
+    uint64_t reset = 0;
+    uint32_t kept = 0;
+    for (uint64_t m = word; m; m &= m - 1) {
+        const uint32_t slot = (uint32_t)(wi * 64 + (uint64_t)__builtin_ctzll(m));
+        const uint32_t p = best[slot];
+        if (p != 0xFFFFFFFFu) {
+            st[slot] = pack_state(st_dist(st[slot]), p);
+            best[slot] = 0xFFFFFFFFu;
+            ++kept;
+        } else {
+            st[slot] = ST_INF;
+            reset |= 1ull << (slot & 63u);
+        }
+    }
+    if (reset) {
+        atomicOr((unsigned long long *)&inv_bm[wi], (unsigned long long)reset);
+        atomicOr((unsigned long long *)&out_bm[wi], (unsigned long long)reset);
+    }
+    if (__ballot(reset != 0) != 0 && lane_id() == 0) *out_act = 1u;
+    const uint32_t k = wave_sum(kept);
+    if (lane_id() == 0) add_stat(bstats, ST_RESCUED, k);
 }
 
 // Offers every inserted edge that survived the batch's deletions in both
@@ -1719,6 +1789,7 @@ void launch_frontier_ept(int mode, const FArgs &a, int grid, hipStream_t s, hipE
     case M_PICK: launch_mode<M_PICK, EPT>(a, grid, s, start, stop); break;
     case M_OFCHECK: launch_mode<M_OFCHECK, EPT>(a, grid, s, start, stop); break;
     case M_SWEEP: launch_kernel<M_SWEEP, EPT, false>(a, grid, s, start, stop); break;  // never 64 trees
+    case M_INVAL_RESCUE: launch_kernel<M_INVAL_RESCUE, EPT, false>(a, grid, s, start, stop); break;  // nor this
     default: launch_mode<M_EMAP, EPT>(a, grid, s, start, stop); break;
     }
 }
@@ -1865,11 +1936,20 @@ void launch_ins_pending(const int4 *rec, int64_t k, GView g, uint32_t *tslot, co
 }
 
 void launch_classify(const int4 *rec, int64_t k, uint32_t kshift, uint64_t *st, uint64_t *out_bm, uint64_t *inv_bm,
-                     uint32_t *out_act, hipStream_t s)
+                     uint32_t *out_act, const VRec *vr, uint32_t rescue_deg, uint64_t *cand_bm, uint32_t *cand_flag,
+                     hipStream_t s)
 {
     if (k > 0)
         hipLaunchKernelGGL(k_classify, dim3(blocks_for(k << kshift)), dim3(RB), 0, s, rec, k, kshift, st, out_bm,
-                           inv_bm, out_act);
+                           inv_bm, out_act, vr, rescue_deg, cand_bm, cand_flag);
+}
+
+void launch_rescue_commit(uint64_t *cand_bm, uint32_t nwords, uint64_t *st, uint32_t *best, uint64_t *inv_bm,
+                          uint64_t *out_bm, uint32_t *out_act, const uint32_t *cand_flag, uint32_t *clear_flag,
+                          uint32_t *aux_hcnt, unsigned long long *bstats, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_rescue_commit, dim3(blocks_for((int64_t)nwords + 1)), dim3(RB), 0, s, cand_bm,
+                       nwords, st, best, inv_bm, out_bm, out_act, cand_flag, clear_flag, aux_hcnt, bstats);
 }
 
 void launch_tomb(const int4 *rec, int64_t k, GView g, unsigned long long *bstats, hipStream_t s)

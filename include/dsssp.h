@@ -101,6 +101,8 @@ typedef struct {
     double t_upkeep_ms;             /* part of t_update_ms: room in the delta
                                        pool, edge map load and item lists made
                                        before the batch's first kernel         */
+    int64_t n_rescued;              /* invalidation candidates kept under another
+                                       tight parent (their subtrees untouched)  */
 } dsssp_stats;
 
 /* Tuning knobs (zero fields keep the current value). */
@@ -137,9 +139,24 @@ typedef struct {
     int32_t give_up_ms;       /* a loop of rounds still running after this long is
                                  abandoned: the call scrubs the tree and returns
                                  DSSSP_ERR_STATE (default 120000)                     */
+    int32_t rescue_degree;    /* invalidation rescue: a vertex about to be reset
+                                 whose row has at least this many entries first
+                                 looks for another live neighbour that is tight for
+                                 its distance and still valid; if there is one it
+                                 keeps its distance under the smallest-id such
+                                 neighbour and its subtree is not invalidated.
+                                 -1: off.  DSSSP_RESCUE_AUTO (default): rows of at
+                                 least 1024 entries, for batches with at least n/8
+                                 deletions (each stage costs a few launches, which
+                                 smaller batches do not win back).  Not with zero
+                                 weights or 64 trees                                */
+    int32_t rescue_levels;    /* rescue stages per update: the deleted tree edges'
+                                 children, then the children found by the first
+                                 rescue_levels - 1 invalidation rounds (default 16) */
 } dsssp_tuning;
 
 #define DSSSP_DELTA_AUTO (-2)
+#define DSSSP_RESCUE_AUTO (-2)
 
 /* ---- library ---------------------------------------------------------- */
 const char *dsssp_last_error(void);
