@@ -91,6 +91,7 @@ constexpr int STAT_SLOTS = 2048;
 struct Counters {
     uint32_t act[3];    // a round marked its output bitmap; indexed by round % 3
     uint32_t hcnt[3];   // heavy chunks queued for the next round; round % 3
+    uint32_t worked;    // rounds of the current loop that had input (bits or chunks)
     // Distance-ordered relaxation (a threshold on the slots a round visits),
     // indexed by round % 3 like act:
     uint32_t near[3];      // a round marked a slot below its threshold

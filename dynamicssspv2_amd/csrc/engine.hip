@@ -104,6 +104,7 @@ inline uint32_t words_of(uint64_t bits) { return (uint32_t)((bits + 63) / 64); }
 
 constexpr uint32_t INIT_DCAP = 4;  // initial delta block capacity of every row
 constexpr int64_t LOCAL_MAX = 32;  // largest continuation budget of a round (tiles)
+constexpr int64_t ROUND_MARGIN = 1;  // rounds a loop's first group adds to the last loop's rounds with input
 // The full edge pass: rows longer than SWEEP_PIECE entries go to the item
 // launch in pieces of that size; both launches have (at least) as many
 // workgroups as the device holds at once at 6 waves per SIMD (24 waves per CU).
@@ -296,6 +297,7 @@ FArgs round_args(dsssp_tree *t, const FArgs &base, int64_t i)
     a.zero_act = &t->c->act[(ci + 2) % 3];
     a.zero_hcnt = &t->c->hcnt[(ci + 2) % 3];
     a.ring = (uint32_t)ci;
+    a.count_round = 1;
     // Deep updates (high-diameter graphs) run many rounds of small frontiers:
     // the continuation budget doubles every 16 rounds, up to LOCAL_MAX tiles.
     if (a.local) a.local = (uint32_t)std::min<int64_t>(LOCAL_MAX, (int64_t)a.local << std::min<int64_t>(i / 16, 8));
@@ -419,12 +421,15 @@ int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms, bool
     hipStream_t s = g->stream;
     const dsssp_tuning tu = tuning();
     const int max_group = 64;
-    // The first group covers about as many rounds as the last loop of this mode
-    // took, so that a loop as deep as the last one needs one convergence check;
-    // later groups start again from round_batch.  The estimate creeps down by
-    // one round per loop that converged within it.
+    // The first group covers as many rounds as the last loop of this mode had
+    // rounds with input, plus ROUND_MARGIN: a loop whose last round with input
+    // marks nothing is over after that round, so a loop as deep as the last
+    // one needs one convergence check and few empty rounds (each ~4 us).
+    // Later groups start again from round_batch.
     const int rb = std::max(1, tu.round_batch);
-    const int first = (int)std::max<int64_t>(rb, std::min<int64_t>(t->last_rounds[mode], max_group));
+    const int first = t->last_rounds[mode] > 0
+                          ? (int)std::min<int64_t>(t->last_rounds[mode] + ROUND_MARGIN, max_group)
+                          : rb;
     int group = first;
     FArgs base = base_args(t);
     const int grid = frontier_grid(t->nw, base.block);
@@ -487,9 +492,10 @@ int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms, bool
         rounds += group;
         HIP_TRY(hipGetLastError());
         const int ci = (int)(i % 3);  // flag / chunk slot written by the last round
-        // act[ci] .. hcnt[ci] in one copy (hcnt follows act in Counters)
-        static_assert(offsetof(Counters, hcnt) == 3 * sizeof(uint32_t), "hcnt follows act");
-        HIP_TRY(hipMemcpyAsync(&t->c_host->act[ci], &t->c->act[ci], 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        // act, hcnt and worked in one copy (they lead Counters)
+        static_assert(offsetof(Counters, hcnt) == 3 * sizeof(uint32_t) &&
+                          offsetof(Counters, worked) == 6 * sizeof(uint32_t), "act, hcnt, worked lead Counters");
+        HIP_TRY(hipMemcpyAsync(t->c_host, t->c, 7 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         if (timed) {
             for (int j = 0; j < group; ++j) {
@@ -507,7 +513,7 @@ int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms, bool
                         (long long)i);
         group = i == first && first < max_group ? rb : std::min(group * 2, max_group);
     }
-    t->last_rounds[mode] = i == first ? first - 1 : i;
+    t->last_rounds[mode] = t->c_host->worked;
     return DSSSP_OK;
 }
 

@@ -91,6 +91,8 @@ struct FArgs {
     uint32_t sweep;
     uint32_t block;  // threads per workgroup: 128, 256 or 512 (the launch's grid is in workgroups of this size)
     uint32_t rescue_deg;
+    uint32_t count_round;  // a round of a loop: counts itself in Counters::worked when it has input
+    uint32_t pad_;
 };
 
 constexpr int FRONTIER_BLOCK = 256;  // default workgroup size of frontier launches

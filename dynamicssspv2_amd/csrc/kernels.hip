@@ -889,6 +889,7 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_
     // A round after convergence has an empty bitmap and no items: it returns
     // without scanning further.
     if (H == 0 && A.in_act && *A.in_act == 0) return;
+    if (A.count_round && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&A.c->worked, 1u);
     uint32_t top = 0;  // slots on this wave's stack (wave-uniform)
     for (uint32_t pb = 0; pb < nblk; pb += nw * 8) {
         if (pb) word = load_word(pb, wi);
