@@ -30,6 +30,7 @@
 #include <cstddef>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -414,8 +415,13 @@ void rescue_stage(dsssp_tree *t, int64_t j);
 // after convergence reads an empty bitmap and is a no-op.  With rescue_levels
 // (invalidation only), rounds 0 .. rescue_levels - 2 leave rescue candidates
 // and each is followed by its rescue stage.
+// A tail, when given, is enqueued after every group of rounds in place of the
+// flag copy and must copy at least the leading act / hcnt / worked words of
+// the counters to the host (the caller's end-of-update copies, so that the
+// last convergence check and the update's final synchronisation are one).
 int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms, bool full_first = false,
-               bool no_alloc = false, int rescue_levels = 0, uint32_t rescue_deg = 0)
+               bool no_alloc = false, int rescue_levels = 0, uint32_t rescue_deg = 0,
+               const std::function<int()> *tail = nullptr)
 {
     dsssp_graph *g = t->g;
     hipStream_t s = g->stream;
@@ -495,7 +501,11 @@ int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms, bool
         // act, hcnt and worked in one copy (they lead Counters)
         static_assert(offsetof(Counters, hcnt) == 3 * sizeof(uint32_t) &&
                           offsetof(Counters, worked) == 6 * sizeof(uint32_t), "act, hcnt, worked lead Counters");
-        HIP_TRY(hipMemcpyAsync(t->c_host, t->c, 7 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        if (tail) {
+            if (int e = (*tail)()) return e;
+        } else {
+            HIP_TRY(hipMemcpyAsync(t->c_host, t->c, 7 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        }
         HIP_TRY(hipStreamSynchronize(s));
         if (timed) {
             for (int j = 0; j < group; ++j) {
@@ -944,16 +954,31 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
                              relax_delta(t) != 0, s);
     }
     HIP_TRY(hipEventRecord(g->ev[4], s));
-    // 5. relaxation
-    if (!code) code = run_rounds(t, M_RELAX, relax_rounds, &relax_kernel_ms, false, no_alloc);
-    HIP_TRY(hipEventRecord(g->ev[5], s));
-    // 6. zero-weight parent repair
-    if (!code && g->zero_w) code = repick_parents(t, true, repair_rounds, no_alloc);
-    HIP_TRY(hipEventRecord(g->ev[6], s));
-    launch_reduce_stats(t->c, t->bstats, s);
-    HIP_TRY(hipMemcpyAsync(t->c_host, t->c, sizeof(Counters), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(g->gc_host, g->gc, sizeof(GraphCtl), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    // 5. relaxation.  Without the zero-weight repair after it, the update's
+    // closing statistics and copies ride on each convergence check of the
+    // relaxation loop, so the last check is also the update's last wait.
+    auto closing = [&]() -> int {
+        HIP_TRY(hipEventRecord(g->ev[6], s));
+        launch_reduce_stats(t->c, t->bstats, s);
+        HIP_TRY(hipMemcpyAsync(t->c_host, t->c, sizeof(Counters), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(g->gc_host, g->gc, sizeof(GraphCtl), hipMemcpyDeviceToHost, s));
+        return DSSSP_OK;
+    };
+    const std::function<int()> relax_tail = [&]() -> int {
+        HIP_TRY(hipEventRecord(g->ev[5], s));
+        return closing();
+    };
+    const bool merged_close = !g->zero_w;
+    if (!code)
+        code = run_rounds(t, M_RELAX, relax_rounds, &relax_kernel_ms, false, no_alloc, 0, 0,
+                          merged_close ? &relax_tail : nullptr);
+    if (code || !merged_close) {
+        HIP_TRY(hipEventRecord(g->ev[5], s));
+        // 6. zero-weight parent repair
+        if (!code && g->zero_w) code = repick_parents(t, true, repair_rounds, no_alloc);
+        if (int e = closing()) return e;
+        HIP_TRY(hipStreamSynchronize(s));
+    }
     HIP_TRY(hipGetLastError());
     if (b->n_ins > 0) {
         g->pool_top = g->gc_host->pool_top;
