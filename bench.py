@@ -206,17 +206,27 @@ def roofline(stats, traffic_applies=False, gather_rate=None):
     return out
 
 
-def full_sweep(fs):
+def full_sweep(fs, n, m_live):
     """Roofline figures of the edge pass of dsssp_tree_relax_full: every live
     edge relaxed in both directions once, from the row of its higher-id
     endpoint (k_frontier<M_SWEEP>: a tile launch and an item launch, timed with
-    HIP events on the graph's stream).  Algorithmic bytes: 32 B per slot, 8 B
-    per scanned entry, 8 B more per entry that reads its neighbour's state,
-    16 B per improvement."""
+    HIP events on the graph's stream).
+
+    Two byte counts.  "frac" prices the pass as the plain relaxation sweep it
+    replaces: 16 B per vertex (row offset + own state) and 16 B per directed
+    entry (entry + neighbour state), n*16 + 2m*16 for m live undirected edges,
+    the per-sweep figure the path's measurement contract states.
+    "frac_bytes_read" counts what this pass itself reads and writes, because it
+    visits each edge once: 32 B per slot, 8 B per scanned entry, 8 B more per
+    entry that reads its neighbour's state, 16 B per improvement."""
     ms = fs["t_sweep_ms"]
-    ach = fs["alg_bytes_sweep"] / (ms / 1e3) / 1e9 if ms > 0 else None
+    ach_read = fs["alg_bytes_sweep"] / (ms / 1e3) / 1e9 if ms > 0 else None
+    sweep_bytes = 16.0 * n + 32.0 * m_live
+    ach = sweep_bytes / (ms / 1e3) / 1e9 if ms > 0 else None
     return {"kernel": "k_frontier<M_SWEEP> (each edge once, both directions)", "achieved": ach,
-            "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBPS if ach else None, "pass_ms": ms,
+            "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBPS if ach else None,
+            "sweep_alg_bytes": sweep_bytes, "achieved_bytes_read": ach_read,
+            "frac_bytes_read": ach_read / HBM_PEAK_GBPS if ach_read else None, "pass_ms": ms,
             "alg_bytes": fs["alg_bytes_sweep"], "slots": fs["frontier_vertices"], "entries_scanned": fs["relax_edges"],
             "rounds_after": fs["relax_rounds"], "rounds_kernel_ms": fs["t_relax_kernel_ms"] - ms,
             "improved": fs["relax_improved"]}
@@ -324,7 +334,7 @@ def run_plain(args, ds, np, n, a, b, w, src, batches, label, cfg, traffic=False,
     ds.set_tuning(kernel_timing=0 if args.no_timing else 1)
     fs = t.relax_full()
     ds.set_tuning(kernel_timing=0)
-    extra["full_sweep"] = full_sweep(fs)
+    extra["full_sweep"] = full_sweep(fs, n, g.info()["m_live"])
     log(f"{label}: full sweep {fs['t_sweep_ms']:.3f} ms, "
         f"{extra['full_sweep']['achieved']} GB/s algorithmic, improved {fs['relax_improved']}")
     par, dist = t.read()

@@ -528,6 +528,12 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // An item is a piece of one row (exactly one lane has entries): every entry
+    // belongs to lane ol, so no owner search, and a pull mode keeps its
+    // candidates in registers and reduces them over the wave once.
+    const bool single = !fresh;
+    const uint32_t ol = single ? (uint32_t)__builtin_ctzll(__ballot(deg != 0)) : 0u;
+    uint64_t acc = ~0ull;
     for (uint32_t eb = start; eb < end; eb += 64 * EPT) {
         uint64_t ent[EPT];
         uint32_t ows[EPT], a0[EPT], a1[EPT], oi[EPT], pos[EPT];
@@ -539,10 +545,12 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
             ent[j] = 0;
             ows[j] = a0[j] = a1[j] = oi[j] = pos[j] = 0;
             if (act[j]) {
-                uint32_t o = 0;
+                uint32_t o = ol;
+                if (!single) {
 #pragma unroll
-                for (uint32_t st = 32; st > 0; st >>= 1)
-                    if (sm.off[o + st] <= e) o += st;
+                    for (uint32_t st = 32; st > 0; st >>= 1)
+                        if (sm.off[o + st] <= e) o += st;
+                }
                 const uint32_t jj = e - sm.off[o];
                 const uint32_t bl = sm.blen[o];
                 oi[j] = o;
@@ -569,13 +577,23 @@ __device__ void do_slots(const FArgs &A, uint32_t os, bool valid, uint32_t start
             uint32_t td = 0;
             const uint32_t tgt = apply<MODE>(A, ows[j], a0[j], a1[j], ent[j], pre[j], S, pull, td);
             if constexpr (is_pull(MODE)) {
-                if (pull != ~0ull) atomicMin(&sm.pull[oi[j]], (unsigned long long)pull);
+                if (single)
+                    acc = min(acc, pull);
+                else if (pull != ~0ull)
+                    atomicMin(&sm.pull[oi[j]], (unsigned long long)pull);
             }
             if constexpr (MODE == M_RELAX || is_inval(MODE) || MODE == M_TBFS || MODE == M_SWEEP)
                 visit_next<MODE>(A, tgt, td, sm.stk, top, S);
         }
     }
     S.edges += end - start;
+    if constexpr (is_pull(MODE)) {
+        if (single) {
+#pragma unroll
+            for (int d = 32; d > 0; d >>= 1) acc = min(acc, (uint64_t)__shfl_xor((unsigned long long)acc, d));
+            if (lane == ol) sm.pull[ol] = acc;
+        }
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
