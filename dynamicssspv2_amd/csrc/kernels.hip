@@ -103,23 +103,29 @@ __device__ __forceinline__ uint32_t emap_kill(const GView &g, uint32_t row, uint
     const uint64_t h = emap_hash(row, col, w);
     const uint32_t fp = emap_fp(h);
     const uint64_t key = pack_entry(col, w);
+    // The first map slot depends only on the key: its load goes out with the
+    // vertex record's.
+    uint64_t s = h & g.emask;
+    uint64_t v = g.emap[s];
     const uint64_t b0 = g.vr[row].boff;
     const uint32_t blen = g.vr[row].blen;
     const uint64_t d = g.vr[row].dbl;
     const uint32_t dl = (uint32_t)d;
     uint32_t probes = 0;
-    for (uint64_t s = h & g.emask;; s = (s + 1) & g.emask) {
-        const uint64_t v = g.emap[s];
+    for (;;) {
         ++probes;
         if (v == ~0ull) break;
         const uint32_t pos = (uint32_t)v;
-        if ((uint32_t)(v >> 32) != fp || pos >= blen + dl) continue;
-        uint64_t *ep = pos < blen ? &g.adj[b0 + pos] : &g.pool[(d >> 32) + (pos - blen)];
-        const uint64_t e = *ep;
-        if (e_key(e) == key && !(e & E_DEAD)) {
-            *ep = e | E_DEAD;
-            ++kills;  // (duplicate records killing one copy together count it twice)
+        if ((uint32_t)(v >> 32) == fp && pos < blen + dl) {
+            uint64_t *ep = pos < blen ? &g.adj[b0 + pos] : &g.pool[(d >> 32) + (pos - blen)];
+            const uint64_t e = *ep;
+            if (e_key(e) == key && !(e & E_DEAD)) {
+                *ep = e | E_DEAD;
+                ++kills;  // (duplicate records killing one copy together count it twice)
+            }
         }
+        s = (s + 1) & g.emask;
+        v = g.emap[s];
     }
     const uint32_t ns = min(dl, EMAP_DSKIP);
     uint64_t *dp = &g.pool[d >> 32];
@@ -1261,14 +1267,12 @@ __global__ __launch_bounds__(RB) void k_classify(const int4 *rec, int64_t k, uin
         const uint32_t a = (uint32_t)r.x, b = (uint32_t)r.y;
         if (r.w == 0 && a != b) {
             const uint32_t sa = (a << kshift) | tree, sb = (b << kshift) | tree;
-            const uint64_t vb = st[sb];
+            const uint64_t vb = st[sb], va = st[sa];  // both at once: most deleted edges are not tree edges
             uint32_t victim = 0xFFFFFFFFu;
-            if (vb != ST_INF && st_parent(vb) == a) {
+            if (vb != ST_INF && st_parent(vb) == a)
                 victim = sb;
-            } else {
-                const uint64_t va = st[sa];
-                if (va != ST_INF && st_parent(va) == b) victim = sa;
-            }
+            else if (va != ST_INF && st_parent(va) == b)
+                victim = sa;
             if (victim != 0xFFFFFFFFu) {
                 const VRec *r = &vr[victim >> kshift];
                 if (rescue_deg && r->blen + (uint32_t)r->dbl >= rescue_deg) {
@@ -1344,16 +1348,17 @@ __global__ __launch_bounds__(RB) void k_ins_offer(const int4 *rec, int64_t k, ui
     const uint32_t tree = (uint32_t)i & ((1u << kshift) - 1u);
     bool hit = false;
     uint32_t fe = 0;  // ~(smallest distance marked)
-    if (ri < k) {
-        const int4 r = rec[ri];
+    const int4 r = ri < k ? rec[ri] : make_int4(0, 0, 0, 0);
+    if (r.w == 1 && r.x != r.y) {
+        const uint32_t a = (uint32_t)r.x, b = (uint32_t)r.y;
+        const uint32_t sa_slot = (a << kshift) | tree, sb_slot = (b << kshift) | tree;
+        const uint32_t w = (uint32_t)r.z;
+        // Both states are loaded together with the checks below (one dependent
+        // step less); the copy in row a is tombstoned when a deletion of the
+        // batch removed it.
+        const uint64_t sa = st[sa_slot], sb = st[sb_slot];
         const uint32_t ta = tslot[2 * ri];
-        // the copy in row a is tombstoned when a deletion of the batch removed it
-        if (r.w == 1 && r.x != r.y && !(ta & SLOT_PENDING) &&
-            !(g.pool[(g.vr[r.x].dbl >> 32) + ta] & E_DEAD)) {
-            const uint32_t a = (uint32_t)r.x, b = (uint32_t)r.y;
-            const uint32_t sa_slot = (a << kshift) | tree, sb_slot = (b << kshift) | tree;
-            const uint32_t w = (uint32_t)r.z;
-            const uint64_t sa = st[sa_slot], sb = st[sb_slot];
+        if (!(ta & SLOT_PENDING) && !(g.pool[(g.vr[a].dbl >> 32) + ta] & E_DEAD)) {
             const bool a_root = sa != ST_INF && st_parent(sa) == a;  // roots never change
             const bool b_root = sb != ST_INF && st_parent(sb) == b;
             if (sa != ST_INF && !b_root) {
