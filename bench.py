@@ -432,6 +432,7 @@ def config4a(args, ds, np):
     f = ds.Tree(g, sources=srcs)
     res = [ds.Batch(g, c) for c in batches]
     tuned = None
+    base_tuning = ds.get_tuning()  # the plain comparison runs with these, not the forest's tuned ones
     if args.autotune and args.warmup > 0:
         t0 = time.perf_counter()
         tuned, tms = ds.autotune(g, f, res[:args.warmup])
@@ -443,10 +444,11 @@ def config4a(args, ds, np):
     # the initial graph each), the same batches, device time per update
     fr = [f.read(j) for j in range(K)]
     g0 = ds.Graph(n, a, b, w)
-    plain_ms = 0.0
+    plain_ms = plain_upkeep_ms = 0.0
     same = True
     forest_tuning = ds.get_tuning()
-    ds.set_tuning(use_graph=0)  # a fresh graph per tree: captures would only add instantiation time
+    # a fresh graph per tree: captures would only add instantiation time
+    ds.set_tuning(**{**base_tuning, "use_graph": 0, "kernel_timing": 0})
     for j in range(K):
         gp = g0.clone()
         tp = ds.Tree(gp, int(srcs[j]))
@@ -454,12 +456,13 @@ def config4a(args, ds, np):
             st = ds.update(gp, tp, c)
             if i >= args.warmup:
                 plain_ms += st["t_update_ms"]
+                plain_upkeep_ms += st["t_upkeep_ms"]
         pp, pd = tp.read()
         same &= bool(np.array_equal(pp, fr[j][0]) and np.array_equal(pd, fr[j][1]))
         tp.close()
         gp.close()
     g0.close()
-    ds.set_tuning(use_graph=forest_tuning["use_graph"])
+    ds.set_tuning(**forest_tuning)
     plain_per_tree = plain_ms / (K * args.steps)
     tree_updates = k * K * args.steps
     value = tree_updates / wall
@@ -468,6 +471,12 @@ def config4a(args, ds, np):
          {"workload": f"rmat-s{scale} K={K} trees, shared batch k={k} ins=0.5", "n": n, "m": len(a), "k": k,
           "trees": K}, stats, {"every_tree_equals_plain": same}, None, unit="changed edges x trees /s",
          extra={"speedup_vs_plain": (plain_ms / 1e3) / wall, "plain_ms_per_step": plain_ms / args.steps,
+                # the same without the resident-CSR upkeep (a repack of each
+                # plain tree's own graph copy against one of the shared graph)
+                "speedup_vs_plain_excl_upkeep": ((plain_ms - plain_upkeep_ms) / 1e3) /
+                                                 (wall - sum(s["t_upkeep_ms"] for s in stats) / 1e3),
+                "plain_upkeep_ms_per_step": plain_upkeep_ms / args.steps,
+                "forest_upkeep_ms_per_step": sum(s["t_upkeep_ms"] for s in stats) / args.steps,
                 "autotuned": tuned,
                 "use_graph": ds.get_tuning()["use_graph"]})
 
@@ -494,6 +503,7 @@ def config4b(args, ds, np):
     g = g0.clone()
     t = ds.Tree(g, roots=roots)  # a tree on the clone; its state is restored per step
     tuned = None
+    base_tuning = ds.get_tuning()  # the plain comparison runs with these, not the merged run's tuned ones
     if args.autotune:
         bt = ds.Batch(g, C)
         tq = time.perf_counter()
@@ -517,8 +527,8 @@ def config4b(args, ds, np):
     plain = 0.0
     par, dist = t.read()
     same = True
-    use_graph = ds.get_tuning()["use_graph"]
-    ds.set_tuning(use_graph=0)
+    merged_tuning = ds.get_tuning()
+    ds.set_tuning(**{**base_tuning, "use_graph": 0, "kernel_timing": 0})
     for n_i, a, b, w, c, s, o in parts:
         gp = ds.Graph(n_i, a, b, w)
         tp = ds.Tree(gp, s)
@@ -529,7 +539,7 @@ def config4b(args, ds, np):
         sp[sp >= 0] -= o
         same &= bool(np.array_equal(sp, pp) and np.array_equal(dist[o:o + n_i], pd))
         gp.close()
-    ds.set_tuning(use_graph=use_graph)
+    ds.set_tuning(**merged_tuning)
     value = len(C) * args.steps / dev
     log(f"{args.graphs} graphs in one launch: {value:.3e} changed edges/s; plain total {plain:.2f} ms vs "
         f"{dev / args.steps * 1e3:.2f} ms")

@@ -1543,6 +1543,9 @@ static int tree_make(dsssp_graph *g, int32_t ntrees, const std::vector<uint32_t>
             s = scratch_passes(t, passes);
             if (!s) s = run_rounds(t, M_RELAX, rounds, nullptr);
             t->deep = rounds > DEEP_ROUNDS;
+            // A build's depth says nothing about an update's: updates start
+            // from round_batch and learn their own.
+            t->last_rounds[M_RELAX] = 0;
         }
         if (!s && g->zero_w) s = repick_parents(t, true, rounds);
     }
