@@ -55,7 +55,7 @@ def parse():
     p.add_argument("--trees", type=int, default=64)
     p.add_argument("--graphs", type=int, default=32)
     p.add_argument("--instances", type=int, default=16)
-    p.add_argument("--requests", type=int, default=400, help="served: requests per run (Poisson, then saturated)")
+    p.add_argument("--requests", type=int, default=1600, help="served: requests per run (Poisson, then saturated)")
     p.add_argument("--rate", type=float, default=5000.0,
                    help="served: Poisson arrivals per second of the first half of the requests (below capacity)")
     p.add_argument("--serve-scale", type=int, default=0, help="served: every instance at this R-MAT scale")
