@@ -571,13 +571,21 @@ int repick_parents(dsssp_tree *t, bool use_lvl, int64_t &rounds, bool no_alloc =
 bool rescue_ready(dsssp_tree *t, bool no_alloc)
 {
     hipStream_t s = t->g->stream;
+    // (an allocation that fails only leaves the rescue off: its message is dropped)
     if (!t->cb) {
-        if (no_alloc || dev_alloc((void **)&t->cb, sizeof(uint64_t) * t->nw, "rescue candidates")) return false;
+        if (no_alloc) return false;
+        if (dev_alloc((void **)&t->cb, sizeof(uint64_t) * t->nw, "rescue candidates")) {
+            clear_error();
+            return false;
+        }
         launch_fill_u64(t->cb, 0ull, t->nw, s);
     }
     if (!t->best) {
-        if (no_alloc || dev_alloc((void **)&t->best, sizeof(uint32_t) * (size_t)t->nslots, "parent picks"))
+        if (no_alloc) return false;
+        if (dev_alloc((void **)&t->best, sizeof(uint32_t) * (size_t)t->nslots, "parent picks")) {
+            clear_error();
             return false;
+        }
         t->best_clean = false;
     }
     if (!t->best_clean) {

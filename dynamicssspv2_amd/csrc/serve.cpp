@@ -85,13 +85,14 @@ uint64_t lane_rows(int64_t n, int64_t m) { return (uint64_t)m + 12 * (uint64_t)n
 constexpr uint64_t LANE_ROWS = 0x7FFFFFFFull - (1u << 20);
 
 // Device bytes of an instance: CSR, delta descriptors/capacities/counters and
-// bitmaps, the initial delta pool (E/2 + 12 n + 2^16 entries), the edge map, and the
-// tree (state, bitmaps, chunk lists).
+// bitmaps, the initial delta pool (E/2 + 12 n + 2^16 entries), the edge map, the
+// tree (state, bitmaps, chunk lists), and the buffers the tree allocates on
+// first need (parent picks and levels, rescue candidates).
 uint64_t instance_bytes(int64_t n, int64_t m)
 {
     const uint64_t E = 2 * (uint64_t)m, N = (uint64_t)n;
     return 8 * (N + 1) + 8 * E + 16 * N + 3 * N / 8 + 8 * (E / 2 + 12 * N + (1u << 16)) + 8 * emap_slots_for(E + E / 8) +
-           8 * N + 3 * N / 8 + 16 * (E / 128 + 1024);
+           8 * N + 3 * N / 8 + 16 * (E / 128 + 1024) + 8 * N + N / 8;
 }
 
 }  // namespace
