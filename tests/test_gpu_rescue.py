@@ -207,3 +207,38 @@ def test_rescue_auto_policy():
         assert np.array_equal(plain[0][0], auto[0][0]) and np.array_equal(plain[0][1], auto[0][1])
         ua, ub, uw = oracle.apply_changes(n, a, b, w, batches[0])
         exact(n, ua, ub, uw, auto[0][0], auto[0][1], [src], f"auto expect={expect}")
+
+
+def test_rescue_in_allocation_free_update():
+    """Once a tree holds the rescue buffers (an earlier update made them), the
+    allocation-free update rescues too, with the same result as the update
+    that uploads its records."""
+    from tests.test_zz_no_alloc import device_records
+    say("rescue in the allocation-free update")
+    scale = 12
+    n = 1 << scale
+    a, b, w = ds.gen_rmat(scale, 16, (0.57, 0.19, 0.19), 1, 20, seed=31)
+    src = int(np.argmax(np.bincount(np.concatenate([a, b]), minlength=n)))
+    rng = np.random.default_rng(37)
+    batches = make_batches(rng, n, a, b, w, 2, 3000, 20)
+    ds.set_tuning(rescue_degree=1, rescue_levels=16)
+    g1, g2 = ds.Graph(n, a, b, w), ds.Graph(n, a, b, w)
+    t1, t2 = ds.Tree(g1, src), ds.Tree(g2, src)
+    keep = []
+    try:
+        ds.update(g1, t1, batches[0])
+        ds.update(g2, t2, batches[0])  # allocates the rescue buffers of t2
+        s1 = ds.update(g1, t1, batches[1])
+        d = device_records(batches[1])
+        keep.append(d)
+        s2 = ds.update_noalloc(g2, t2, d.data_ptr(), len(batches[1]))
+        assert s2["n_rescued"] > 0 and s2["n_rescued"] == s1["n_rescued"]
+        p1, d1 = t1.read()
+        p2, d2 = t2.read()
+        assert np.array_equal(p1, p2) and np.array_equal(d1, d2)
+        ca, cb, cw = oracle.apply_changes(n, a, b, w, batches[0])
+        ca, cb, cw = oracle.apply_changes(n, ca, cb, cw, batches[1])
+        exact(n, ca, cb, cw, p2, d2, [src], "allocation-free with rescue")
+    finally:
+        for o in keep + [t1, t2, g1, g2]:
+            o.close()
