@@ -242,3 +242,46 @@ def test_rescue_in_allocation_free_update():
     finally:
         for o in keep + [t1, t2, g1, g2]:
             o.close()
+
+
+def test_rescue_random_stress():
+    """Random connected graphs and multigraphs (parallel edges, self-loops,
+    several components), single trees, forests and many-root trees, narrow
+    and wide weight ranges, sequences of mixed batches: every candidate offered
+    (degree 1, 2 or 4; 1 to 64 stages) against the rescue off and the oracle."""
+    from tests._util import random_multigraph
+    rng = np.random.default_rng(7)
+    rescued = 0
+    for case in range(40):
+        kind = str(rng.choice(["tree", "tree", "forest", "roots"]))
+        n = int(rng.integers(30, 2000))
+        whi = int(rng.choice([2, 5, 20, 1000]))
+        if rng.random() < 0.7:
+            a, b, w = random_connected(rng, n, int(rng.integers(n, 8 * n)), 1, whi)
+        else:
+            a, b, w = random_multigraph(rng, n, int(rng.integers(n, 8 * n)), 1, whi)
+        K = int(rng.integers(2, 9)) if kind != "tree" else 1
+        srcs = [int(x) for x in rng.choice(n, K, replace=False)]
+        batches, ca, cb, cw = [], a, b, w
+        for _ in range(int(rng.integers(1, 4))):
+            c = random_batch(rng, n, ca, cb, cw, int(rng.integers(1, max(2, len(ca) // 2))),
+                             float(rng.uniform(0, 0.6)), 1, whi)
+            batches.append(c)
+            ca, cb, cw = oracle.apply_changes(n, ca, cb, cw, c)
+        say(f"rescue stress case {case}: {kind} n={n} m={len(a)} K={K} w<={whi} batches={len(batches)}")
+        tun = {"rescue_degree": int(rng.choice([1, 2, 4])), "rescue_levels": int(rng.choice([1, 2, 3, 8, 64]))}
+        forest = kind == "forest"
+        off, _ = run_sequence(n, a, b, w, srcs, batches, {"rescue_degree": OFF}, forest=forest)
+        on, r = run_sequence(n, a, b, w, srcs, batches, tun, forest=forest)
+        rescued += r
+        ca, cb, cw = a, b, w
+        for i, c in enumerate(batches):
+            ca, cb, cw = oracle.apply_changes(n, ca, cb, cw, c)
+            if forest:
+                for j, s in enumerate(srcs):
+                    assert np.array_equal(off[i][j][0], on[i][j][0]) and np.array_equal(off[i][j][1], on[i][j][1])
+                    exact(n, ca, cb, cw, on[i][j][0], on[i][j][1], [s], f"case {case} batch {i} tree {j}")
+            else:
+                assert np.array_equal(off[i][0], on[i][0]) and np.array_equal(off[i][1], on[i][1]), (case, i)
+                exact(n, ca, cb, cw, on[i][0], on[i][1], srcs, f"case {case} batch {i}")
+    assert rescued > 0
