@@ -109,7 +109,10 @@ constexpr int64_t ROUND_MARGIN = 1;  // rounds a loop's first group adds to the 
 // The full edge pass: rows longer than SWEEP_PIECE entries go to the item
 // launch in pieces of that size; both launches have (at least) as many
 // workgroups as the device holds at once at 6 waves per SIMD (24 waves per CU).
-constexpr uint32_t SWEEP_PIECE = 512;
+// Measured on scale-24 R-MAT (pass ms): pieces of 256 3.77, 512 3.71, 768 3.56,
+// 1024 3.57, 1280 3.62, 1536 3.73, 2048 4.31; with pieces of 1024, a tile
+// budget of 512 entries 3.95, 1024 (the tuning default) 3.68, 2048 4.92.
+constexpr uint32_t SWEEP_PIECE = 1024;
 constexpr int SWEEP_WAVES_PER_CU = 24;
 // A tree built from scratch starts with full edge passes while a pass still
 // lowers at least 1/SCRATCH_PASS_SHARE of the slots (low-diameter graphs),

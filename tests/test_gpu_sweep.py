@@ -293,3 +293,37 @@ def test_relax_full_converges_upper_bounds(sweep):
     assert np.array_equal(dist, dref)
     assert st["relax_improved"] > 0
     assert oracle.certificate(csr, dist, par, [src])[0] == 0
+
+
+def test_edge_pass_pieces_of_long_rows():
+    """Rows with more entries below their own id than the pass's piece size
+    (1024) go to the pass's item launch in pieces: high-id vertices joined to
+    2500-5000 lower-id vertices.  A tree built from scratch (which starts with
+    edge passes) and a full pass over an upper-bound tree both give the
+    oracle's distances and canonical parents."""
+    rng = np.random.default_rng(2100)
+    n = 12000
+    a, b, w = random_connected(rng, n, 30000, 1, 100)
+    hubs = [n - 1, n - 2, n - 7]
+    ea, eb = [a], [b]
+    for h, cnt in zip(hubs, (5000, 3000, 2500)):
+        nb = rng.choice(h, cnt, replace=False).astype(np.int32)  # all below the hub's id
+        ea.append(np.full(cnt, h, np.int32))
+        eb.append(nb)
+    a = np.concatenate(ea)
+    b = np.concatenate(eb)
+    w = np.concatenate([w, rng.integers(1, 101, len(a) - len(w)).astype(np.int32)])
+    m = Mirror(n, a, b, w)
+    say("edge pass pieces: tree from scratch")
+    g = ds.Graph(n, a, b, w)
+    t = ds.Tree(g, 0)
+    check_tree(m, *t.read(), [0], "from scratch with long rows")
+    say("edge pass pieces: relax_full from upper bounds")
+    sub = np.arange(len(a)) < n - 1  # the spanning tree of random_connected
+    csr_sub = oracle.CSR.from_edges(n, a[sub], b[sub], w[sub])
+    _, dsub = oracle.dijkstra(csr_sub, [0])
+    psub = oracle.canonical_parents(csr_sub, dsub, [0], rule=0)
+    t2 = ds.Tree(g, 0, parent=psub, dist=dsub)
+    st = t2.relax_full()
+    assert st["relax_improved"] > 0
+    check_tree(m, *t2.read(), [0], "relax_full with long rows")
