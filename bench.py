@@ -237,6 +237,8 @@ def line(args, value, wall, steps, cfg, stats, check, cpu, unit="changed edges/s
     import numpy as np
     lat = [s["t_update_ms"] for s in stats]
     alg = sum(s["alg_bytes"] for s in stats)
+    phases = {ph: sum(s[f"t_{ph}_ms"] for s in stats) / max(1, len(stats))
+              for ph in ("upkeep", "insert", "classify", "invalidate", "seed", "relax")}
     out = {
         "metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": args.warmup,
         "ms_per_step": wall / max(1, steps) * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -246,8 +248,8 @@ def line(args, value, wall, steps, cfg, stats, check, cpu, unit="changed edges/s
         "dist_bitexact": bool(check.get("gpu_recompute_equal", True) and check.get("full_sweep_fixed_point", True)
                               and check.get("dijkstra_mismatches", 0) == 0 and check.get("certificate", 0) == 0),
         "check": check, "roofline": roofline(stats, traffic, gather_rate), "cpu_baseline": cpu,
-        "phases_ms": {ph: sum(s[f"t_{ph}_ms"] for s in stats) / max(1, len(stats))
-                      for ph in ("upkeep", "insert", "classify", "invalidate", "seed", "relax")},
+        # (null when the updates ran without the events between phases)
+        "phases_ms": phases if any(phases.values()) else None,
         "rounds": {"invalidate": sum(s["inv_rounds"] for s in stats) / max(1, len(stats)),
                    "relax": sum(s["relax_rounds"] for s in stats) / max(1, len(stats))},
         # per update: vertices invalidated (reset, then seeded), candidates kept by
@@ -289,43 +291,57 @@ def run_plain(args, ds, np, n, a, b, w, src, batches, label, cfg, traffic=False,
         extra["autotuned"] = tun
     for r in res[:args.warmup]:
         ds.update(g, t, r)
-    snap = None
-    if reruns:  # the state before the timed steps, to run the same batches again
-        snap = (g.clone(),)
-        snap = (snap[0], ds.Tree(snap[0], src))
-        snap[1].copy_from(t)
+    # the state before the timed steps, to run the same batches again
+    snap = (g.clone(),)
+    snap = (snap[0], ds.Tree(snap[0], src))
+    snap[1].copy_from(t)
     stats, wall = timed_steps(ds, g, t, res, args.warmup, not args.no_timing, warm=False)
     k = len(batches[-1])
     value = k * args.steps / wall
     log(f"{label}: {args.steps} steps in {wall:.3f}s -> {value:.3e} changed edges/s, "
         f"p50 {pct([s['t_update_ms'] for s in stats], 50):.3f} ms")
-    if snap:
+    final = (g.clone(),)
+    final = (final[0], ds.Tree(final[0], src))
+    final[1].copy_from(t)
+    same = []
+
+    def rerun(rs, timing=False, phases=False):
+        g.copy_from(snap[0])
+        t.copy_from(snap[1])
+        ds.set_tuning(phase_timing=1 if phases else 0)
+        st_, wall_ = timed_steps(ds, g, t, rs, args.warmup, timing, warm=False)
+        ds.set_tuning(phase_timing=0)
+        same.append(bool(np.array_equal(np.stack(t.read()), np.stack(final[1].read()))))
+        return st_, wall_
+
+    if reruns:
         # The same batches from the same state without per-launch timing
         # events (whose dispatch-side recording adds a few microseconds to
         # every launch), then from host memory through dsssp_update, which adds
         # the PCIe copy of each batch (t_h2d_ms, outside t_update_ms).
-        final = (g.clone(),)
-        final = (final[0], ds.Tree(final[0], src))
-        final[1].copy_from(t)
-        g.copy_from(snap[0])
-        t.copy_from(snap[1])
-        st2, wall2 = timed_steps(ds, g, t, res, args.warmup, False, warm=False)
+        st2, wall2 = rerun(res)
         extra["value_without_launch_events"] = k * args.steps / wall2
         extra["p50_ms_without_launch_events"] = pct([s["t_update_ms"] for s in st2], 50)
         log(f"{label}: without launch events {extra['value_without_launch_events']:.3e} changed edges/s")
-        g.copy_from(snap[0])
-        t.copy_from(snap[1])
-        st3, wall3 = timed_steps(ds, g, t, batches, args.warmup, False, warm=False)
+        st3, wall3 = rerun(batches)
         extra["value_pcie_inclusive"] = k * args.steps / wall3
         extra["t_h2d_ms"] = sum(s["t_h2d_ms"] for s in st3) / len(st3)
         log(f"{label}: from host memory {extra['value_pcie_inclusive']:.3e} changed edges/s, "
             f"H2D {extra['t_h2d_ms']:.3f} ms per batch")
-        eq = bool(np.array_equal(np.stack(t.read()), np.stack(final[1].read())))
-        extra["reruns_equal"] = eq
-        g.copy_from(final[0])
-        t.copy_from(final[1])
-        for o in (final[1], final[0], snap[1], snap[0]):
-            o.close()
+    # The phases of the same updates: once more with an event between each
+    # two phases (phase_timing; the timed steps above record only the start
+    # and end of each update, as the library does by default).
+    st4, wall4 = rerun(res, timing=not args.no_timing, phases=True)
+    extra["phases_ms"] = {ph: sum(s[f"t_{ph}_ms"] for s in st4) / len(st4)
+                          for ph in ("upkeep", "insert", "classify", "invalidate", "seed", "relax")}
+    extra["value_with_phase_events"] = k * args.steps / wall4
+    log(f"{label}: with phase events {extra['value_with_phase_events']:.3e} changed edges/s, "
+        f"phases {extra['phases_ms']}")
+    extra["reruns_equal"] = all(same)
+    g.copy_from(final[0])
+    t.copy_from(final[1])
+    for o in (final[1], final[0], snap[1], snap[0]):
+        o.close()
     for r in res:
         r.close()
     # One full relaxation sweep over the converged tree: every live entry of
@@ -396,6 +412,7 @@ def config3(args, ds, np):
     g = g0.clone()
     t = ds.Tree(g, parent=par0, dist=t0.read()[1])  # a tree on the clone; its state is restored per step
     stats = []
+    ds.set_tuning(phase_timing=1)  # a few us of events on updates of ~0.1 s
     for i, c in enumerate(batches):
         g.copy_from(g0)
         t.copy_from(t0)
@@ -405,7 +422,7 @@ def config3(args, ds, np):
         bt.close()
         if i >= args.warmup:
             stats.append(st)
-    ds.set_tuning(kernel_timing=0)
+    ds.set_tuning(kernel_timing=0, phase_timing=0)
     dev = sum(s["t_update_ms"] for s in stats) / 1e3
     value = k * args.steps / dev
     log(f"grid {side}: {value:.3e} changed edges/s (device time), p50 {pct([s['t_update_ms'] for s in stats], 50):.2f} ms"
@@ -437,6 +454,9 @@ def config4a(args, ds, np):
         t0 = time.perf_counter()
         tuned, tms = ds.autotune(g, f, res[:args.warmup])
         log(f"forest: autotuned in {time.perf_counter() - t0:.1f}s: {tuned}, {tms:.2f} ms per warm-up batch")
+    # upkeep is reported apart for the forest and the plain trees: both run
+    # with the events between phases
+    ds.set_tuning(phase_timing=1)
     stats, wall = timed_steps(ds, g, f, res, args.warmup, not args.no_timing)
     for r in res:
         r.close()
@@ -448,7 +468,7 @@ def config4a(args, ds, np):
     same = True
     forest_tuning = ds.get_tuning()
     # a fresh graph per tree: captures would only add instantiation time
-    ds.set_tuning(**{**base_tuning, "use_graph": 0, "kernel_timing": 0})
+    ds.set_tuning(**{**base_tuning, "use_graph": 0, "kernel_timing": 0, "phase_timing": 1})
     for j in range(K):
         gp = g0.clone()
         tp = ds.Tree(gp, int(srcs[j]))

@@ -95,7 +95,7 @@ class Tuning(ctypes.Structure):
     _fields_ = [(f, ctypes.c_int32) for f in ("block", "grid", "heavy_degree", "chunk", "round_batch",
                                               "kernel_timing", "use_graph", "edges_per_thread",
                                               "tile_entries", "local_tiles", "delta", "sweep", "give_up_ms",
-                                              "rescue_degree", "rescue_levels")]
+                                              "rescue_degree", "rescue_levels", "phase_timing")]
 
 
 def build(force: bool = True) -> None:

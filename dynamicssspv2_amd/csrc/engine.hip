@@ -45,7 +45,7 @@ using namespace dsssp;
 namespace {
 
 dsssp_tuning g_tuning = {FRONTIER_BLOCK, 1024, 256, 256, 4, 0, 0, 4, 1024, 2, DSSSP_DELTA_AUTO, -1, 120000,
-                         DSSSP_RESCUE_AUTO, 16};
+                         DSSSP_RESCUE_AUTO, 16, 0};
 std::mutex g_tuning_mu;
 thread_local int t_device = 0;
 
@@ -170,7 +170,9 @@ struct dsssp_graph {
     GraphCtl *gc_host = nullptr;  // pinned
     bool zero_w = false;          // some weight was ever 0: parents need the tight-BFS repair
     uint32_t auto_delta = 1;      // step of ordered relaxation: twice the mean input weight
-    // ev[0..6]: update phases; ev[7]: upkeep start; ev[8..9]: host batch copy
+    // ev[0..6]: update phases; ev[7]: upkeep start; ev[8..9]: host batch copy.
+    // Timing events only, created without the system-scope fence (no cache
+    // writeback and invalidation at each record); nothing waits on them.
     hipEvent_t ev[10] = {};
     std::vector<hipEvent_t> kev;  // per-launch timing events (kernel_timing)
     // Trees and batches attached to this graph.  dsssp_graph_destroy while any
@@ -487,7 +489,8 @@ int run_rounds(dsssp_tree *t, int mode, int64_t &rounds, double *kernel_ms, bool
             if (g->kev.size() < (size_t)(2 * group)) {
                 const size_t old = g->kev.size();
                 g->kev.resize((size_t)(2 * group));
-                for (size_t j = old; j < g->kev.size(); ++j) HIP_TRY(hipEventCreate(&g->kev[j]));
+                for (size_t j = old; j < g->kev.size(); ++j)
+                    HIP_TRY(hipEventCreateWithFlags(&g->kev[j], hipEventDisableSystemFence));
             }
             for (int j = 0; j < group; ++j) launch_round(i + j, g->kev[2 * j], g->kev[2 * j + 1]);
         } else if (graphs && !(full_first && i == 0) && !(rescue_deg && i + 1 < rescue_levels)) {
@@ -900,6 +903,11 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
 {
     DevGuard dg(g->device);
     hipStream_t s = g->stream;
+    const dsssp_tuning tu = tuning();
+    // ev[7] and ev[6] bracket the update; with phase_timing, ev[0] .. ev[5]
+    // mark the ends of its phases (each record holds the stream a few us).
+    const bool phases = tu.phase_timing != 0;
+    auto phase_mark = [&](int k) { return phases ? hipEventRecord(g->ev[k], s) : hipSuccess; };
     // Upkeep of the resident graph (pool room, edge map load, item lists) is
     // part of the update's time: ev[7] opens the window.
     HIP_TRY(hipEventRecord(g->ev[7], s));
@@ -912,7 +920,7 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
     int64_t inv_rounds = 0, relax_rounds = 0, repair_rounds = 0;
     double relax_kernel_ms = 0;
 
-    HIP_TRY(hipEventRecord(g->ev[0], s));
+    HIP_TRY(phase_mark(0));
     launch_fill_u32((uint32_t *)t->c, 0u, sizeof(Counters) / 4, s);
     // 1. insertions into the resident CSR
     if (b->n_ins > 0) {
@@ -925,13 +933,12 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
         g->emap_used += 2 * (uint64_t)b->n_ins;
         if (b->zero_w) g->zero_w = true;
     }
-    HIP_TRY(hipEventRecord(g->ev[1], s));
+    HIP_TRY(phase_mark(1));
     // 2-3. deletions: classify, then invalidate the subtrees.  With rescue, a
     // vertex with a long row is not reset at once: a rescue stage first looks
     // for another tight parent that still holds (positive weights only, so a
     // kept vertex's parent is strictly closer to the root; the tree-vector
     // layout of 64 trees resets as before).
-    const dsssp_tuning tu = tuning();
     uint32_t rdeg = 0;
     if (tu.rescue_degree > 0)
         rdeg = (uint32_t)tu.rescue_degree;
@@ -946,11 +953,11 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
                         &t->c->cand[0], s);
         tombstone(g, t, b);
     }
-    HIP_TRY(hipEventRecord(g->ev[2], s));
+    HIP_TRY(phase_mark(2));
     if (rescue) rescue_stage(t, 0);
     if (b->n_del > 0)
         code = run_rounds(t, M_INVAL, inv_rounds, nullptr, false, no_alloc, rescue ? tu.rescue_levels : 0, rdeg);
-    HIP_TRY(hipEventRecord(g->ev[3], s));
+    HIP_TRY(phase_mark(3));
     // 4. seed invalidated slots from valid neighbours, offer inserted edges
     if (!code) {
         zero_ring(t);
@@ -964,7 +971,7 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
             launch_ins_offer(b->rec, b->k, t->kshift, b->tslot, g->view(), t->st, t->fb[0], &t->c->act[0], t->c,
                              relax_delta(t) != 0, s);
     }
-    HIP_TRY(hipEventRecord(g->ev[4], s));
+    HIP_TRY(phase_mark(4));
     // 5. relaxation.  Without the zero-weight repair after it, the update's
     // closing statistics and copies ride on each convergence check of the
     // relaxation loop, so the last check is also the update's last wait.
@@ -976,7 +983,7 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
         return DSSSP_OK;
     };
     const std::function<int()> relax_tail = [&]() -> int {
-        HIP_TRY(hipEventRecord(g->ev[5], s));
+        HIP_TRY(phase_mark(5));
         return closing();
     };
     const bool merged_close = !g->zero_w;
@@ -984,7 +991,7 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
         code = run_rounds(t, M_RELAX, relax_rounds, &relax_kernel_ms, false, no_alloc, 0, 0,
                           merged_close ? &relax_tail : nullptr);
     if (code || !merged_close) {
-        HIP_TRY(hipEventRecord(g->ev[5], s));
+        HIP_TRY(phase_mark(5));
         // 6. zero-weight parent repair
         if (!code && g->zero_w) code = repick_parents(t, true, repair_rounds, no_alloc);
         if (int e = closing()) return e;
@@ -1019,12 +1026,14 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
         S.other_edges = (int64_t)c.stats[ST_OTHER_EDGES];
         S.probes = (int64_t)c.stats[ST_PROBES];
         S.t_h2d_ms = h2d_timed ? elapsed(g->ev[8], g->ev[9]) : 0.0;
-        S.t_upkeep_ms = elapsed(g->ev[7], g->ev[0]);
-        S.t_insert_ms = elapsed(g->ev[0], g->ev[1]);
-        S.t_classify_ms = elapsed(g->ev[1], g->ev[2]);
-        S.t_invalidate_ms = elapsed(g->ev[2], g->ev[3]);
-        S.t_seed_ms = elapsed(g->ev[3], g->ev[4]);
-        S.t_relax_ms = elapsed(g->ev[4], g->ev[5]) + elapsed(g->ev[5], g->ev[6]);
+        if (phases) {
+            S.t_upkeep_ms = elapsed(g->ev[7], g->ev[0]);
+            S.t_insert_ms = elapsed(g->ev[0], g->ev[1]);
+            S.t_classify_ms = elapsed(g->ev[1], g->ev[2]);
+            S.t_invalidate_ms = elapsed(g->ev[2], g->ev[3]);
+            S.t_seed_ms = elapsed(g->ev[3], g->ev[4]);
+            S.t_relax_ms = elapsed(g->ev[4], g->ev[5]) + elapsed(g->ev[5], g->ev[6]);
+        }
         S.t_update_ms = elapsed(g->ev[7], g->ev[6]);
         S.t_relax_kernel_ms = relax_kernel_ms;
         // Algorithmic bytes of relaxation: per frontier slot 16 B row bounds +
@@ -1118,10 +1127,12 @@ extern "C" int dsssp_set_tuning(const dsssp_tuning *t)
     if (t->rescue_levels) nt.rescue_levels = t->rescue_levels;
     nt.kernel_timing = t->kernel_timing;
     nt.use_graph = t->use_graph;
+    nt.phase_timing = t->phase_timing;
     if (nt.grid < 1 || nt.grid > 65536 || nt.heavy_degree < 64 || nt.chunk < 64 || nt.chunk % 64 ||
         nt.round_batch < 1 || nt.round_batch > 64 || nt.tile_entries < 64 || nt.local_tiles < -1 ||
         nt.local_tiles > 1024 || nt.delta < DSSSP_DELTA_AUTO || nt.sweep < -1 || nt.sweep > 1000 || nt.give_up_ms < 1 ||
         nt.rescue_degree < DSSSP_RESCUE_AUTO || nt.rescue_levels < 1 || nt.rescue_levels > 1024 ||
+        (nt.phase_timing != 0 && nt.phase_timing != 1) ||
         (nt.block != 128 && nt.block != 256 && nt.block != 512) ||
         (nt.edges_per_thread != 1 && nt.edges_per_thread != 2 && nt.edges_per_thread != 4))
         return fail(DSSSP_ERR_ARG, "tuning out of range");
@@ -1182,7 +1193,7 @@ static int graph_alloc(dsssp_graph *g, uint64_t pool_cap)
 {
     DevGuard dg(g->device);
     HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
-    for (auto &e : g->ev) HIP_TRY(hipEventCreate(&e));
+    for (auto &e : g->ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     const size_t n = (size_t)g->n;
     g->nw = words_of(n);
     if (int s = dev_alloc((void **)&g->vr, sizeof(VRec) * n, "vertex records")) return s;
@@ -2194,6 +2205,7 @@ extern "C" int dsssp_autotune(dsssp_graph *g, dsssp_tree *t, dsssp_batch *const 
     dsssp_tuning cur = start;
     cur.kernel_timing = 0;
     cur.use_graph = 0;
+    cur.phase_timing = 0;
     double cur_ms = 0;
     unsigned long long ref = 0;
     if (!s) s = dsssp_set_tuning(&cur);
@@ -2245,6 +2257,7 @@ extern "C" int dsssp_autotune(dsssp_graph *g, dsssp_tree *t, dsssp_batch *const 
     if (!s) {
         fin.kernel_timing = start.kernel_timing;
         fin.use_graph = start.use_graph;
+        fin.phase_timing = start.phase_timing;
     }
     (void)dsssp_set_tuning(&fin);
     if (s) return fail(s, "%s", keep.empty() ? last_error() : keep.c_str());

@@ -97,6 +97,11 @@ int main(int argc, char **argv)
     dsssp_batch *batch = nullptr;
     if (int s = dsssp_batch_upload(g, chg, k, &batch)) return die("uploading the change batch", s);
     dsssp_stats st{};
+    // The stage timers below need the events between the update's phases.
+    dsssp_tuning tu{};
+    if (int s = dsssp_get_tuning(&tu)) return die("reading the tuning", s);
+    tu.phase_timing = 1;
+    if (int s = dsssp_set_tuning(&tu)) return die("setting the tuning", s);
     if (int s = dsssp_update_batch(g, t, batch, &st)) return die("updating", s);
     const double step1 = st.t_insert_ms + st.t_classify_ms;
     printf("**Time taken for SSSP update STEP1: %.3f milliseconds**\n", step1);

@@ -74,8 +74,10 @@ typedef struct dsssp_batch dsssp_batch;  /* change batch resident in HBM        
 /* Per-update statistics.  Times are measured with HIP events on the graph's
  * stream.  t_update_ms spans "batch resident in HBM" to "converged": upkeep of
  * the resident graph, insertion into the resident CSR, deletion classification
- * and tombstoning, subtree invalidation, seeding and relaxation.  t_h2d_ms is
- * the host-to-device copy of a batch given as host records (dsssp_update),
+ * and tombstoning, subtree invalidation, seeding and relaxation.  The times of
+ * the phases (t_upkeep_ms, t_insert_ms .. t_relax_ms) are measured only with
+ * the phase_timing knob set, and read 0 otherwise.  t_h2d_ms is the
+ * host-to-device copy of a batch given as host records (dsssp_update),
  * outside t_update_ms. */
 typedef struct {
     int64_t n_insert, n_delete;     /* records with op 1 / op 0                */
@@ -153,6 +155,11 @@ typedef struct {
     int32_t rescue_levels;    /* rescue stages per update: the deleted tree edges'
                                  children, then the children found by the first
                                  rescue_levels - 1 invalidation rounds (default 16) */
+    int32_t phase_timing;     /* 1: events between an update's phases fill
+                                 t_upkeep_ms and t_insert_ms .. t_relax_ms (each
+                                 event holds the stream for a few microseconds,
+                                 about 2 % of a 1 ms update); 0 (default): only
+                                 t_update_ms is measured, the phase times read 0  */
 } dsssp_tuning;
 
 #define DSSSP_DELTA_AUTO (-2)

@@ -456,6 +456,53 @@ def test_hipgraph_rounds_match_eager():
     check_tree(m, *results[1][-1], [0], "hipgraph")
 
 
+def test_phase_timing_changes_only_the_timers():
+    """The events between an update's phases (phase_timing) fill the phase
+    times and nothing else: the same batches with and without them give the
+    same trees, the phase times are positive only with them and sum to at most
+    the update's time, and the update's own time is measured either way."""
+    a, b, w = ds.gen_rmat(12, 16, seed=41)
+    n = 1 << 12
+    rng = np.random.default_rng(41)
+    batches, ca, cb, cw = [], a, b, w
+    for i in range(4):
+        c = ds.gen_changes(n, ca, cb, cw, 1500, [0.0, 0.5, 1.0, 0.5][i], 1, 1000, seed=int(rng.integers(1 << 30)))
+        batches.append(c)
+        ca, cb, cw = oracle.apply_changes(n, ca, cb, cw, c)
+    old = ds.get_tuning()
+    phases = ("t_upkeep_ms", "t_insert_ms", "t_classify_ms", "t_invalidate_ms", "t_seed_ms", "t_relax_ms")
+    results = {}
+    try:
+        for pt in (0, 1):
+            ds.set_tuning(phase_timing=pt)
+            say(f"phase_timing={pt}")
+            g = ds.Graph(n, a, b, w)
+            t = ds.Tree(g, 0)
+            seq = []
+            for i, c in enumerate(batches):
+                st = ds.update(g, t, c)
+                assert st["t_update_ms"] > 0
+                if pt:
+                    assert st["t_relax_ms"] > 0 and st["t_insert_ms"] >= 0
+                    if i != 2:  # the batch of insertions only has no deletions to classify
+                        assert st["t_classify_ms"] > 0 and st["t_invalidate_ms"] > 0
+                    assert sum(st[f] for f in phases) <= st["t_update_ms"] * 1.001 + 1e-3
+                else:
+                    assert all(st[f] == 0 for f in phases)
+                seq.append(t.read())
+            results[pt] = seq
+            t.close()
+            g.close()
+    finally:
+        ds.set_tuning(**old)
+    for (p0, d0), (p1, d1) in zip(results[0], results[1]):
+        assert np.array_equal(p0, p1) and np.array_equal(d0, d1)
+    m = Mirror(n, a, b, w)
+    for c in batches:
+        m.apply(c)
+    check_tree(m, *results[1][-1], [0], "phase timing")
+
+
 def test_op_tools_sssp_matches_reference_dijkstra(tmp_path):
     """`op_tools sssp` (the op_ssspSeq role on the GPU) against the reference
     Dijkstra's own output on the golden graphs: same distances, exact integers."""

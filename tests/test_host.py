@@ -25,8 +25,8 @@ def test_library_exports_every_header_symbol():
 
 def test_tuning_knobs_validated():
     """Host-side checks of dsssp_set_tuning (no device call): the
-    push-vs-full-sweep threshold is -1 (never) or 1..1000 per mille, and a
-    rejected setting leaves the tuning as it was."""
+    push-vs-full-sweep threshold is -1 (never) or 1..1000 per mille, phase
+    timing 0 or 1, and a rejected setting leaves the tuning as it was."""
     old = ds.get_tuning()
     assert old["sweep"] == -1  # default: push rounds only
     try:
@@ -38,6 +38,16 @@ def test_tuning_knobs_validated():
             assert e.value.code == ds.ERR_ARG
             assert ds.get_tuning()["sweep"] == 250
         ds.set_tuning(sweep=-1)
+        assert ds.get_tuning() == old
+        # phase timing: 0 (default, only the update's start and end) or 1
+        assert old["phase_timing"] == 0
+        ds.set_tuning(phase_timing=1)
+        assert ds.get_tuning()["phase_timing"] == 1
+        for bad in (-1, 2):
+            with pytest.raises(ds.DssspError) as e:
+                ds.set_tuning(phase_timing=bad)
+            assert e.value.code == ds.ERR_ARG
+        ds.set_tuning(phase_timing=0)
         assert ds.get_tuning() == old
     finally:
         ds.set_tuning(**old)
