@@ -310,21 +310,25 @@ FArgs round_args(dsssp_tree *t, const FArgs &base, int64_t i)
     return a;
 }
 
-// Clears the round flags, chunk counts and threshold ring (act .. fenc) before
-// a producer.
+// Clears the round flags, chunk counts and threshold ring (act .. aux_hcnt)
+// before a producer.
 void zero_ring(dsssp_tree *t)
 {
-    launch_fill_u32(&t->c->act[0], 0u, offsetof(Counters, aux_hcnt) / sizeof(uint32_t), t->g->stream);
+    static_assert(offsetof(Counters, err) == offsetof(Counters, aux_hcnt) + sizeof(uint32_t), "aux_hcnt before err");
+    launch_fill_u32(&t->c->act[0], 0u, offsetof(Counters, err) / sizeof(uint32_t), t->g->stream);
 }
 
 // One pass of a single-pass mode (seed, parent pick) into bitmap 0 / flag 0.
 // Rows longer than the heavy threshold are queued as chunks on list 1 (idle
 // here) by the first launch and processed by a second, chunk-only launch.
-void single_pass(dsssp_tree *t, int mode, FArgs a, int min_grid = 0, uint64_t *out_bm = nullptr)
+// ring_clear: the caller has just cleared the ring (zero_ring), chunk count
+// included.
+void single_pass(dsssp_tree *t, int mode, FArgs a, int min_grid = 0, uint64_t *out_bm = nullptr,
+                 bool ring_clear = false)
 {
     hipStream_t s = t->g->stream;
     const int grid = std::max(frontier_grid(t->nw, a.block), min_grid);
-    launch_fill_u32(&t->c->aux_hcnt, 0u, 1, s);
+    if (!ring_clear) launch_fill_u32(&t->c->aux_hcnt, 0u, 1, s);
     a.ring = 2;  // output ring slot 0, read by round 0 of the loop that follows
     a.out_bm = out_bm ? out_bm : t->fb[0];
     a.out_act = &t->c->act[0];
@@ -564,7 +568,7 @@ int repick_parents(dsssp_tree *t, bool use_lvl, int64_t &rounds, bool no_alloc =
     a.consume = 0;
     a.use_lvl = use_lvl ? 1u : 0u;
     zero_ring(t);
-    single_pass(t, M_PICK, a);
+    single_pass(t, M_PICK, a, 0, nullptr, true);
     launch_apply_best(ns, t->kshift, t->st, t->best, t->c, g->stream);
     t->best_clean = false;
     HIP_TRY(hipGetLastError());
@@ -774,7 +778,7 @@ int confirm_overflow(dsssp_tree *t)
     a.identity = 1;
     a.consume = 0;
     zero_ring(t);
-    single_pass(t, M_OFCHECK, a);
+    single_pass(t, M_OFCHECK, a, 0, nullptr, true);
     launch_reduce_stats(t->c, t->bstats, s);
     HIP_TRY(hipMemcpyAsync(t->c_host, t->c, sizeof(Counters), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -921,10 +925,10 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
     double relax_kernel_ms = 0;
 
     HIP_TRY(phase_mark(0));
-    launch_fill_u32((uint32_t *)t->c, 0u, sizeof(Counters) / 4, s);
+    // the tree's counters (ring included) and the graph's insertion error word
+    launch_zero2_u32((uint32_t *)t->c, sizeof(Counters) / 4, &g->gc->err, b->n_ins > 0 ? 1 : 0, s);
     // 1. insertions into the resident CSR
     if (b->n_ins > 0) {
-        launch_fill_u32(&g->gc->err, 0u, 1, s);
         launch_ins_place(b->rec, b->k, g->view(), b->tslot, g->tbm, s);
         launch_ins_wordsum(g->tbm, g->nw, g->vr, g->wsum, g->bsum, s);
         launch_ins_scan(g->bsum, g->nw, g->bbase, g->pool_cap, g->gc, s);
@@ -948,7 +952,8 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
     if (!rescue) rdeg = 0;
     int code = DSSSP_OK;
     if (b->n_del > 0) {
-        zero_ring(t);
+        // (the ring is still clear from the start of the update: the insertion
+        // kernels touch only the graph)
         launch_classify(b->rec, b->k, t->kshift, t->st, t->fb[0], t->ib, &t->c->act[0], g->vr, rdeg, t->cb,
                         &t->c->cand[0], s);
         tombstone(g, t, b);
@@ -965,7 +970,7 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
             FArgs a = base_args(t);
             a.in_bm = t->ib;  // consumed, so clean for the next update
             a.delta = relax_delta(t);
-            single_pass(t, M_SEED, a);
+            single_pass(t, M_SEED, a, 0, nullptr, true);
         }
         if (b->n_ins > 0)
             launch_ins_offer(b->rec, b->k, t->kshift, b->tslot, g->view(), t->st, t->fb[0], &t->c->act[0], t->c,

@@ -103,6 +103,7 @@ void launch_frontier(int mode, const FArgs &a, int grid, hipStream_t s, hipEvent
                      hipEvent_t stop = nullptr);
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
+void launch_zero2_u32(uint32_t *p, uint64_t n, uint32_t *q, uint64_t m, hipStream_t s);
 void launch_reduce_stats(Counters *c, unsigned long long *bstats, hipStream_t s);
 // Checks k records in device memory: cnt[0..2] = insertions, deletions,
 // zero-weight insertions; cnt[3] = smallest index of an invalid record (~0: none).

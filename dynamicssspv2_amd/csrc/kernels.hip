@@ -1647,6 +1647,17 @@ __global__ __launch_bounds__(RB) void k_fill_u32(uint32_t *p, uint32_t v, uint64
     for (uint64_t i = (uint64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * RB) p[i] = v;
 }
 
+// Zeroes p[0, n) and q[0, m) in one launch.
+__global__ __launch_bounds__(RB) void k_zero2_u32(uint32_t *p, uint64_t n, uint32_t *q, uint64_t m)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * RB + threadIdx.x; i < n + m; i += (uint64_t)gridDim.x * RB) {
+        if (i < n)
+            p[i] = 0u;
+        else
+            q[i - n] = 0u;
+    }
+}
+
 // Device-side check of k change records (the host check of dsssp_update,
 // for records already in device memory): cnt[0] insertions, cnt[1] deletions,
 // cnt[2] zero-weight insertions, cnt[3] the smallest index of a record with
@@ -1845,6 +1856,13 @@ void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s)
     if (n == 0) return;
     const unsigned g = (unsigned)std::min<uint64_t>((n + RB - 1) / RB, 8192);
     hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(RB), 0, s, p, v, n);
+}
+
+void launch_zero2_u32(uint32_t *p, uint64_t n, uint32_t *q, uint64_t m, hipStream_t s)
+{
+    if (n + m == 0) return;
+    const unsigned g = (unsigned)std::min<uint64_t>((n + m + RB - 1) / RB, 8192);
+    hipLaunchKernelGGL(k_zero2_u32, dim3(g), dim3(RB), 0, s, p, n, q, m);
 }
 
 void launch_digest(const uint64_t *p, uint64_t n, unsigned long long *out, hipStream_t s)
