@@ -11,8 +11,8 @@
 //          emap uint64[2^k]   edge map: per live directed entry, a 32-bit key
 //                             fingerprint and its position in its row
 //          tbm  uint64[n/64]  rows whose delta block must grow in this batch
-//          wsum/bsum/bbase    pool space per bitmap word / per 256 words, and the
-//                             pool offset of each 256-word group, for one batch
+//          wsum/bsum          pool space per bitmap word / per 256 words, for
+//                             one batch
 //   tree   st   uint64[S]     packed state (distance << 32 | parent), S = n * K'
 //                             slots (K' = trees rounded up to a power of two)
 //          fb   2 x uint64[S/64]  frontier bitmaps (ping-pong between rounds)
@@ -107,6 +107,12 @@ struct Counters {
     uint32_t err;
     unsigned long long stats[NSTAT];
     unsigned long long popc;  // set bits counted by the last bitmap merge
+    // The graph's words an update's caller reads (GraphCtl::pool_top, err),
+    // copied here by the statistics reduction, so that one copy to the host
+    // brings the results of both.
+    unsigned long long g_pool_top;
+    uint32_t g_err;
+    uint32_t g_pad;
 };
 
 // Device-side control block of one graph (delta pool allocation, upkeep).

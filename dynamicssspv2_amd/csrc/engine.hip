@@ -165,7 +165,6 @@ struct dsssp_graph {
     uint64_t emap_used = 0;    // entries entered since the map was built (dead ones too)
     uint32_t *wsum = nullptr;   // per bitmap word: pool entries needed by the batch
     uint64_t *bsum = nullptr;   // per 256 words: sum of wsum
-    uint64_t *bbase = nullptr;  // per 256 words: pool offset
     GraphCtl *gc = nullptr;
     GraphCtl *gc_host = nullptr;  // pinned
     bool zero_w = false;          // some weight was ever 0: parents need the tight-BFS repair
@@ -931,8 +930,7 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
     if (b->n_ins > 0) {
         launch_ins_place(b->rec, b->k, g->view(), b->tslot, g->tbm, s);
         launch_ins_wordsum(g->tbm, g->nw, g->vr, g->wsum, g->bsum, s);
-        launch_ins_scan(g->bsum, g->nw, g->bbase, g->pool_cap, g->gc, s);
-        launch_ins_alloc(g->tbm, g->nw, g->wsum, g->bbase, g->vr, g->pool, g->gc, s);
+        launch_ins_alloc(g->tbm, g->nw, g->wsum, g->bsum, g->vr, g->pool, g->pool_cap, g->gc, s);
         launch_ins_pending(b->rec, b->k, g->view(), b->tslot, g->gc, s);
         g->emap_used += 2 * (uint64_t)b->n_ins;
         if (b->zero_w) g->zero_w = true;
@@ -982,9 +980,8 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
     // relaxation loop, so the last check is also the update's last wait.
     auto closing = [&]() -> int {
         HIP_TRY(hipEventRecord(g->ev[6], s));
-        launch_reduce_stats(t->c, t->bstats, s);
+        launch_reduce_stats(t->c, t->bstats, s, g->gc);  // the graph's pool top and error word too
         HIP_TRY(hipMemcpyAsync(t->c_host, t->c, sizeof(Counters), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipMemcpyAsync(g->gc_host, g->gc, sizeof(GraphCtl), hipMemcpyDeviceToHost, s));
         return DSSSP_OK;
     };
     const std::function<int()> relax_tail = [&]() -> int {
@@ -1004,12 +1001,12 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
     }
     HIP_TRY(hipGetLastError());
     if (b->n_ins > 0) {
-        g->pool_top = g->gc_host->pool_top;
+        g->pool_top = t->c_host->g_pool_top;
         g->delta_entries += 2 * (uint64_t)b->n_ins;
     }
     g->dead_entries += t->c_host->stats[ST_KILLS];
     if (!code) code = confirm_overflow(t);
-    if (!code) code = finish_errors(t, b->n_ins > 0 ? g->gc_host->err : 0u);
+    if (!code) code = finish_errors(t, b->n_ins > 0 ? t->c_host->g_err : 0u);
     if (code) {
         const std::string keep = last_error();
         scrub(t);
@@ -1181,7 +1178,6 @@ static void graph_release(dsssp_graph *g)
     if (g->stage) (void)hipHostFree(g->stage);
     dev_free(g->wsum);
     dev_free(g->bsum);
-    dev_free(g->bbase);
     dev_free(g->gc);
     dev_free(g->mkeys);
     dev_free(g->mreq);
@@ -1207,7 +1203,6 @@ static int graph_alloc(dsssp_graph *g, uint64_t pool_cap)
     if (int s = dev_alloc((void **)&g->wsum, sizeof(uint32_t) * g->nw, "word sizes")) return s;
     const size_t nb = (g->nw + 255) / 256;
     if (int s = dev_alloc((void **)&g->bsum, sizeof(uint64_t) * nb, "group sizes")) return s;
-    if (int s = dev_alloc((void **)&g->bbase, sizeof(uint64_t) * nb, "group bases")) return s;
     if (int s = dev_alloc((void **)&g->gc, sizeof(GraphCtl), "graph control")) return s;
     g->pool_cap = pool_cap;
     if (int s = dev_alloc((void **)&g->pool, sizeof(uint64_t) * pool_cap, "delta pool")) return s;

@@ -104,7 +104,9 @@ void launch_frontier(int mode, const FArgs &a, int grid, hipStream_t s, hipEvent
 void launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 void launch_zero2_u32(uint32_t *p, uint64_t n, uint32_t *q, uint64_t m, hipStream_t s);
-void launch_reduce_stats(Counters *c, unsigned long long *bstats, hipStream_t s);
+// Sums the per-workgroup statistics into c->stats; with gc, also copies the
+// graph's pool_top and err into c->g_pool_top / g_err.
+void launch_reduce_stats(Counters *c, unsigned long long *bstats, hipStream_t s, const GraphCtl *gc = nullptr);
 // Checks k records in device memory: cnt[0..2] = insertions, deletions,
 // zero-weight insertions; cnt[3] = smallest index of an invalid record (~0: none).
 void launch_validate(const int4 *rec, int64_t k, int64_t n, unsigned long long *cnt, hipStream_t s);
@@ -122,17 +124,16 @@ void launch_digest(const uint64_t *p, uint64_t n, unsigned long long *out, hipSt
 // Insertion of a batch into the delta blocks.  Thread 2i+s places copy s of
 // record i (tslot[2i+s] = its position in the delta block, bit 31 set while
 // pending); rows that overflow are marked in tbm and grown: pool space per
-// bitmap word (wsum) and per 256-word group (bsum), the scan of the group sums
-// (bbase, pool top), the block moves, then the pending copies.
+// bitmap word (wsum) and per 256-word group (bsum), the block moves (each
+// group's base from the pool top and the sums before it), then the pending
+// copies (and the pool top's advance).
 constexpr uint32_t SLOT_PENDING = 0x80000000u;
 void launch_ins_place(const int4 *rec, int64_t k, GView g, uint32_t *tslot, uint64_t *tbm, hipStream_t s);
 void launch_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const VRec *vr, uint32_t *wsum, uint64_t *bsum,
                         hipStream_t s);
-void launch_ins_scan(const uint64_t *bsum, uint32_t nwords, uint64_t *bbase, uint64_t pool_cap, GraphCtl *gc,
-                     hipStream_t s);
-void launch_ins_alloc(uint64_t *tbm, uint32_t nwords, const uint32_t *wsum, const uint64_t *bbase, VRec *vr,
-                      uint64_t *pool, const GraphCtl *gc, hipStream_t s);
-void launch_ins_pending(const int4 *rec, int64_t k, GView g, uint32_t *tslot, const GraphCtl *gc, hipStream_t s);
+void launch_ins_alloc(uint64_t *tbm, uint32_t nwords, const uint32_t *wsum, const uint64_t *bsum, VRec *vr,
+                      uint64_t *pool, uint64_t pool_cap, GraphCtl *gc, hipStream_t s);
+void launch_ins_pending(const int4 *rec, int64_t k, GView g, uint32_t *tslot, GraphCtl *gc, hipStream_t s);
 // Deletions: the tree-edge test (marks the roots of invalidated subtrees) and
 // the tombstoning of every copy of each deleted entry.
 // With rescue_deg set, a victim whose row has at least rescue_deg entries is

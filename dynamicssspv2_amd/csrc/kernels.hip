@@ -1013,8 +1013,12 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(6))) void k_
 // Sums the per-workgroup statistics into the counters and clears the slots:
 // REDUCE_WG workgroups share the slots, one atomicAdd per statistic each.
 constexpr int REDUCE_WG = 8;
-__global__ __launch_bounds__(256) void k_reduce_stats(Counters *c, unsigned long long *bstats)
+__global__ __launch_bounds__(256) void k_reduce_stats(Counters *c, unsigned long long *bstats, const GraphCtl *gc)
 {
+    if (gc && blockIdx.x == 0 && threadIdx.x == 0) {
+        c->g_pool_top = gc->pool_top;
+        c->g_err = gc->err;
+    }
     __shared__ unsigned long long part[256][NSTAT];
     unsigned long long acc[NSTAT] = {};
     for (int i = blockIdx.x * 256 + threadIdx.x; i < STAT_SLOTS; i += REDUCE_WG * 256) {
@@ -1145,56 +1149,46 @@ __global__ __launch_bounds__(RB) void k_ins_wordsum(const uint64_t *tbm, uint32_
     if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
-// Single-workgroup exclusive scan of the nb workgroup sums into pool offsets
-// bbase (from the current pool top); advances the pool top.
-__global__ __launch_bounds__(1024) void k_ins_scan(const uint64_t *bsum, uint32_t nb, uint64_t *bbase,
-                                                   uint64_t pool_cap, GraphCtl *gc)
-{
-    __shared__ unsigned long long part[1024];
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (nb + 1023) / 1024;
-    const uint32_t lo = min(t * per, nb), hi = min(lo + per, nb);
-    unsigned long long s = 0;
-    for (uint32_t i = lo; i < hi; ++i) s += bsum[i];
-    part[t] = s;
-    __syncthreads();
-    for (uint32_t d = 1; d < 1024; d <<= 1) {
-        const unsigned long long y = t >= d ? part[t - d] : 0ull;
-        __syncthreads();
-        part[t] += y;
-        __syncthreads();
-    }
-    const unsigned long long top = gc->pool_top;
-    unsigned long long run = top + part[t] - s;
-    for (uint32_t i = lo; i < hi; ++i) {
-        bbase[i] = run;
-        run += bsum[i];
-    }
-    if (t == 1023) {
-        const unsigned long long total = part[1023];
-        gc->batch_new = total;
-        if (top + total > pool_cap)
-            gc->err |= ERR_POOL;
-        else
-            gc->pool_top = top + total;
-    }
-}
-
 // Moves each marked row's block (its entries written so far) to a new block;
 // the length already counts the pending copies.  Clears the marks.  Word w's
-// new blocks start at its workgroup's base plus the workgroup-local prefix of
-// wsum.
+// new blocks start at the pool top, plus the sums of the workgroups before
+// its own (bsum[0, blockIdx.x), summed by every workgroup itself), plus the
+// workgroup-local prefix of wsum.  Every workgroup also sums all nb of them:
+// new blocks past the pool's capacity are an error (ERR_POOL, raised by
+// workgroup 0), and then nothing moves.  The pool top itself advances in
+// k_ins_pending, after every workgroup here has read it.
 __global__ __launch_bounds__(RB) void k_ins_alloc(uint64_t *tbm, uint32_t nwords, const uint32_t *wsum,
-                                                  const uint64_t *bbase, VRec *vr, uint64_t *pool, const GraphCtl *gc)
+                                                  const uint64_t *bsum, uint32_t nb, VRec *vr, uint64_t *pool,
+                                                  uint64_t pool_cap, GraphCtl *gc)
 {
+    unsigned long long before = 0, all = 0;
+    for (uint32_t i = threadIdx.x; i < nb; i += RB) {
+        const unsigned long long v = bsum[i];
+        all += v;
+        if (i < blockIdx.x) before += v;
+    }
+    before = block_sum(before);  // (valid in thread 0)
+    all = block_sum(all);
+    __shared__ unsigned long long base_all[2];
+    const unsigned long long top = gc->pool_top;
+    if (threadIdx.x == 0) {
+        base_all[0] = top + before;
+        base_all[1] = all;
+        if (blockIdx.x == 0) {
+            gc->batch_new = all;
+            if (top + all > pool_cap) gc->err |= ERR_POOL;
+        }
+    }
+    __syncthreads();
+    const bool over = top + base_all[1] > pool_cap;
     const uint32_t w = blockIdx.x * RB + threadIdx.x;
     const uint32_t mine = w < nwords ? wsum[w] : 0u;
     uint64_t total;
-    uint64_t nb = bbase[blockIdx.x] + block_exclusive_scan(mine, &total);
+    uint64_t nb_ = base_all[0] + block_exclusive_scan(mine, &total);
     if (w >= nwords || !mine) return;
     uint64_t word = tbm[w];
     tbm[w] = 0;
-    if (gc->err & ERR_POOL) return;
+    if (over) return;
     while (word) {
         const uint32_t v = w * 64 + (uint32_t)__builtin_ctzll(word);
         word &= word - 1;
@@ -1202,18 +1196,20 @@ __global__ __launch_bounds__(RB) void k_ins_alloc(uint64_t *tbm, uint32_t nwords
         const uint64_t d = r->dbl;
         const uint32_t len = (uint32_t)d, cap = r->dcap;
         const uint64_t beg = d >> 32;
-        for (uint32_t j = 0; j < cap; ++j) pool[nb + j] = pool[beg + j];
-        r->dbl = (nb << 32) | len;
+        for (uint32_t j = 0; j < cap; ++j) pool[nb_ + j] = pool[beg + j];
+        r->dbl = (nb_ << 32) | len;
         r->dcap = grown_cap(cap, len);
-        nb += r->dcap;
+        nb_ += r->dcap;
     }
 }
 
-// Writes the copies that waited for their row's new block.
+// Writes the copies that waited for their row's new block; thread 0 advances
+// the pool top past the new blocks (k_ins_alloc has read it).
 __global__ __launch_bounds__(RB) void k_ins_pending(const int4 *rec, int64_t k, GView g, uint32_t *tslot,
-                                                    const GraphCtl *gc)
+                                                    GraphCtl *gc)
 {
     const int64_t t = (int64_t)blockIdx.x * RB + threadIdx.x;
+    if (t == 0 && !(gc->err & ERR_POOL)) gc->pool_top += gc->batch_new;
     uint32_t row, col, w;
     if (t >= 2 * k || !(tslot[t] & SLOT_PENDING) || !ins_copy(rec, t, row, col, w)) return;
     if (gc->err & ERR_POOL) return;  // reported to the host; nothing was placed
@@ -1941,9 +1937,9 @@ void launch_gather_tree(const uint64_t *st, uint64_t n, uint32_t kshift, uint32_
     hipLaunchKernelGGL(k_gather_tree, dim3(g), dim3(RB), 0, s, st, n, kshift, tree, out);
 }
 
-void launch_reduce_stats(Counters *c, unsigned long long *bstats, hipStream_t s)
+void launch_reduce_stats(Counters *c, unsigned long long *bstats, hipStream_t s, const GraphCtl *gc)
 {
-    hipLaunchKernelGGL(k_reduce_stats, dim3(REDUCE_WG), dim3(256), 0, s, c, bstats);
+    hipLaunchKernelGGL(k_reduce_stats, dim3(REDUCE_WG), dim3(256), 0, s, c, bstats, gc);
 }
 
 void launch_ins_place(const int4 *rec, int64_t k, GView g, uint32_t *tslot, uint64_t *tbm, hipStream_t s)
@@ -1958,21 +1954,16 @@ void launch_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const VRec *vr, ui
         hipLaunchKernelGGL(k_ins_wordsum, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, vr, wsum, bsum);
 }
 
-void launch_ins_scan(const uint64_t *bsum, uint32_t nwords, uint64_t *bbase, uint64_t pool_cap, GraphCtl *gc,
-                     hipStream_t s)
-{
-    hipLaunchKernelGGL(k_ins_scan, dim3(1), dim3(1024), 0, s, bsum, blocks_for(nwords), bbase, pool_cap, gc);
-}
 
-void launch_ins_alloc(uint64_t *tbm, uint32_t nwords, const uint32_t *wsum, const uint64_t *bbase, VRec *vr,
-                      uint64_t *pool, const GraphCtl *gc, hipStream_t s)
+void launch_ins_alloc(uint64_t *tbm, uint32_t nwords, const uint32_t *wsum, const uint64_t *bsum, VRec *vr,
+                      uint64_t *pool, uint64_t pool_cap, GraphCtl *gc, hipStream_t s)
 {
     if (nwords > 0)
-        hipLaunchKernelGGL(k_ins_alloc, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, wsum, bbase, vr, pool,
-                           gc);
+        hipLaunchKernelGGL(k_ins_alloc, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, wsum, bsum,
+                           blocks_for(nwords), vr, pool, pool_cap, gc);
 }
 
-void launch_ins_pending(const int4 *rec, int64_t k, GView g, uint32_t *tslot, const GraphCtl *gc, hipStream_t s)
+void launch_ins_pending(const int4 *rec, int64_t k, GView g, uint32_t *tslot, GraphCtl *gc, hipStream_t s)
 {
     if (k > 0) hipLaunchKernelGGL(k_ins_pending, dim3(blocks_for(2 * k)), dim3(RB), 0, s, rec, k, g, tslot, gc);
 }
