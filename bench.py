@@ -233,7 +233,7 @@ def full_sweep(fs, n, m_live):
 
 
 def line(args, value, wall, steps, cfg, stats, check, cpu, unit="changed edges/s", extra=None, traffic=False,
-         gather_rate=None):
+         gather_rate=None, roof_stats=None):
     import numpy as np
     lat = [s["t_update_ms"] for s in stats]
     alg = sum(s["alg_bytes"] for s in stats)
@@ -247,7 +247,7 @@ def line(args, value, wall, steps, cfg, stats, check, cpu, unit="changed edges/s
         "update_alg_GBps": alg / (sum(lat) / 1e3) / 1e9 if sum(lat) > 0 else None,
         "dist_bitexact": bool(check.get("gpu_recompute_equal", True) and check.get("full_sweep_fixed_point", True)
                               and check.get("dijkstra_mismatches", 0) == 0 and check.get("certificate", 0) == 0),
-        "check": check, "roofline": roofline(stats, traffic, gather_rate), "cpu_baseline": cpu,
+        "check": check, "roofline": roofline(roof_stats or stats, traffic, gather_rate), "cpu_baseline": cpu,
         # (null when the updates ran without the events between phases)
         "phases_ms": phases if any(phases.values()) else None,
         "rounds": {"invalidate": sum(s["inv_rounds"] for s in stats) / max(1, len(stats)),
@@ -411,17 +411,29 @@ def config3(args, ds, np):
     # every step starts from the same graph and tree (restored outside the timing)
     g = g0.clone()
     t = ds.Tree(g, parent=par0, dist=t0.read()[1])  # a tree on the clone; its state is restored per step
-    stats = []
+    # Each timed batch runs twice from the restored state: without launch
+    # events (the update's time, ~2000 rounds each) and with them (the
+    # relaxation kernel's duration for the roofline); both must end equal.
+    stats, tstats = [], []
+    same = True
     ds.set_tuning(phase_timing=1)  # a few us of events on updates of ~0.1 s
     for i, c in enumerate(batches):
-        g.copy_from(g0)
-        t.copy_from(t0)
-        bt = ds.Batch(g, c)
-        ds.set_tuning(kernel_timing=0 if args.no_timing or i < args.warmup else 1)
-        st = ds.update(g, t, bt)
-        bt.close()
-        if i >= args.warmup:
-            stats.append(st)
+        timed = i >= args.warmup
+        for with_events in ((False, True) if timed and not args.no_timing else (False,)):
+            g.copy_from(g0)
+            t.copy_from(t0)
+            bt = ds.Batch(g, c)
+            ds.set_tuning(kernel_timing=1 if with_events else 0)
+            st = ds.update(g, t, bt)
+            bt.close()
+            if not timed:
+                continue
+            if with_events:
+                tstats.append(st)
+                same &= bool(np.array_equal(np.stack(t.read()), first))
+            else:
+                stats.append(st)
+                first = np.stack(t.read())
     ds.set_tuning(kernel_timing=0, phase_timing=0)
     dev = sum(s["t_update_ms"] for s in stats) / 1e3
     value = k * args.steps / dev
@@ -432,10 +444,12 @@ def config3(args, ds, np):
     cpu = None
     if not args.no_cpu:
         check, cpu = cpu_check(ds, np, g, n, [0], par, dist, k, f"grid-{side}")
+    check["launch_event_rerun_equal"] = same
     line(args, value, dev, args.steps, {"workload": f"grid {side}x{side} drop 1% k=1e5 del 70% (half tree)",
                                           "n": n, "m": len(a), "k": k, "src": 0,
-                                          "note": "value = k / device update time; state restored between steps"},
-         stats, check, cpu)
+                                          "note": "value = k / device update time; state restored between steps; "
+                                                  "roofline from a rerun of each batch with launch events"},
+         stats, check, cpu, roof_stats=tstats or None)
 
 
 def config4a(args, ds, np):
