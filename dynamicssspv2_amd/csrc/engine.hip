@@ -1057,7 +1057,7 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
     return DSSSP_OK;
 }
 
-int validate_changes(int64_t n, const dsssp_change *chg, int64_t k, int64_t &ni, int64_t &nd, bool &zero_w)
+int validate_range(int64_t n, const dsssp_change *chg, int64_t k, int64_t &ni, int64_t &nd, bool &zero_w)
 {
     ni = nd = 0;
     zero_w = false;
@@ -1078,6 +1078,58 @@ int validate_changes(int64_t n, const dsssp_change *chg, int64_t k, int64_t &ni,
         }
     }
     return DSSSP_OK;
+}
+
+// The host-side check of a batch of records: counts of insertions and
+// deletions, whether an insertion has weight 0, and the first invalid record
+// (smallest index) as the error.  Large batches are checked by the host
+// threads in ranges; a range with an invalid record sends the whole batch
+// through the sequential check, which names the first one.
+int validate_changes(int64_t n, const dsssp_change *chg, int64_t k, int64_t &ni, int64_t &nd, bool &zero_w)
+{
+    const int T = host_threads();
+    if (k < (int64_t)1 << 16 || T <= 1) return validate_range(n, chg, k, ni, nd, zero_w);
+    std::vector<int64_t> cni((size_t)T, 0), cnd((size_t)T, 0);
+    std::vector<char> czw((size_t)T, 0), cbad((size_t)T, 0);
+    parallel_ranges(k, [&](int64_t b, int64_t e, int t) {
+        int64_t a = 0, d = 0;
+        bool z = false, bad = false;
+        for (int64_t i = b; i < e; ++i) {
+            const dsssp_change &c = chg[i];
+            if (c.op != 0 && c.op != 1) continue;
+            if (c.a < 0 || c.a >= n || c.b < 0 || c.b >= n || c.w < 0) {
+                bad = true;
+                break;
+            }
+            if (c.op == 1) {
+                ++a;
+                z |= c.w == 0;
+            } else {
+                ++d;
+            }
+        }
+        cni[(size_t)t] = a;
+        cnd[(size_t)t] = d;
+        czw[(size_t)t] = z;
+        cbad[(size_t)t] = bad;
+    }, T);
+    ni = nd = 0;
+    zero_w = false;
+    for (int t = 0; t < T; ++t) {
+        if (cbad[(size_t)t]) return validate_range(n, chg, k, ni, nd, zero_w);
+        ni += cni[(size_t)t];
+        nd += cnd[(size_t)t];
+        zero_w |= czw[(size_t)t] != 0;
+    }
+    return DSSSP_OK;
+}
+
+// memcpy of a large batch by the host threads, in ranges.
+void copy_records(dsssp_change *dst, const dsssp_change *src, int64_t k)
+{
+    parallel_ranges(k, [&](int64_t b, int64_t e, int) {
+        memcpy(dst + b, src + b, sizeof(dsssp_change) * (size_t)(e - b));
+    });
 }
 
 }  // namespace
@@ -1944,7 +1996,7 @@ extern "C" int dsssp_update(dsssp_graph *g, dsssp_tree *t, const dsssp_change *c
     DevGuard dg(g->device);
     if (int s = ensure_batch(g, k)) return s;
     if (int s = ensure_stage(g, k)) return s;
-    memcpy(g->stage, chg, sizeof(dsssp_change) * (size_t)k);
+    copy_records(g->stage, chg, k);
     HIP_TRY(hipEventRecord(g->ev[8], g->stream));
     HIP_TRY(hipMemcpyAsync(g->upd->rec, g->stage, sizeof(int4) * (size_t)k, hipMemcpyHostToDevice, g->stream));
     HIP_TRY(hipEventRecord(g->ev[9], g->stream));

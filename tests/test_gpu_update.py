@@ -222,6 +222,46 @@ def test_errors_leave_state_untouched():
     assert e.value.code == ds.ERR_TREE  # a root (parent itself) with nonzero distance
 
 
+def test_large_host_batches_checked_in_ranges():
+    """Host records of batches large enough for the threaded check and copy
+    (at least 2^16 records): a valid batch gives the tree of the same batch
+    uploaded first, and an invalid record (several, in different ranges) is
+    reported by the index of the first one with nothing changed."""
+    a, b, w = ds.gen_rmat(13, 16, seed=51)
+    n = 1 << 13
+    chg = ds.gen_changes(n, a, b, w, 90_000, 0.5, 1, 1000, seed=52)
+    assert len(chg) >= 1 << 16
+    g1, g2 = ds.Graph(n, a, b, w), ds.Graph(n, a, b, w)
+    t1, t2 = ds.Tree(g1, 0), ds.Tree(g2, 0)
+    try:
+        bt = ds.Batch(g1, chg)
+        s1 = ds.update(g1, t1, bt)
+        bt.close()
+        s2 = ds.update(g2, t2, chg)
+        assert (s1["n_insert"], s1["n_delete"]) == (s2["n_insert"], s2["n_delete"])
+        assert s1["n_insert"] + s1["n_delete"] == len(chg)
+        p1, d1 = t1.read()
+        p2, d2 = t2.read()
+        assert np.array_equal(p1, p2) and np.array_equal(d1, d2)
+        m = Mirror(n, a, b, w)
+        m.apply(chg)
+        check_tree(m, p2, d2, [0], "large host batch")
+        before = t2.read()
+        for first, code, rec in ((70_001, ds.ERR_RANGE, [3, n + 5, 7, 1]), (66_000, ds.ERR_WEIGHT, [3, 4, -7, 0])):
+            bad = np.array(chg, np.int32, copy=True)
+            bad[first] = rec
+            bad[first + 15_000] = [n, 1, 1, 0]  # a later invalid record, in another range
+            with pytest.raises(ds.DssspError) as e:
+                ds.update(g2, t2, bad)
+            assert e.value.code == code
+            assert f"change {first} " in str(e.value), str(e.value)
+        after = t2.read()
+        assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+    finally:
+        for o in (t1, t2, g1, g2):
+            o.close()
+
+
 def test_overflow_is_reported():
     n = 4
     g = ds.Graph(n, [0, 1, 2], [1, 2, 3], [2_000_000_000] * 3)
