@@ -1349,14 +1349,21 @@ __global__ __launch_bounds__(RB) void k_ins_offer(const int4 *rec, int64_t k, ui
         const uint32_t a = (uint32_t)r.x, b = (uint32_t)r.y;
         const uint32_t sa_slot = (a << kshift) | tree, sb_slot = (b << kshift) | tree;
         const uint32_t w = (uint32_t)r.z;
-        // Both states are loaded together with the checks below (one dependent
-        // step less); the copy in row a is tombstoned when a deletion of the
-        // batch removed it.
         const uint64_t sa = st[sa_slot], sb = st[sb_slot];
         const uint32_t ta = tslot[2 * ri];
-        if (!(ta & SLOT_PENDING) && !(g.pool[(g.vr[a].dbl >> 32) + ta] & E_DEAD)) {
-            const bool a_root = sa != ST_INF && st_parent(sa) == a;  // roots never change
-            const bool b_root = sb != ST_INF && st_parent(sb) == b;
+        const bool a_root = sa != ST_INF && st_parent(sa) == a;  // roots never change
+        const bool b_root = sb != ST_INF && st_parent(sb) == b;
+        // Whether the edge would lower an endpoint (or overflow): only then is
+        // it checked against the batch's deletions (the copy in row a is
+        // tombstoned when one removed it), which costs two more dependent
+        // random reads that most insertions, lowering nothing, skip.
+        auto acts = [&](uint64_t sf, uint64_t sto, uint32_t from, bool to_root) {
+            if (sf == ST_INF || to_root) return false;
+            const uint64_t nd = (uint64_t)st_dist(sf) + w;
+            return nd >= DIST_INF ? sto == ST_INF : pack_state((uint32_t)nd, from) < sto;
+        };
+        const bool want = acts(sa, sb, a, b_root) || acts(sb, sa, b, a_root);
+        if (want && !(ta & SLOT_PENDING) && !(g.pool[(g.vr[a].dbl >> 32) + ta] & E_DEAD)) {
             if (sa != ST_INF && !b_root) {
                 const uint64_t nd = (uint64_t)st_dist(sa) + w;
                 if (nd >= DIST_INF) {
