@@ -1124,12 +1124,41 @@ int validate_changes(int64_t n, const dsssp_change *chg, int64_t k, int64_t &ni,
     return DSSSP_OK;
 }
 
-// memcpy of a large batch by the host threads, in ranges.
-void copy_records(dsssp_change *dst, const dsssp_change *src, int64_t k)
+// validate_changes and a copy of the records to dst in one pass of the host
+// threads over a large batch (dst is left unspecified when a record is
+// invalid; the error is the sequential check's).
+int validate_and_copy(int64_t n, const dsssp_change *chg, int64_t k, dsssp_change *dst, int64_t &ni, int64_t &nd,
+                      bool &zero_w)
 {
-    parallel_ranges(k, [&](int64_t b, int64_t e, int) {
-        memcpy(dst + b, src + b, sizeof(dsssp_change) * (size_t)(e - b));
-    });
+    const int T = host_threads();
+    std::vector<int64_t> cni((size_t)T, 0), cnd((size_t)T, 0);
+    std::vector<char> czw((size_t)T, 0), cbad((size_t)T, 0);
+    parallel_ranges(k, [&](int64_t b, int64_t e, int t) {
+        int64_t a = 0, d = 0;
+        bool z = false, bad = false;
+        for (int64_t i = b; i < e; ++i) {
+            const dsssp_change c = chg[i];
+            dst[i] = c;
+            if (c.op != 0 && c.op != 1) continue;
+            bad |= c.a < 0 || c.a >= n || c.b < 0 || c.b >= n || c.w < 0;
+            a += c.op == 1;
+            d += c.op == 0;
+            z |= c.op == 1 && c.w == 0;
+        }
+        cni[(size_t)t] = a;
+        cnd[(size_t)t] = d;
+        czw[(size_t)t] = z;
+        cbad[(size_t)t] = bad;
+    }, T);
+    ni = nd = 0;
+    zero_w = false;
+    for (int t = 0; t < T; ++t) {
+        if (cbad[(size_t)t]) return validate_range(n, chg, k, ni, nd, zero_w);
+        ni += cni[(size_t)t];
+        nd += cnd[(size_t)t];
+        zero_w |= czw[(size_t)t] != 0;
+    }
+    return DSSSP_OK;
 }
 
 }  // namespace
@@ -1988,15 +2017,28 @@ extern "C" int dsssp_update(dsssp_graph *g, dsssp_tree *t, const dsssp_change *c
     if (t->g != g) return fail(DSSSP_ERR_ARG, "tree belongs to another graph");
     int64_t ni, nd;
     bool zw;
-    if (int s = validate_changes(g->n, chg, k, ni, nd, zw)) return s;
-    if (k == 0) {
-        if (stats) *stats = dsssp_stats{};
-        return DSSSP_OK;
+    // A large batch whose buffers are already big enough is checked while the
+    // host threads copy it to the pinned staging buffer; otherwise the check
+    // comes first (an invalid batch is reported before anything is allocated).
+    const bool fused = k >= (int64_t)1 << 16 && host_threads() > 1 && g->stage_cap >= k && g->upd &&
+                       g->upd->cap >= k;
+    if (!fused) {
+        if (int s = validate_changes(g->n, chg, k, ni, nd, zw)) return s;
+        if (k == 0) {
+            if (stats) *stats = dsssp_stats{};
+            return DSSSP_OK;
+        }
     }
     DevGuard dg(g->device);
-    if (int s = ensure_batch(g, k)) return s;
-    if (int s = ensure_stage(g, k)) return s;
-    copy_records(g->stage, chg, k);
+    if (fused) {
+        if (int s = validate_and_copy(g->n, chg, k, g->stage, ni, nd, zw)) return s;
+    } else {
+        if (int s = ensure_batch(g, k)) return s;
+        if (int s = ensure_stage(g, k)) return s;
+        parallel_ranges(k, [&](int64_t b, int64_t e, int) {
+            memcpy(g->stage + b, chg + b, sizeof(dsssp_change) * (size_t)(e - b));
+        });
+    }
     HIP_TRY(hipEventRecord(g->ev[8], g->stream));
     HIP_TRY(hipMemcpyAsync(g->upd->rec, g->stage, sizeof(int4) * (size_t)k, hipMemcpyHostToDevice, g->stream));
     HIP_TRY(hipEventRecord(g->ev[9], g->stream));
