@@ -11,8 +11,9 @@
 //          emap uint64[2^k]   edge map: per live directed entry, a 32-bit key
 //                             fingerprint and its position in its row
 //          tbm  uint64[n/64]  rows whose delta block must grow in this batch
-//          wsum/bsum          pool space per bitmap word / per 256 words, for
-//                             one batch
+//          wsum/bsum/bbase    pool space per bitmap word / per 256 words, and
+//                             (graphs past 2048 x 256 words) the pool offset of
+//                             each 256-word group, for one batch
 //   tree   st   uint64[S]     packed state (distance << 32 | parent), S = n * K'
 //                             slots (K' = trees rounded up to a power of two)
 //          fb   2 x uint64[S/64]  frontier bitmaps (ping-pong between rounds)

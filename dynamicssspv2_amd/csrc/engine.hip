@@ -165,6 +165,7 @@ struct dsssp_graph {
     uint64_t emap_used = 0;    // entries entered since the map was built (dead ones too)
     uint32_t *wsum = nullptr;   // per bitmap word: pool entries needed by the batch
     uint64_t *bsum = nullptr;   // per 256 words: sum of wsum
+    uint64_t *bbase = nullptr;  // per 256 words: pool offset (only past INS_FOLD_GROUPS groups)
     GraphCtl *gc = nullptr;
     GraphCtl *gc_host = nullptr;  // pinned
     bool zero_w = false;          // some weight was ever 0: parents need the tight-BFS repair
@@ -930,7 +931,7 @@ int update_impl(dsssp_graph *g, dsssp_tree *t, dsssp_batch *b, dsssp_stats *stat
     if (b->n_ins > 0) {
         launch_ins_place(b->rec, b->k, g->view(), b->tslot, g->tbm, s);
         launch_ins_wordsum(g->tbm, g->nw, g->vr, g->wsum, g->bsum, s);
-        launch_ins_alloc(g->tbm, g->nw, g->wsum, g->bsum, g->vr, g->pool, g->pool_cap, g->gc, s);
+        launch_ins_alloc(g->tbm, g->nw, g->wsum, g->bsum, g->bbase, g->vr, g->pool, g->pool_cap, g->gc, s);
         launch_ins_pending(b->rec, b->k, g->view(), b->tslot, g->gc, s);
         g->emap_used += 2 * (uint64_t)b->n_ins;
         if (b->zero_w) g->zero_w = true;
@@ -1259,6 +1260,7 @@ static void graph_release(dsssp_graph *g)
     if (g->stage) (void)hipHostFree(g->stage);
     dev_free(g->wsum);
     dev_free(g->bsum);
+    dev_free(g->bbase);
     dev_free(g->gc);
     dev_free(g->mkeys);
     dev_free(g->mreq);
@@ -1284,6 +1286,8 @@ static int graph_alloc(dsssp_graph *g, uint64_t pool_cap)
     if (int s = dev_alloc((void **)&g->wsum, sizeof(uint32_t) * g->nw, "word sizes")) return s;
     const size_t nb = (g->nw + 255) / 256;
     if (int s = dev_alloc((void **)&g->bsum, sizeof(uint64_t) * nb, "group sizes")) return s;
+    if (nb > INS_FOLD_GROUPS)
+        if (int s = dev_alloc((void **)&g->bbase, sizeof(uint64_t) * nb, "group bases")) return s;
     if (int s = dev_alloc((void **)&g->gc, sizeof(GraphCtl), "graph control")) return s;
     g->pool_cap = pool_cap;
     if (int s = dev_alloc((void **)&g->pool, sizeof(uint64_t) * pool_cap, "delta pool")) return s;

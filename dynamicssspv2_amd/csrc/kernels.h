@@ -131,8 +131,12 @@ constexpr uint32_t SLOT_PENDING = 0x80000000u;
 void launch_ins_place(const int4 *rec, int64_t k, GView g, uint32_t *tslot, uint64_t *tbm, hipStream_t s);
 void launch_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const VRec *vr, uint32_t *wsum, uint64_t *bsum,
                         hipStream_t s);
-void launch_ins_alloc(uint64_t *tbm, uint32_t nwords, const uint32_t *wsum, const uint64_t *bsum, VRec *vr,
-                      uint64_t *pool, uint64_t pool_cap, GraphCtl *gc, hipStream_t s);
+// Up to this many 256-word groups, each workgroup of the block moves sums the
+// group sums before its own (no scan launch); beyond it a single-workgroup scan
+// into bbase (nwords / 256 entries) runs first, so the work stays linear.
+constexpr uint32_t INS_FOLD_GROUPS = 2048;
+void launch_ins_alloc(uint64_t *tbm, uint32_t nwords, const uint32_t *wsum, const uint64_t *bsum, uint64_t *bbase,
+                      VRec *vr, uint64_t *pool, uint64_t pool_cap, GraphCtl *gc, hipStream_t s);
 void launch_ins_pending(const int4 *rec, int64_t k, GView g, uint32_t *tslot, GraphCtl *gc, hipStream_t s);
 // Deletions: the tree-edge test (marks the roots of invalidated subtrees) and
 // the tombstoning of every copy of each deleted entry.

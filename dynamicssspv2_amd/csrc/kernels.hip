@@ -1149,38 +1149,79 @@ __global__ __launch_bounds__(RB) void k_ins_wordsum(const uint64_t *tbm, uint32_
     if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
+// For graphs of more than INS_FOLD_GROUPS workgroup sums: a single-workgroup
+// exclusive scan of them into pool offsets bbase (from the current pool top);
+// the total goes to gc->batch_new (the pool top advances in k_ins_pending).
+__global__ __launch_bounds__(1024) void k_ins_scan(const uint64_t *bsum, uint32_t nb, uint64_t *bbase,
+                                                   uint64_t pool_cap, GraphCtl *gc)
+{
+    __shared__ unsigned long long part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nb + 1023) / 1024;
+    const uint32_t lo = min(t * per, nb), hi = min(lo + per, nb);
+    unsigned long long s = 0;
+    for (uint32_t i = lo; i < hi; ++i) s += bsum[i];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {
+        const unsigned long long y = t >= d ? part[t - d] : 0ull;
+        __syncthreads();
+        part[t] += y;
+        __syncthreads();
+    }
+    const unsigned long long top = gc->pool_top;
+    unsigned long long run = top + part[t] - s;
+    for (uint32_t i = lo; i < hi; ++i) {
+        bbase[i] = run;
+        run += bsum[i];
+    }
+    if (t == 1023) {
+        const unsigned long long total = part[1023];
+        gc->batch_new = total;
+        if (top + total > pool_cap) gc->err |= ERR_POOL;
+    }
+}
+
 // Moves each marked row's block (its entries written so far) to a new block;
 // the length already counts the pending copies.  Clears the marks.  Word w's
-// new blocks start at the pool top, plus the sums of the workgroups before
-// its own (bsum[0, blockIdx.x), summed by every workgroup itself), plus the
-// workgroup-local prefix of wsum.  Every workgroup also sums all nb of them:
-// new blocks past the pool's capacity are an error (ERR_POOL, raised by
-// workgroup 0), and then nothing moves.  The pool top itself advances in
-// k_ins_pending, after every workgroup here has read it.
+// new blocks start at its workgroup's base plus the workgroup-local prefix of
+// wsum.  The base is bbase[blockIdx.x] when k_ins_scan ran; otherwise (at most
+// INS_FOLD_GROUPS workgroups) every workgroup takes the pool top plus the sums
+// of the workgroups before its own, and sums all nb of them: new blocks past
+// the pool's capacity are an error (ERR_POOL, raised by workgroup 0), and then
+// nothing moves.  The pool top itself advances in k_ins_pending, after every
+// workgroup here has read it.
 __global__ __launch_bounds__(RB) void k_ins_alloc(uint64_t *tbm, uint32_t nwords, const uint32_t *wsum,
-                                                  const uint64_t *bsum, uint32_t nb, VRec *vr, uint64_t *pool,
-                                                  uint64_t pool_cap, GraphCtl *gc)
+                                                  const uint64_t *bsum, uint32_t nb, const uint64_t *bbase,
+                                                  VRec *vr, uint64_t *pool, uint64_t pool_cap, GraphCtl *gc)
 {
-    unsigned long long before = 0, all = 0;
-    for (uint32_t i = threadIdx.x; i < nb; i += RB) {
-        const unsigned long long v = bsum[i];
-        all += v;
-        if (i < blockIdx.x) before += v;
-    }
-    before = block_sum(before);  // (valid in thread 0)
-    all = block_sum(all);
     __shared__ unsigned long long base_all[2];
-    const unsigned long long top = gc->pool_top;
-    if (threadIdx.x == 0) {
-        base_all[0] = top + before;
-        base_all[1] = all;
-        if (blockIdx.x == 0) {
-            gc->batch_new = all;
-            if (top + all > pool_cap) gc->err |= ERR_POOL;
+    bool over;
+    if (bbase) {
+        if (threadIdx.x == 0) base_all[0] = bbase[blockIdx.x];
+        __syncthreads();
+        over = (gc->err & ERR_POOL) != 0;
+    } else {
+        unsigned long long before = 0, all = 0;
+        for (uint32_t i = threadIdx.x; i < nb; i += RB) {
+            const unsigned long long v = bsum[i];
+            all += v;
+            if (i < blockIdx.x) before += v;
         }
+        before = block_sum(before);  // (valid in thread 0)
+        all = block_sum(all);
+        const unsigned long long top = gc->pool_top;
+        if (threadIdx.x == 0) {
+            base_all[0] = top + before;
+            base_all[1] = all;
+            if (blockIdx.x == 0) {
+                gc->batch_new = all;
+                if (top + all > pool_cap) gc->err |= ERR_POOL;
+            }
+        }
+        __syncthreads();
+        over = top + base_all[1] > pool_cap;
     }
-    __syncthreads();
-    const bool over = top + base_all[1] > pool_cap;
     const uint32_t w = blockIdx.x * RB + threadIdx.x;
     const uint32_t mine = w < nwords ? wsum[w] : 0u;
     uint64_t total;
@@ -1962,12 +2003,15 @@ void launch_ins_wordsum(const uint64_t *tbm, uint32_t nwords, const VRec *vr, ui
 }
 
 
-void launch_ins_alloc(uint64_t *tbm, uint32_t nwords, const uint32_t *wsum, const uint64_t *bsum, VRec *vr,
-                      uint64_t *pool, uint64_t pool_cap, GraphCtl *gc, hipStream_t s)
+void launch_ins_alloc(uint64_t *tbm, uint32_t nwords, const uint32_t *wsum, const uint64_t *bsum, uint64_t *bbase,
+                      VRec *vr, uint64_t *pool, uint64_t pool_cap, GraphCtl *gc, hipStream_t s)
 {
-    if (nwords > 0)
-        hipLaunchKernelGGL(k_ins_alloc, dim3(blocks_for(nwords)), dim3(RB), 0, s, tbm, nwords, wsum, bsum,
-                           blocks_for(nwords), vr, pool, pool_cap, gc);
+    if (nwords == 0) return;
+    const uint32_t nb = blocks_for(nwords);
+    const bool scan = nb > INS_FOLD_GROUPS;
+    if (scan) hipLaunchKernelGGL(k_ins_scan, dim3(1), dim3(1024), 0, s, bsum, nb, bbase, pool_cap, gc);
+    hipLaunchKernelGGL(k_ins_alloc, dim3(nb), dim3(RB), 0, s, tbm, nwords, wsum, bsum, nb,
+                       scan ? (const uint64_t *)bbase : nullptr, vr, pool, pool_cap, gc);
 }
 
 void launch_ins_pending(const int4 *rec, int64_t k, GView g, uint32_t *tslot, GraphCtl *gc, hipStream_t s)

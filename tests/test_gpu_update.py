@@ -262,6 +262,48 @@ def test_large_host_batches_checked_in_ranges():
             o.close()
 
 
+def test_delta_growth_past_the_folded_group_count():
+    """A graph of more than 2048 x 256 bitmap words (33.5 M vertices): the
+    pool offsets of grown delta blocks come from the scan launch, not from
+    each workgroup's own sum of the group sizes.  Batches that grow thousands
+    of rows at the top of the id range leave the graph and the tree equal to
+    the oracle's."""
+    n = (1 << 25) + (1 << 21)
+    rng = np.random.default_rng(61)
+    core = np.concatenate([np.arange(100_000), np.arange(n - 100_000, n)]).astype(np.int64)
+    par = core[(rng.random(len(core) - 1) * np.arange(1, len(core))).astype(np.int64)]  # a random tree
+    extra = rng.integers(0, len(core), (100_000, 2))
+    a = np.concatenate([par, core[extra[:, 0]]]).astype(np.int32)
+    b = np.concatenate([core[1:], core[extra[:, 1]]]).astype(np.int32)
+    keep = a != b
+    a, b = a[keep], b[keep]
+    w = rng.integers(1, 101, len(a)).astype(np.int32)
+    say(f"big-n graph n={n} m={len(a)}")
+    g = ds.Graph(n, a, b, w)
+    t = ds.Tree(g, 0)
+    m = Mirror(n, a, b, w)
+    try:
+        for step in range(2):
+            hot = core[len(core) - 5_000:]  # high ids: rows in bitmap groups past 2048
+            ins_a = np.repeat(hot, 12)
+            ins_b = core[rng.integers(0, len(core), len(ins_a))]
+            ok = ins_a != ins_b
+            ins = np.stack([ins_a[ok], ins_b[ok], rng.integers(1, 101, int(ok.sum())), np.ones(int(ok.sum()))], 1)
+            cur = np.stack([m.a, m.b, m.w], 1)
+            dele = cur[rng.choice(len(cur), 20_000, replace=False)]
+            dele = np.concatenate([dele, np.zeros((len(dele), 1))], 1)
+            chg = np.concatenate([ins, dele]).astype(np.int32)
+            chg = chg[rng.permutation(len(chg))]
+            say(f"big-n batch {step}: k={len(chg)}")
+            ds.update(g, t, chg)
+            m.apply(chg)
+            assert np.array_equal(edge_multiset(*g.export()), edge_multiset(m.a, m.b, m.w))
+            check_tree(m, *t.read(), [0], f"big-n step {step}")
+    finally:
+        t.close()
+        g.close()
+
+
 def test_overflow_is_reported():
     n = 4
     g = ds.Graph(n, [0, 1, 2], [1, 2, 3], [2_000_000_000] * 3)
