@@ -111,6 +111,13 @@ __device__ __forceinline__ uint32_t emap_kill(const GView &g, uint32_t row, uint
     const uint32_t blen = g.vr[row].blen;
     const uint64_t d = g.vr[row].dbl;
     const uint32_t dl = (uint32_t)d;
+    // The row's first delta entries (never in the map, so no kill of the probe
+    // below touches them) are loaded before the probe starts.
+    const uint32_t ns = min(dl, EMAP_DSKIP);
+    uint64_t *dp = &g.pool[d >> 32];
+    uint64_t de[EMAP_DSKIP];
+#pragma unroll
+    for (uint32_t i = 0; i < EMAP_DSKIP; ++i) de[i] = i < ns ? dp[i] : E_DEAD;
     uint32_t probes = 0;
     for (;;) {
         ++probes;
@@ -127,15 +134,12 @@ __device__ __forceinline__ uint32_t emap_kill(const GView &g, uint32_t row, uint
         s = (s + 1) & g.emask;
         v = g.emap[s];
     }
-    const uint32_t ns = min(dl, EMAP_DSKIP);
-    uint64_t *dp = &g.pool[d >> 32];
-    for (uint32_t i = 0; i < ns; ++i) {
-        const uint64_t e = dp[i];
-        if (e_key(e) == key && !(e & E_DEAD)) {
-            dp[i] = e | E_DEAD;
+#pragma unroll
+    for (uint32_t i = 0; i < EMAP_DSKIP; ++i)
+        if (!(de[i] & E_DEAD) && e_key(de[i]) == key) {  // (entries past ns read as dead)
+            dp[i] = de[i] | E_DEAD;
             ++kills;
         }
-    }
     return probes + ns;
 }
 
