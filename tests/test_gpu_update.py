@@ -222,11 +222,11 @@ def test_errors_leave_state_untouched():
     assert e.value.code == ds.ERR_TREE  # a root (parent itself) with nonzero distance
 
 
-def test_large_host_batches_checked_while_staged():
-    """Host records of a large batch (the check rides on the copy to the
-    pinned staging buffer once the buffers are big enough): a valid batch
-    gives the tree of the same batch uploaded first, and of several invalid
-    records the first one is reported, by its index, with nothing changed."""
+def test_large_host_batches_from_host_memory():
+    """Host records of a large batch (checked on the host, staged in pinned
+    memory, copied to the device): a valid batch gives the tree of the same
+    batch uploaded first, and of several invalid records the first one is
+    reported, by its index, with nothing changed."""
     a, b, w = ds.gen_rmat(13, 16, seed=51)
     n = 1 << 13
     chg = ds.gen_changes(n, a, b, w, 90_000, 0.5, 1, 1000, seed=52)
