@@ -838,10 +838,10 @@ __device__ void drain(const FArgs &A, WaveSmem &sm, WaveStats &S, uint32_t &top)
 // Least waves per SIMD, which caps the register budget: 5 (up to 102
 // registers; relaxation takes 96 and spills 4 values instead of 20) for the
 // frontier modes, 6 (up to 80) for the full edge pass, whose grid is sized for
-// 6 per SIMD, and for the 64-tree layout.
-// Measured on the scale-24 headline (MI355X): relaxation 0.327 -> 0.320 ms per
-// update at 5, the edge pass 3.64 -> 4.9 ms at 5 with the same grid.
-constexpr int frontier_waves(int mode, bool forest) { return mode == M_SWEEP || forest ? 6 : 5; }
+// 6 per SIMD.  Measured on MI355X: relaxation 0.327 -> 0.320 ms per update of
+// the scale-24 headline at 5, the 64-tree forest 119.5 -> 115.8 ms per batch,
+// the edge pass 3.64 -> 4.9 ms at 5 with the same grid.
+constexpr int frontier_waves(int mode, bool) { return mode == M_SWEEP ? 6 : 5; }
 
 template <int MODE, int EPT, bool FOREST, int FB>
 __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(frontier_waves(MODE, FOREST)))) void k_frontier(FArgs A)
