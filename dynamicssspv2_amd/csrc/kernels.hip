@@ -849,6 +849,31 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(frontier_wav
     constexpr int FW = FB / 64;  // waves per workgroup
     __shared__ WaveSmem smem[FW];
     const uint32_t lane = lane_id();
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t gw = blockIdx.x * FW + wv;
+    const uint32_t nw = gridDim.x * FW;
+    // The bitmap is read in blocks of 8 words (64 bytes); in one pass a wave
+    // reads 8 blocks nw apart, so that a cluster of non-empty words (frontiers
+    // cluster by vertex id) spreads over many waves.  The set slots of the 64
+    // words are then taken 64 at a time, in word order.
+    const uint32_t nblk = A.tiles ? (A.nwords + 7) / 8 : 0u;
+    // The first pass's words, the item count and the input flag are loaded
+    // before this launch stores anything: a store in between (none of them to
+    // these addresses) would otherwise hold the later loads back, one memory
+    // latency each.
+    uint32_t wi = 0;
+    bool in0 = false;
+    uint64_t raw0 = 0;
+    if (nblk) {
+        const uint32_t blk = (lane >> 3) * nw + gw;
+        wi = blk * 8 + (lane & 7);
+        in0 = blk < nblk && wi < A.nwords;
+        if (in0 && !A.identity) raw0 = A.in_bm[wi];
+    }
+    // (without a count or flag, the launch's error word stands in for the
+    // load, so that both loads issue unconditionally, together)
+    const uint32_t hraw = *(A.in_hcnt ? A.in_hcnt : &A.c->err);
+    const uint32_t act_in = *(A.in_act ? A.in_act : &A.c->err);
     if (blockIdx.x == 0 && threadIdx.x < 64) {
         if (threadIdx.x == 0) {
             if (A.zero_act) *A.zero_act = 0;
@@ -890,15 +915,7 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(frontier_wav
             if (sweep && blockIdx.x == 0 && threadIdx.x == 0) add_stat(A.bstats, ST_SWEEPS, 1);
         }
     }
-    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t gw = blockIdx.x * FW + wv;
-    const uint32_t nw = gridDim.x * FW;
     WaveSmem &sm = smem[wv];
-    // The bitmap is read in blocks of 8 words (64 bytes); in one pass a wave
-    // reads 8 blocks nw apart, so that a cluster of non-empty words (frontiers
-    // cluster by vertex id) spreads over many waves.  The set slots of the 64
-    // words are then taken 64 at a time, in word order.
-    const uint32_t nblk = A.tiles ? (A.nwords + 7) / 8 : 0u;
     auto load_word = [&](uint32_t pb, uint32_t &wi) -> uint64_t {
         const uint32_t blk = pb + (lane >> 3) * nw + gw;
         wi = blk * 8 + (lane & 7);
@@ -915,14 +932,23 @@ __global__ __launch_bounds__(FB) __attribute__((amdgpu_waves_per_eu(frontier_wav
         }
         return word;
     };
-    // The first pass's words are loaded alongside the item count.
-    uint32_t wi = 0;
-    uint64_t word = nblk ? load_word(0, wi) : 0ull;
+    // The first pass's words (loaded above), as load_word takes them.
+    uint64_t word = 0;
+    if (in0) {
+        if (A.identity || sweep) {
+            const uint32_t lo = wi * 64;
+            word = A.nslots - lo >= 64 ? ~0ull : ((1ull << (A.nslots - lo)) - 1ull);
+            if (sweep && (A.identity ? A.in_bm[wi] : raw0)) A.in_bm[wi] = 0;  // consumed all the same
+        } else {
+            word = raw0;
+            if (word && A.consume) A.in_bm[wi] = 0;
+        }
+    }
     // Items left by the previous launch; count clamped to the list capacity.
-    const uint32_t H = A.in_hcnt ? min(*A.in_hcnt, A.hcap) : 0u;
+    const uint32_t H = A.in_hcnt ? min(hraw, A.hcap) : 0u;
     // A round after convergence has an empty bitmap and no items: it returns
     // without scanning further.
-    if (H == 0 && A.in_act && *A.in_act == 0) return;
+    if (H == 0 && A.in_act && act_in == 0) return;
     if (A.count_round && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&A.c->worked, 1u);
     uint32_t top = 0;  // slots on this wave's stack (wave-uniform)
     for (uint32_t pb = 0; pb < nblk; pb += nw * 8) {
